@@ -1,0 +1,280 @@
+/*
+ * xrtk.h — C ABI of the MI355X (gfx950) Kirchhoff diffraction engine.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never lets
+ * an exception cross the boundary, and returns an int status (XRTK_OK == 0).
+ * The caller owns every input and output buffer; inputs are borrowed for the
+ * duration of the call only.  Ray sets uploaded with xrtk_rayset_* live in the
+ * library until released.
+ *
+ * Reference interfaces replaced (physwkim/xrt-rs):
+ *   GpuRay / GpuPixel layouts ......... src/gpu/kirchhoff.rs:19-45
+ *   kirchhoff_gpu(ctx, rays, pixels) .. src/gpu/kirchhoff.rs:77-83
+ *   kirchhoff_auto(rays, pixels) ...... src/gpu/fallback.rs:19-26
+ *   GpuContext::new / ::shared ........ src/gpu/context.rs:20-31
+ *   diffraction_integral(rays, pixels)  src/waves/diffraction.rs:56-64
+ *   prepare_wave(WaveParams) .......... src/waves/prepare_wave.rs:40-68
+ *   CHBAR ............................. src/core/consts.rs:74
+ */
+#ifndef XRTK_H
+#define XRTK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define XRTK_OK        0
+#define XRTK_ENODEV    1  /* no usable gfx950 device                        */
+#define XRTK_EINVAL    2  /* NULL pointer, bad length or bad option         */
+#define XRTK_ENOMEM    3  /* device allocation failed or HBM budget exceeded */
+#define XRTK_ELAUNCH   4  /* a HIP launch or copy failed                     */
+#define XRTK_EAGAIN    5  /* serving: result not ready yet                   */
+#define XRTK_ETIMEDOUT 6  /* serving: wait timed out                         */
+#define XRTK_ESHUTDOWN 7  /* serving: server is shutting down                */
+
+/* Physical constant used for k = E / CHBAR * 1e7 [mm^-1] (eV * Angstrom).
+ * Same value as src/core/consts.rs:74. */
+#define XRTK_CHBAR 1973.2697177417986
+
+/* ---- data layouts ------------------------------------------------------ */
+
+/* Byte-identical to GpuRay (src/gpu/kirchhoff.rs:19-35), 48 bytes. */
+typedef struct XrtkGpuRay {
+    float x, y, z;        /* position on the optical element [mm] */
+    float nx, ny, nz;     /* surface normal                       */
+    float nl;             /* obliquity factor n . beam direction  */
+    float energy;         /* [eV]                                 */
+    float es_re, es_im;   /* s-polarised amplitude                */
+    float ep_re, ep_im;   /* p-polarised amplitude                */
+} XrtkGpuRay;
+
+/* Byte-identical to GpuPixel (src/gpu/kirchhoff.rs:37-45), 16 bytes. */
+typedef struct XrtkGpuPixel {
+    float x, y, z, _pad;
+} XrtkGpuPixel;
+
+/* f64 rays as twelve structure-of-arrays columns, the field order of
+ * DiffractionRay (src/waves/diffraction.rs:15-33) and of the Python binding
+ * diffraction_integral_rs (crates/xrt-python/src/lib.rs:321-338).  Any
+ * column may be shared between fields (for instance one zero column). */
+typedef struct XrtkRaysF64 {
+    const double *x, *y, *z;
+    const double *nx, *ny, *nz;
+    const double *nl;
+    const double *es_re, *es_im;
+    const double *ep_re, *ep_im;
+    const double *energy;
+} XrtkRaysF64;
+
+/* Regular screen exactly as prepare_wave builds it
+ * (src/waves/prepare_wave.rs:40-68): pixel (ix, iz) lies at
+ * x = cx + (ix/(nx-1) - 0.5)*2*dx, y = cy, z = cz + (iz/(nz-1) - 0.5)*2*dz,
+ * index ix*nz + iz; a dimension of 1 maps to the centre. */
+typedef struct XrtkWaveParams {
+    double center[3];
+    double dx, dz;
+    uint32_t nx, nz;
+} XrtkWaveParams;
+
+/* ---- options ----------------------------------------------------------- */
+
+/* Arithmetic used on the device. */
+#define XRTK_MODE_F32R 0  /* FP32 phase-reduced pair kernel, FP64 centroid,
+                             base phase and merge (the default)            */
+#define XRTK_MODE_F64  1  /* plain FP64 pair kernel with Kahan sums        */
+
+/* Where the pair kernel reads ray records from. */
+#define XRTK_RAYSRC_AUTO 0
+#define XRTK_RAYSRC_SMEM 1  /* scalar loads into SGPRs (wave-uniform)      */
+#define XRTK_RAYSRC_LDS  2  /* workgroup stages a ray tile in LDS          */
+
+typedef struct XrtkOpts {
+    int32_t mode;             /* XRTK_MODE_*                                */
+    int32_t pixels_per_lane;  /* 0 = autotune, else 1, 2, 4 or 8            */
+    int32_t waves_per_group;  /* 0 = autotune, else 1, 2 or 4               */
+    int32_t tile_rays;        /* 0 = autotune: rays per FP32 partial sum    */
+    int32_t split;            /* 0 = autotune: ray chunks per pixel block   */
+    int32_t ray_src;          /* XRTK_RAYSRC_*                              */
+    int32_t persistent;       /* -1 = autotune, 0 = one group per item,
+                                 1 = persistent groups pulling a queue       */
+    int32_t reserved;
+} XrtkOpts;
+
+/* Fills *o with the defaults (mode F32R, everything autotuned). */
+void xrtk_opts_default(XrtkOpts *o);
+
+/* ---- context (replaces GpuContext, src/gpu/context.rs:6-31) ------------ */
+
+typedef struct xrtk_ctx xrtk_ctx;
+
+/* Process-wide context, created on first use, thread-safe.  NULL when no
+ * gfx950 device is present (the "is there a GPU?" answer kirchhoff_auto
+ * uses, src/gpu/context.rs:24-31).  Never destroy it. */
+xrtk_ctx *xrtk_ctx_shared(void);
+
+/* Private context on device `device` (GpuContext::new, context.rs:20-22). */
+int xrtk_ctx_create(int device, xrtk_ctx **out);
+void xrtk_ctx_destroy(xrtk_ctx *ctx);
+
+/* Adapter name (GpuContext::adapter_name, context.rs:70-72). */
+int xrtk_ctx_device_name(const xrtk_ctx *ctx, char *buf, size_t len);
+
+/* ---- one-shot solves --------------------------------------------------- */
+
+/* kirchhoff_gpu (src/gpu/kirchhoff.rs:77-298): f32 rays and pixels in,
+ * np x {es_re, es_im, ep_re, ep_im} f64 out (the memory order of
+ * Vec<(Complex64, Complex64)>).  Pairs with path < 1e-20 mm are skipped as
+ * in kirchhoff.wgsl:85-87.  0 rays gives zeros; 0 pixels writes nothing.
+ * opts may be NULL. */
+int xrtk_kirchhoff_gpu_f32(xrtk_ctx *ctx,
+                           const XrtkGpuRay *rays, size_t n_rays,
+                           const XrtkGpuPixel *pixels, size_t n_pixels,
+                           double *out4, const XrtkOpts *opts);
+
+/* kirchhoff_auto (src/gpu/fallback.rs:19-26): the shared context when a
+ * device exists, else the f64 CPU loop of fallback.rs:29-71 (no Kahan, skip
+ * below 1e-20 mm).  Setting XRTK_REQUIRE_GPU=1 in the environment turns the
+ * CPU branch into XRTK_ENODEV.  Never fails otherwise. */
+int xrtk_kirchhoff_auto_f32(const XrtkGpuRay *rays, size_t n_rays,
+                            const XrtkGpuPixel *pixels, size_t n_pixels,
+                            double *out4);
+
+/* diffraction_integral (src/waves/diffraction.rs:56-117) on the device:
+ * f64 rays (SoA) and pixel coordinates in, np x 4 f64 out, pairs with
+ * path < 1e-30 mm skipped.  opts->mode picks FP32-reduced or plain FP64. */
+int xrtk_diffraction_f64(xrtk_ctx *ctx, const XrtkRaysF64 *rays, size_t n_rays,
+                         const double *pix_x, const double *pix_y,
+                         const double *pix_z, size_t n_pixels,
+                         double *out4, const XrtkOpts *opts);
+
+/* Same, with the screen generated on the device from a prepare_wave grid
+ * (nx*nz pixels, output in prepare_wave's order). */
+int xrtk_diffraction_grid_f64(xrtk_ctx *ctx, const XrtkRaysF64 *rays,
+                              size_t n_rays, const XrtkWaveParams *screen,
+                              double *out4, const XrtkOpts *opts);
+
+/* prepare_wave (src/waves/prepare_wave.rs:40-68) on the host: writes
+ * nx*nz pixel coordinates into px, py, pz. */
+int xrtk_prepare_wave(const XrtkWaveParams *w, double *px, double *py,
+                      double *pz);
+
+/* beam_to_diffraction_rays (src/waves/prepare_wave.rs:78-111): gathers rays
+ * good[0..n_good) from a Beam's SoA columns with normals n (3 per beam ray),
+ * computing nl = n . (a, b, c).  es/ep columns may be NULL (then Es = 1,
+ * Ep = 0, prepare_wave.rs:89-95).  The 12 output columns hold n_good each. */
+int xrtk_beam_to_rays(const double *bx, const double *by, const double *bz,
+                      const double *ba, const double *bb, const double *bc,
+                      const double *be, const double *es_re,
+                      const double *es_im, const double *ep_re,
+                      const double *ep_im, size_t n_beam,
+                      const double *normals3, const size_t *good,
+                      size_t n_good, double *const out_cols[12]);
+
+/* ---- resident ray sets and batched solves ------------------------------ */
+
+typedef uint64_t xrtk_rayset;
+
+/* Copies a ray set to HBM once and computes its FP64 centroid there.
+ * path_up (may be NULL) is an upstream optical path per ray [mm]: each solve
+ * multiplies Es and Ep by exp(i k path_up) in FP64 on the device, the term
+ * beam_to_diffraction_rays leaves to the caller (prepare_wave.rs:78-111). */
+int xrtk_rayset_upload(xrtk_ctx *ctx, const XrtkRaysF64 *rays,
+                       const double *path_up, size_t n_rays,
+                       xrtk_rayset *out);
+int xrtk_rayset_release(xrtk_ctx *ctx, xrtk_rayset rs);
+/* Bytes of HBM the ray set holds. */
+int xrtk_rayset_bytes(xrtk_ctx *ctx, xrtk_rayset rs, size_t *bytes);
+
+/* One diffraction instance of a batch: a resident ray set at one energy
+ * (overrides the rays' own energy column when energy > 0) on a
+ * prepare_wave screen.  out4 receives nx*nz x 4 f64. */
+typedef struct XrtkJob {
+    xrtk_rayset rays;
+    double energy;
+    XrtkWaveParams screen;
+    double *out4;
+} XrtkJob;
+
+/* Runs every job of the batch in one persistent launch of the pair kernel
+ * (captured with its prep and merge kernels in one hipGraph, replayed while
+ * the batch shape repeats).  Each job's result equals its standalone
+ * xrtk_solve_jobs() result bit for bit. */
+int xrtk_solve_jobs(xrtk_ctx *ctx, const XrtkJob *jobs, size_t n_jobs,
+                    const XrtkOpts *opts);
+
+/* Device time in ms of the last pair-kernel launch of this context (HIP
+ * events around the launch on the context's stream) and of the whole last
+ * solve (prep + pair + merge, excluding host copies). */
+int xrtk_last_timing(xrtk_ctx *ctx, double *pair_ms, double *solve_ms);
+
+/* The plan the last solve used. */
+typedef struct XrtkPlan {
+    int32_t mode, pixels_per_lane, waves_per_group, tile_rays;
+    int32_t split, ray_src, persistent, n_items;
+    int32_t checked;    /* 1 if the near-singular (skip-test) path ran     */
+    int32_t backend;    /* 1 = device, 0 = CPU fallback                     */
+} XrtkPlan;
+int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);
+/* Backend used by the last xrtk_kirchhoff_auto_f32 call of this process:
+ * 1 = device, 0 = CPU fallback, -1 = none yet. */
+int xrtk_auto_last_backend(void);
+
+/* ---- autotuner ---------------------------------------------------------- */
+
+/* Times the candidate plans for an (n_rays, n_pixels) bucket on the device
+ * and records the fastest in the context's table (and in the JSON cache file
+ * named by XRTK_TUNE_CACHE, when set).  mode is XRTK_MODE_*. */
+int xrtk_autotune(xrtk_ctx *ctx, size_t n_rays, size_t n_pixels, int mode,
+                  XrtkOpts *best, double *best_ms);
+
+/* ---- serving ------------------------------------------------------------ */
+
+typedef struct xrtk_server xrtk_server;
+typedef uint64_t xrtk_ticket;
+
+typedef struct XrtkServerCfg {
+    size_t hbm_budget_bytes;   /* admission budget (resident rays excluded) */
+    int32_t max_active;        /* concurrent requests on the device         */
+    double quantum_ms;         /* target device time of one batched launch  */
+    int32_t mode;              /* XRTK_MODE_*                               */
+    int32_t reserved;
+} XrtkServerCfg;
+
+typedef struct XrtkServerStats {
+    uint64_t submitted, admitted, completed, rejected;
+    uint64_t launches;          /* batched pair-kernel launches             */
+    uint64_t max_concurrent;    /* most requests sharing one launch          */
+    double pairs_done;          /* ray-pixel pairs evaluated                */
+    double busy_ms;             /* device time inside pair kernels          */
+    size_t hbm_in_use, hbm_peak;
+} XrtkServerStats;
+
+int xrtk_server_create(xrtk_ctx *ctx, const XrtkServerCfg *cfg,
+                       xrtk_server **out);
+/* Queues one solve (non-blocking).  out4 must stay valid until the ticket
+ * completes.  Returns XRTK_ENOMEM at once when the request alone exceeds the
+ * budget. */
+int xrtk_server_submit(xrtk_server *s, const XrtkJob *job,
+                       xrtk_ticket *ticket);
+/* XRTK_OK when done, XRTK_EAGAIN while pending. */
+int xrtk_server_poll(xrtk_server *s, xrtk_ticket t);
+/* Blocks until done or timeout_ms elapses (negative: no limit).  On
+ * success fills the request's timestamps (seconds, steady clock):
+ * t[0] = submit, t[1] = admit, t[2] = first launch, t[3] = done. */
+int xrtk_server_wait(xrtk_server *s, xrtk_ticket t, double timeout_ms,
+                     double t_out[4]);
+int xrtk_server_stats(xrtk_server *s, XrtkServerStats *st);
+/* Completes the queued work, then stops the scheduler thread. */
+void xrtk_server_destroy(xrtk_server *s);
+
+/* Steady-clock seconds, the clock of xrtk_server_wait's timestamps. */
+double xrtk_now(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XRTK_H */

@@ -1,0 +1,311 @@
+"""Device parity: the FP64 pair kernel against the oracle, the FP32
+phase-reduced kernel against both, the reference's GPU tests restated, edge
+cases and the workload shapes (gfx950 only)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import xrtk
+from xrtk import workloads as W
+from helpers import f32_exact, fixture_case, mp_cases, ray, rays_of, to_gpu_pixels, to_gpu_rays
+
+pytestmark = pytest.mark.gpu
+
+REL_F32R = 1e-4   # complex rel-L2 bar of the FP32-reduced kernel vs FP64
+REL_F64 = 1e-11   # FP64 kernel vs the oracle (same formula, same rounding but sin/cos ulps)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def grid(scr):
+    return xrtk.WaveParams(scr.center, scr.dx, scr.dz, scr.nx, scr.nz)
+
+
+# ---- golden vector ------------------------------------------------------------
+def test_f64_kernel_fixture(ctx):
+    """golden_diffraction_numerical bar (1e-10 abs) on the FP64 device kernel."""
+    rays, pix, exp = fixture_case()
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f64")
+    assert np.max(np.abs(got - exp)) < 1e-10
+    assert xrtk.GpuContext.last_plan(ctx)["backend"] == 1
+
+
+def test_f32r_kernel_fixture(ctx):
+    """golden_gpu_long_distance_vs_fixture bar (rel < 1e-3 per pixel); the
+    reduced kernel with FP64 centroid/base phase is far inside it."""
+    rays, pix, exp = fixture_case()
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
+    rel = np.abs(got - exp).max(axis=1) / np.abs(exp).max(axis=1)
+    assert np.all(rel < 1e-3)
+    assert rel_l2(got, exp) < 1e-5
+
+
+@pytest.mark.parametrize("case", mp_cases(), ids=lambda c: c[0])
+def test_f64_kernel_vs_oracle_mp_cases(ctx, case):
+    name, rays, pix, exp = case
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f64")
+    ref = O.diffraction_f64(rays, *pix)
+    assert rel_l2(got, ref) < REL_F64
+
+
+@pytest.mark.parametrize("case", mp_cases(), ids=lambda c: c[0])
+def test_f32r_kernel_vs_mpmath(ctx, case):
+    name, rays, pix, exp = case
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
+    assert rel_l2(got, exp) < 1e-5
+
+
+# ---- the reference's GPU tests (tests/golden_gpu_kirchhoff.rs) restated -------------
+def test_long_distance_vs_cpu(ctx):
+    """golden_gpu_long_distance_vs_cpu (:103-145): f32 inputs, rel < 1e-3."""
+    rays, pix, _ = fixture_case()
+    r32, p32 = to_gpu_rays(rays), to_gpu_pixels(pix)
+    got = xrtk.kirchhoff_gpu(ctx, r32, p32)
+    cpu = O.fallback_f32(r32, p32)
+    assert np.all(np.abs(got[:, 0]) > 1e-10)
+    assert np.all(np.abs(got[:, 0] - cpu[:, 0]) / np.abs(cpu[:, 0]) < 1e-3)
+    assert rel_l2(got, cpu) < 1e-5
+
+
+def test_short_distance_vs_cpu(ctx):
+    """golden_gpu_short_distance_vs_cpu (:184-247): 1 keV, 0.1 mm, rel < 5e-2."""
+    rays = rays_of([ray(energy=1000.0, es=1 + 0j, ep=0.5 + 0.1j),
+                    ray(x=0.01, energy=1000.0, es=0.8 - 0.1j, ep=0.6 + 0j)])
+    pix = ([0.0, 0.005], [0.0, 0.0], [0.1, 0.1])
+    r32, p32 = to_gpu_rays(rays), to_gpu_pixels(pix)
+    got = xrtk.kirchhoff_gpu(ctx, r32, p32)
+    cpu = O.fallback_f32(r32, p32)
+    assert np.all(np.abs(got[:, 0] - cpu[:, 0]) / np.abs(cpu[:, 0]) < 5e-2)
+    assert rel_l2(got, cpu) < 1e-5
+
+
+def test_symmetry(ctx):
+    """golden_gpu_symmetry (:249-311): equal |Es| within 1e-4 relative."""
+    rays = rays_of([ray(es=1 + 0j, ep=1 + 0j)])
+    got = xrtk.kirchhoff_gpu(ctx, to_gpu_rays(rays),
+                             to_gpu_pixels(([-0.01, 0.01], [0.0, 0.0], [1000.0, 1000.0])))
+    a, b = abs(got[0, 0]), abs(got[1, 0])
+    assert a > 1e-10 and b > 1e-10
+    assert abs(a - b) < 1e-4 * a
+
+
+def test_auto_dispatch_uses_device(ctx):
+    """golden_gpu_auto_dispatch (:313-327) — and that the device ran it."""
+    rays, pix, _ = fixture_case()
+    got = xrtk.kirchhoff_auto(to_gpu_rays(rays), to_gpu_pixels(pix))
+    assert xrtk.auto_last_backend() == 1
+    assert np.all(np.isfinite(got)) and np.all(np.abs(got[:, 0]) > 1e-10)
+
+
+def test_extreme_distance(ctx):
+    """golden_gpu_extreme_distance (:329-391): 10 m, rel < 0.05."""
+    rays = rays_of([ray(es=1 + 0j, ep=0.5 + 0j), ray(x=0.01, es=0.8 + 0.1j, ep=0.6 - 0.1j)])
+    pix = ([0.0], [0.0], [10000.0])
+    r32, p32 = to_gpu_rays(rays), to_gpu_pixels(pix)
+    got = xrtk.kirchhoff_gpu(ctx, r32, p32)
+    cpu = O.fallback_f32(r32, p32)
+    assert abs(got[0, 0]) > 1e-10
+    assert abs(got[0, 0] - cpu[0, 0]) / abs(cpu[0, 0]) < 0.05
+    assert rel_l2(got, cpu) < 1e-5
+
+
+# ---- edge cases ------------------------------------------------------------------------
+def test_zero_rays_gives_zeros(ctx):
+    rays = rays_of([ray()])
+    empty = {k: v[:0] for k, v in rays.items()}
+    for mode in ("f32r", "f64"):
+        got = xrtk.diffraction_integral(empty, ([0.0, 1.0], [0.0, 0.0], [1.0, 1.0]), ctx=ctx,
+                                        mode=mode)
+        assert got.shape == (2, 2) and np.all(got == 0)
+    got = xrtk.kirchhoff_gpu(ctx, xrtk.gpu_rays(0), to_gpu_pixels(([0.0], [0.0], [1.0])))
+    assert np.all(got == 0)
+
+
+def test_zero_pixels(ctx):
+    rays, _, _ = fixture_case()
+    got = xrtk.diffraction_integral(rays, ([], [], []), ctx=ctx)
+    assert got.shape == (0, 2)
+
+
+def test_ray_at_pixel_skipped(ctx):
+    """ray_at_pixel_location_skipped: the near-singular path skips path == 0."""
+    rays = rays_of([ray(x=1.0, y=2.0, z=3.0, es=1 + 0j, ep=1 + 0j),
+                    ray(x=1.5, y=2.0, z=3.0, es=0.5 + 0j, ep=0j, energy=10000.0)])
+    pix = ([1.0, 1.2], [2.0, 2.0], [3.0, 3.0])
+    ref = O.diffraction_f64(rays, *pix)
+    for mode in ("f64", "f32r"):
+        got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode=mode)
+        assert np.all(np.isfinite(got))
+        assert rel_l2(got, ref) < (REL_F64 if mode == "f64" else 1e-5)
+    one = rays_of([ray(x=1.0, y=2.0, z=3.0, es=1 + 0j, ep=1 + 0j)])
+    for mode in ("f64", "f32r"):
+        got = xrtk.diffraction_integral(one, ([1.0], [2.0], [3.0]), ctx=ctx, mode=mode)
+        assert np.all(got == 0)
+
+
+def test_polychromatic_groups(ctx):
+    """Two energies run as two monochromatic instances (each with its own
+    base phase); the reference GPU path uses one mean energy here."""
+    rng = np.random.default_rng(3)
+    lst = [ray(x=rng.uniform(-.1, .1), z=rng.uniform(-.1, .1), nx=0, ny=1, nz=0,
+               energy=8000.0 if i % 2 else 12000.0, es=complex(rng.normal(), rng.normal()),
+               ep=complex(rng.normal(), rng.normal())) for i in range(600)]
+    rays = rays_of(lst)
+    pix = (np.linspace(-.02, .02, 37), np.full(37, 1000.0), np.linspace(.01, -.01, 37))
+    ref = O.diffraction_f64(rays, *pix)
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
+    assert rel_l2(got, ref) < REL_F32R
+    got64 = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f64")
+    assert rel_l2(got64, ref) < REL_F64
+
+
+def test_broadband_routes_to_fp64(ctx):
+    rng = np.random.default_rng(4)
+    lst = [ray(x=rng.uniform(-.05, .05), y=rng.uniform(-.05, .05), energy=5000.0 + 10.0 * i)
+           for i in range(100)]
+    rays = rays_of(lst)
+    pix = ([0.0, 0.001], [0.0, 0.0], [100.0, 100.0])
+    got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
+    assert ctx.last_plan()["mode"] == xrtk.MODE_F64
+    assert rel_l2(got, O.diffraction_f64(rays, *pix)) < REL_F64
+
+
+# ---- launch-shape invariance (every instantiated kernel) ------------------------------------
+SHAPES_F32 = [(P, W, s, per) for s in (1, 2) for P in (2, 4, 8) for W in (1, 4)
+              for per in (0, 1) if not (s == 2 and W == 1)]
+SHAPES_F64 = [(P, W, 1, per) for P in (1, 2) for W in (1, 4) for per in (0, 1)]
+
+
+def test_f32r_result_independent_of_launch_shape(ctx):
+    """Pixels-per-lane, waves per group, ray source and persistence only move
+    work between lanes: results are bit-identical."""
+    rays = W.point_source_aperture(20000, seed=9)
+    scr = W.screen(1000.0, 0.02, 40)
+    base = None
+    for P, Wv, s, per in SHAPES_F32:
+        got = xrtk.diffraction_integral(rays, grid(scr), ctx=ctx, mode="f32r", pixels_per_lane=P,
+                                        waves_per_group=Wv, ray_src=s, persistent=per,
+                                        tile_rays=1024, split=3)
+        if base is None:
+            base = got
+        else:
+            assert np.array_equal(got, base), (P, Wv, s, per)
+
+
+def test_f64_result_independent_of_launch_shape(ctx):
+    rays = W.point_source_aperture(3000, seed=10)
+    scr = W.screen(1000.0, 0.02, 24)
+    base = None
+    for P, Wv, s, per in SHAPES_F64:
+        got = xrtk.diffraction_integral(rays, grid(scr), ctx=ctx, mode="f64", pixels_per_lane=P,
+                                        waves_per_group=Wv, persistent=per, split=2)
+        if base is None:
+            base = got
+        else:
+            assert np.array_equal(got, base), (P, Wv, s, per)
+
+
+def test_split_changes_only_fp64_rounding(ctx):
+    rays = W.point_source_aperture(30000, seed=11)
+    scr = W.screen(1000.0, 0.02, 32)
+    a = xrtk.diffraction_integral(rays, grid(scr), ctx=ctx, mode="f32r", split=1)
+    b = xrtk.diffraction_integral(rays, grid(scr), ctx=ctx, mode="f32r", split=7)
+    assert rel_l2(b, a) < 1e-12
+
+
+# ---- workload shapes -------------------------------------------------------------------------------
+def test_point_source_full_size(ctx):
+    """65,536 rays x 256^2 pixels at 1000 mm, 10 keV: FP32-reduced vs the FP64
+    kernel over every pixel, and the FP64 kernel vs the oracle on a strided
+    pixel subsample."""
+    rays = W.point_source_aperture(65536, seed=1)
+    scr = W.screen(1000.0, 0.02, 256)
+    g = grid(scr)
+    f64 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
+    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    err = rel_l2(f32, f64)
+    print(f"point source 65536 x 256^2: rel-L2(f32r, f64) = {err:.3e}, "
+          f"intensity err (max, p99) = {W.intensity_error(f32, f64)}")
+    assert err < REL_F32R
+    px, py, pz = xrtk.prepare_wave(g)
+    stride = 1031
+    ref = O.diffraction_f64(rays, px, py, pz, stride=stride)
+    assert rel_l2(f64[::stride], ref) < REL_F64
+
+
+def test_gaussian_source_full_size(ctx):
+    """1,048,576 rays x 512^2 at 1000 mm: FP32-reduced vs FP64 on every pixel."""
+    rays = W.gaussian_source(1048576, seed=2)
+    g = grid(W.screen(1000.0, 0.02, 512))
+    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    f64 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
+    err = rel_l2(f32, f64)
+    print(f"gaussian source 1M x 512^2: rel-L2 = {err:.3e}, "
+          f"intensity err = {W.intensity_error(f32, f64)}")
+    assert err < REL_F32R
+    px, py, pz = xrtk.prepare_wave(g)
+    stride = 4099
+    ref = O.diffraction_f64(rays, px, py, pz, stride=stride)
+    assert rel_l2(f64[::stride], ref) < 1e-9
+
+
+@pytest.mark.parametrize("energy", W.STRESS_ENERGIES)
+@pytest.mark.parametrize("distance", W.STRESS_DISTANCES)
+def test_stress_cells(ctx, energy, distance):
+    """262,144 rays x 256^2 for every (E, L) cell: FP32-reduced vs FP64."""
+    rays, scr, info = W.stress_cell(energy, distance)
+    g = grid(scr)
+    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    f64 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
+    err = rel_l2(f32, f64)
+    print(f"E={energy:.0f} L={distance:.0f} a={info['aperture_mm']:.3f} "
+          f"NF={info['fresnel_number']:.1f}: rel-L2 = {err:.3e}")
+    assert err < REL_F32R
+
+
+# ---- resident ray sets, upstream path and batches ----------------------------------------------
+def test_path_up_phase(ctx):
+    rng = np.random.default_rng(12)
+    rays = W.point_source_aperture(4000, seed=12)
+    pu = 10000.0 + rng.uniform(0.0, 0.01, 4000)
+    for k in ("es_re", "es_im"):
+        rays[k] = rays[k].copy()
+    rays["es_re"][:] = 1.0
+    rays["es_im"][:] = 0.0
+    scr = W.screen(1000.0, 0.02, 16)
+    px, py, pz = xrtk.prepare_wave(grid(scr))
+    ref = O.diffraction_f64(rays, px, py, pz, path_up=pu)
+    rs = xrtk.RaySet(ctx, rays, path_up=pu)
+    try:
+        for mode, tol in (("f32r", REL_F32R), ("f64", 1e-9)):
+            (got,) = xrtk.solve_jobs(ctx, [xrtk.Job(rs, 10000.0, grid(scr))], mode=mode)
+            assert rel_l2(got, ref) < tol, mode
+    finally:
+        rs.release()
+
+
+def test_batch_equals_standalone_bitwise(ctx):
+    specs = W.batch_instances(n=6, seed=21, max_rays=60000, max_side=96)
+    sets = [xrtk.RaySet(ctx, W.instance_rays(s)) for s in specs]
+    try:
+        jobs = [xrtk.Job(rs, s["energy"], grid(W.instance_screen(s))) for rs, s in zip(sets, specs)]
+        mixed = xrtk.solve_jobs(ctx, jobs)
+        for j, m in zip(jobs, mixed):
+            (alone,) = xrtk.solve_jobs(ctx, [j])
+            assert np.array_equal(alone, m)
+        # graph replays of the same batch, with eager work in between
+        for _ in range(3):
+            again = xrtk.solve_jobs(ctx, jobs)
+            xrtk.diffraction_integral(W.point_source_aperture(500, seed=1),
+                                      grid(W.screen(1000.0, 0.02, 8)), ctx=ctx)
+            for a, m in zip(again, mixed):
+                assert np.array_equal(a, m)
+        # and each job against the FP64 kernel
+        for j, m, s in zip(jobs, mixed, specs):
+            ref = xrtk.diffraction_integral(W.instance_rays(s), j.screen, ctx=ctx, mode="f64")
+            assert rel_l2(m, ref) < REL_F32R
+    finally:
+        for rs in sets:
+            rs.release()

@@ -1,0 +1,385 @@
+// Batched solves over resident ray sets (one persistent pair-kernel launch,
+// replayed from a hipGraph while the batch repeats) and the autotuner.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "runtime.h"
+
+using namespace xrtk;
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+uint64_t bits(double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  return b;
+}
+
+size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// The launch configuration of batched solves: fixed, so that a job's result
+// does not depend on which other jobs share the launch.
+LaunchCfg batch_cfg(uint32_t mode, const XrtkOpts* o) {
+  LaunchCfg c;
+  if (mode == MODE_F32R) { c.P = 4; c.W = 4; } else { c.P = 1; c.W = 4; }
+  c.ray_src = 1;
+  c.persistent = 1;
+  c.tile = 1024;
+  if (o) {
+    if (o->pixels_per_lane > 0) c.P = o->pixels_per_lane;
+    if (o->waves_per_group > 0) c.W = o->waves_per_group;
+    if (o->ray_src > 0) c.ray_src = o->ray_src;
+    if (o->persistent == 0) c.persistent = 0;
+    if (o->tile_rays > 0) c.tile = (o->tile_rays + 63) / 64 * 64;
+  }
+  return c;
+}
+
+}  // namespace
+
+namespace xrtk {
+
+// Builds the instance specs of a batch inside `pixbuf` / `outbuf` (device).
+int build_job_specs(xrtk_ctx* ctx, uint32_t mode, const XrtkJob* jobs, size_t n,
+                    DBuf& pixbuf, DBuf& outbuf, std::vector<InstSpec>& specs,
+                    std::vector<size_t>& out_off) {
+  size_t pix_total = 0;
+  std::vector<size_t> pix_off(n);
+  out_off.assign(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const XrtkJob& j = jobs[i];
+    const size_t np = (size_t)j.screen.nx * j.screen.nz;
+    if (np > 0x7fffffffull) return XRTK_EINVAL;
+    if (np && !j.out4) return XRTK_EINVAL;
+    auto it = ctx->raysets.find(j.rays);
+    if (it == ctx->raysets.end()) return XRTK_EINVAL;
+    const RaySet& rs = it->second;
+    if (mode == MODE_F32R && !(j.energy > 0) && !rs.mono) return XRTK_EINVAL;
+    pix_off[i] = pix_total;
+    out_off[i] = pix_total;
+    pix_total += a256(np);
+  }
+  if (pixbuf.ensure(sizeof(PixF64) * std::max<size_t>(pix_total, 1)) != hipSuccess ||
+      outbuf.ensure(sizeof(double) * 4 * std::max<size_t>(pix_total, 1)) != hipSuccess)
+    return XRTK_ENOMEM;
+  specs.clear();
+  for (size_t i = 0; i < n; ++i) {
+    const XrtkJob& j = jobs[i];
+    const RaySet& rs = ctx->raysets.find(j.rays)->second;
+    InstSpec s;
+    s.rays = rs.rays.as<RayRec64>();
+    s.nr = rs.nr;
+    s.pix = pixbuf.as<PixF64>() + pix_off[i];
+    s.np = j.screen.nx * j.screen.nz;
+    s.energy = j.energy > 0 ? j.energy : (mode == MODE_F32R ? rs.energy : 0.0);
+    s.skip = 1e-30;
+    s.static_flags = rs.has_path ? INST_F_PATHUP : 0u;
+    s.out = outbuf.as<double>() + 4 * out_off[i];
+    specs.push_back(s);
+  }
+  return XRTK_OK;
+}
+
+int enqueue_job_grids(xrtk_ctx* ctx, const XrtkJob* jobs, const std::vector<InstSpec>& specs) {
+  for (size_t i = 0; i < specs.size(); ++i) {
+    const XrtkWaveParams& w = jobs[i].screen;
+    hipError_t e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz,
+                                   const_cast<PixF64*>(specs[i].pix), ctx->stream);
+    if (e != hipSuccess) return map_err(e);
+  }
+  return XRTK_OK;
+}
+
+}  // namespace xrtk
+
+extern "C" {
+
+int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const XrtkOpts* opts) {
+  if (!ctx) return XRTK_ENODEV;
+  if (n_jobs && !jobs) return XRTK_EINVAL;
+  if (!n_jobs) return XRTK_OK;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    const uint32_t mode = opts && opts->mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
+    const LaunchCfg cfg = batch_cfg(mode, opts);
+    if (!pair_cfg_supported(mode, cfg)) return XRTK_EINVAL;
+    const bool use_graph = !(std::getenv("XRTK_NO_GRAPH") && *std::getenv("XRTK_NO_GRAPH") == '1');
+    // graph key: everything the captured nodes bake in
+    std::vector<uint64_t> key;
+    key.push_back(n_jobs);
+    key.push_back(mode);
+    key.push_back(((uint64_t)cfg.P << 32) | ((uint64_t)cfg.W << 16) | ((uint64_t)cfg.ray_src << 8) |
+                  (uint64_t)cfg.persistent);
+    key.push_back(cfg.tile);
+    for (size_t i = 0; i < n_jobs; ++i) {
+      const XrtkJob& j = jobs[i];
+      key.push_back(j.rays);
+      key.push_back(bits(j.energy));
+      key.push_back(bits(j.screen.center[0]));
+      key.push_back(bits(j.screen.center[1]));
+      key.push_back(bits(j.screen.center[2]));
+      key.push_back(bits(j.screen.dx));
+      key.push_back(bits(j.screen.dz));
+      key.push_back(((uint64_t)j.screen.nx << 32) | j.screen.nz);
+    }
+    xrtk_ctx::GraphEntry* ge = nullptr;
+    for (auto* g2 : ctx->graphs)
+      if (g2->key == key) { ge = g2; break; }
+    std::vector<InstSpec> specs;
+    std::vector<size_t> out_off;
+    hipError_t e;
+    if (!ge) {
+      ge = new xrtk_ctx::GraphEntry();
+      ge->key = key;
+      int rc = build_job_specs(ctx, mode, jobs, n_jobs, ge->pix, ge->outd, specs, out_off);
+      if (rc != XRTK_OK) { ge->pix.release(); ge->outd.release(); delete ge; return rc; }
+      for (size_t i = 0; i < n_jobs; ++i) ge->outs.emplace_back(specs[i].out, (size_t)specs[i].np);
+      // first run: eager, with the metadata upload; then capture the kernels
+      rc = enqueue_job_grids(ctx, jobs, specs);
+      Plan plan;
+      if (rc == XRTK_OK)
+        rc = run_instances(ctx, mode, cfg, specs, ge->work, ge->meta, true, &plan, 0, 0, true);
+      if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
+      if (rc != XRTK_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ge->work.release(); ge->meta.release(); ge->pix.release(); ge->outd.release();
+        delete ge;
+        return rc;
+      }
+      if (use_graph) {
+        // Capture the same device work; the metadata is already in place.
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          int rc2 = enqueue_job_grids(ctx, jobs, specs);
+          if (rc2 == XRTK_OK)
+            rc2 = run_instances(ctx, mode, cfg, specs, ge->work, ge->meta, false, nullptr, 0, 0,
+                                true, /*upload=*/false);
+          e = hipStreamEndCapture(ctx->stream, &graph);
+          if (rc2 == XRTK_OK && e == hipSuccess && graph) {
+            e = hipGraphInstantiate(&ge->exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) ge->exec = nullptr;
+          }
+          if (graph) (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+      }
+      ctx->graphs.push_back(ge);
+      float a = 0, b = 0;
+      (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]);
+      (void)hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]);
+      ctx->last_pair_ms = a;
+      ctx->last_solve_ms = b;
+      XrtkPlan& q = ctx->last_plan;
+      q.mode = (int32_t)mode; q.pixels_per_lane = cfg.P; q.waves_per_group = cfg.W;
+      q.tile_rays = cfg.tile; q.split = 0; q.ray_src = cfg.ray_src; q.persistent = cfg.persistent;
+      q.n_items = (int32_t)plan.n_items; q.checked = 0; q.backend = 1;
+    } else {
+      (void)hipEventRecord(ctx->ev[0], ctx->stream);
+      if (ge->exec) {
+        e = hipGraphLaunch(ge->exec, ctx->stream);
+        if (e != hipSuccess) return map_err(e);
+      } else {
+        return XRTK_ELAUNCH;
+      }
+      (void)hipEventRecord(ctx->ev[3], ctx->stream);
+      e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return map_err(e);
+      float b = 0;
+      (void)hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]);
+      ctx->last_solve_ms = b;
+      ctx->last_pair_ms = b;
+    }
+    // results -> host
+    for (size_t i = 0; i < n_jobs; ++i) {
+      const size_t np = ge->outs[i].second;
+      if (!np) continue;
+      e = hipMemcpyAsync(jobs[i].out4, ge->outs[i].first, sizeof(double) * 4 * np,
+                         hipMemcpyDeviceToHost, ctx->stream);
+      if (e != hipSuccess) return map_err(e);
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    return map_err(e);
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+namespace {
+
+int ilog2u(uint64_t v) {
+  int l = 0;
+  while (v > 1) { v >>= 1; ++l; }
+  return l;
+}
+
+// Synthetic instance of the bucket: the flat-aperture point-source geometry
+// (rays on y = 0 within +-0.1 mm, screen 1000 mm downstream, 10 keV).
+void synth(size_t nr, size_t np, std::vector<double>& cols, std::vector<double>& pix) {
+  cols.assign(13 * nr, 0.0);
+  uint64_t s = 0x9e3779b97f4a7c15ull;
+  auto u = [&] {
+    s += 0x9e3779b97f4a7c15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (size_t i = 0; i < nr; ++i) {
+    cols[0 * nr + i] = (u() - 0.5) * 0.2;
+    cols[2 * nr + i] = (u() - 0.5) * 0.2;
+    cols[4 * nr + i] = 1.0;
+    cols[6 * nr + i] = 1.0;
+    cols[7 * nr + i] = 1.0;
+    cols[11 * nr + i] = 10000.0;
+  }
+  const size_t side = (size_t)std::max<double>(1.0, std::floor(std::sqrt((double)np)));
+  pix.assign(3 * np, 0.0);
+  for (size_t i = 0; i < np; ++i) {
+    const size_t ix = i / side, iz = i % side;
+    pix[i] = ((double)ix / (double)side - 0.5) * 0.04;
+    pix[np + i] = 1000.0;
+    pix[2 * np + i] = ((double)iz / (double)side - 0.5) * 0.04;
+  }
+}
+
+void save_cache(xrtk_ctx* ctx) {
+  const char* path = std::getenv("XRTK_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::ofstream f(path);
+  if (!f) return;
+  f << "[\n";
+  bool first = true;
+  for (auto& kv : ctx->tune) {
+    if (!first) f << ",\n";
+    first = false;
+    f << "  {\"mode\": " << std::get<0>(kv.first) << ", \"log2_rays\": " << std::get<1>(kv.first)
+      << ", \"log2_pixels\": " << std::get<2>(kv.first) << ", \"P\": " << kv.second.P
+      << ", \"W\": " << kv.second.W << ", \"ray_src\": " << kv.second.ray_src
+      << ", \"persistent\": " << kv.second.persistent << ", \"split\": " << kv.second.split
+      << ", \"tile\": " << kv.second.tile << "}";
+  }
+  f << "\n]\n";
+}
+
+}  // namespace
+
+namespace xrtk {
+
+// Loads the JSON cache written by save_cache (one object per line).
+void load_tune_cache(xrtk_ctx* ctx) {
+  const char* path = std::getenv("XRTK_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    int mode, lr, lp, P, W, src, per, split, tile;
+    if (std::sscanf(line.c_str(),
+                    " {\"mode\": %d, \"log2_rays\": %d, \"log2_pixels\": %d, \"P\": %d, \"W\": %d, "
+                    "\"ray_src\": %d, \"persistent\": %d, \"split\": %d, \"tile\": %d",
+                    &mode, &lr, &lp, &P, &W, &src, &per, &split, &tile) == 9) {
+      LaunchCfg c;
+      c.P = P; c.W = W; c.ray_src = src; c.persistent = per; c.split = split; c.tile = tile;
+      if (pair_cfg_supported((uint32_t)mode, c)) ctx->tune[std::make_tuple(mode, lr, lp)] = c;
+    }
+  }
+}
+
+}  // namespace xrtk
+
+extern "C" int xrtk_autotune(xrtk_ctx* ctx, size_t n_rays, size_t n_pixels, int mode_in,
+                             XrtkOpts* best, double* best_ms) {
+  if (!ctx) return XRTK_ENODEV;
+  if (!n_rays || !n_pixels || n_rays > 0xffffffffull || n_pixels > 0x7fffffffull)
+    return XRTK_EINVAL;
+  const uint32_t mode = mode_in == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
+  try {
+    std::vector<double> cols, pix;
+    // Time on the full pixel count and a ray sample: per-pair cost does not
+    // depend on the ray count once chunks are long.
+    const size_t nr = std::min<size_t>(n_rays, mode == MODE_F32R ? 131072 : 16384);
+    synth(nr, n_pixels, cols, pix);
+    XrtkRaysF64 r;
+    r.x = &cols[0]; r.y = &cols[nr]; r.z = &cols[2 * nr];
+    r.nx = &cols[3 * nr]; r.ny = &cols[4 * nr]; r.nz = &cols[5 * nr];
+    r.nl = &cols[6 * nr]; r.es_re = &cols[7 * nr]; r.es_im = &cols[8 * nr];
+    r.ep_re = &cols[9 * nr]; r.ep_im = &cols[10 * nr]; r.energy = &cols[11 * nr];
+    std::vector<double> out(4 * n_pixels);
+    std::vector<LaunchCfg> cands;
+    if (mode == MODE_F32R) {
+      for (int src : {1, 2})
+        for (int P : {2, 4, 8})
+          for (int W : {1, 4})
+            for (int per : {0, 1}) {
+              LaunchCfg c; c.P = P; c.W = W; c.ray_src = src; c.persistent = per; c.tile = 1024;
+              if (pair_cfg_supported(mode, c)) cands.push_back(c);
+            }
+    } else {
+      for (int P : {1, 2})
+        for (int W : {1, 4})
+          for (int per : {0, 1}) {
+            LaunchCfg c; c.P = P; c.W = W; c.ray_src = 1; c.persistent = per; c.tile = 1024;
+            cands.push_back(c);
+          }
+    }
+    double bt = 1e300;
+    LaunchCfg bc = cands.front();
+    for (const LaunchCfg& c : cands) {
+      XrtkOpts o;
+      xrtk_opts_default(&o);
+      o.mode = (int)mode;
+      o.pixels_per_lane = c.P; o.waves_per_group = c.W; o.ray_src = c.ray_src;
+      o.persistent = c.persistent; o.tile_rays = c.tile;
+      double t = 1e300;
+      for (int rep = 0; rep < 3; ++rep) {
+        const int rc = xrtk_diffraction_f64(ctx, &r, nr, &pix[0], &pix[n_pixels],
+                                            &pix[2 * n_pixels], n_pixels, out.data(), &o);
+        if (rc != XRTK_OK) { t = 1e300; break; }
+        if (rep) t = std::min(t, ctx->last_pair_ms);
+      }
+      if (t < bt) { bt = t; bc = c; }
+    }
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->tune[std::make_tuple((int)mode, ilog2u(n_rays), ilog2u(n_pixels))] = bc;
+      save_cache(ctx);
+    }
+    if (best) {
+      xrtk_opts_default(best);
+      best->mode = (int)mode;
+      best->pixels_per_lane = bc.P; best->waves_per_group = bc.W; best->ray_src = bc.ray_src;
+      best->persistent = bc.persistent; best->tile_rays = bc.tile;
+    }
+    if (best_ms) *best_ms = bt * (double)n_rays / (double)nr;
+    return XRTK_OK;
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
