@@ -87,10 +87,10 @@ typedef struct XrtkWaveParams {
                              base phase and merge (the default)            */
 #define XRTK_MODE_F64  1  /* plain FP64 pair kernel with Kahan sums        */
 
-/* Where the pair kernel reads ray records from. */
+/* Where the pair kernel reads ray records from: scalar loads into SGPRs
+ * (wave-uniform), with the per-(ray, pixel tile) values staged in LDS. */
 #define XRTK_RAYSRC_AUTO 0
-#define XRTK_RAYSRC_SMEM 1  /* scalar loads into SGPRs (wave-uniform)      */
-#define XRTK_RAYSRC_LDS  2  /* workgroup stages a ray tile in LDS          */
+#define XRTK_RAYSRC_SMEM 1
 
 typedef struct XrtkOpts {
     int32_t mode;             /* XRTK_MODE_*                                */
@@ -215,7 +215,7 @@ int xrtk_last_timing(xrtk_ctx *ctx, double *pair_ms, double *solve_ms);
 typedef struct XrtkPlan {
     int32_t mode, pixels_per_lane, waves_per_group, tile_rays;
     int32_t split, ray_src, persistent, n_items;
-    int32_t checked;    /* 1 if the near-singular (skip-test) path ran     */
+    int32_t n_fp64;     /* instances the bound routed to the FP64 kernel    */
     int32_t backend;    /* 1 = device, 0 = CPU fallback                     */
 } XrtkPlan;
 int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);

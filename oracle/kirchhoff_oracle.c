@@ -16,6 +16,11 @@
  *                            :96-115,276-297 + src/gpu/shaders/kirchhoff.wgsl
  *                            :65-129, in float32 (the reference GPU path)
  *   oracle_prepare_wave      src/waves/prepare_wave.rs:40-68
+ *   oracle_diffraction_ld    the same sum as oracle_diffraction_f64 with the
+ *                            path and phase in x87 extended precision (64-bit
+ *                            significand, sinl/cosl): the checker for phases
+ *                            k*path ~ 1e12 rad, where any FP64 evaluation
+ *                            rounds each term's phase by ~1e-4 rad
  *
  * Arithmetic follows the reference's operation order.  Build with
  * -ffp-contract=off and without fast-math (Rust neither contracts nor
@@ -208,6 +213,55 @@ void oracle_prepare_wave(const double center[3], double dx, double dz, size_t nx
       pz[k] = center[2] + fz * 2.0 * dz;
       ++k;
     }
+  }
+}
+
+/* Extended-precision restatement for large phases: path and k*path in long
+ * double (rounding ~ phase * 2^-64), the amplitude in long double, Kahan sums
+ * in long double.  Inputs are the same f64 columns. */
+static void pixel_ld(const oracle_rays *r, const double *path_up, size_t nr, double px,
+                     double py, double pz, double *o4) {
+  long double es_re = 0, es_im = 0, ep_re = 0, ep_im = 0;
+  const long double PI_L = 3.141592653589793238462643383279502884L;
+  for (size_t j = 0; j < nr; ++j) {
+    const long double dx = (long double)px - r->x[j];
+    const long double dy = (long double)py - r->y[j];
+    const long double dz = (long double)pz - r->z[j];
+    const long double path = sqrtl(dx * dx + dy * dy + dz * dz);
+    if (path < 1e-30L) continue;
+    const long double ns = (r->nx[j] * dx + r->ny[j] * dy + r->nz[j] * dz) / path;
+    const long double k = (long double)(r->energy[j] / CHBAR * 1e7);  /* the f64 k */
+    const long double phase = k * path;
+    const long double sp = sinl(phase), cp = cosl(phase);
+    const long double amp = k / (4.0L * PI_L) * (r->nl[j] + ns) / path;
+    const long double ur = -(amp * sp), ui = amp * cp;
+    long double er = r->es_re[j], ei = r->es_im[j], qr = r->ep_re[j], qi = r->ep_im[j];
+    if (path_up) {
+      const long double ph = k * path_up[j];
+      const long double s = sinl(ph), c = cosl(ph);
+      const long double er2 = er * c - ei * s, ei2 = er * s + ei * c;
+      const long double qr2 = qr * c - qi * s, qi2 = qr * s + qi * c;
+      er = er2; ei = ei2; qr = qr2; qi = qi2;
+    }
+    es_re += er * ur - ei * ui;
+    es_im += er * ui + ei * ur;
+    ep_re += qr * ur - qi * ui;
+    ep_im += qr * ui + qi * ur;
+  }
+  o4[0] = (double)es_re; o4[1] = (double)es_im; o4[2] = (double)ep_re; o4[3] = (double)ep_im;
+}
+
+void oracle_diffraction_ld(const double *const cols[12], const double *path_up, size_t nr,
+                           const double *px, const double *py, const double *pz, size_t np,
+                           size_t pix_stride, int threads, double *out4) {
+  oracle_rays r = {cols[0], cols[1], cols[2], cols[3], cols[4], cols[5],
+                   cols[6], cols[7], cols[8], cols[9], cols[10], cols[11]};
+  if (pix_stride < 1) pix_stride = 1;
+  const long long nout = (long long)((np + pix_stride - 1) / pix_stride);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (long long q = 0; q < nout; ++q) {
+    const size_t p = (size_t)q * pix_stride;
+    pixel_ld(&r, path_up, nr, px[p], py[p], pz[p], out4 + 4 * q);
   }
 }
 

@@ -28,6 +28,8 @@ def _load():
     lib.oracle_diffraction_f64.argtypes = [C.POINTER(dp), dp, C.c_size_t, dp, dp, dp,
                                            C.c_size_t, C.c_size_t, C.c_int, dp]
     lib.oracle_diffraction_f64.restype = None
+    lib.oracle_diffraction_ld.argtypes = lib.oracle_diffraction_f64.argtypes
+    lib.oracle_diffraction_ld.restype = None
     lib.oracle_fallback_f32.argtypes = [fp, C.c_size_t, fp, C.c_size_t, C.c_int, dp]
     lib.oracle_fallback_f32.restype = None
     lib.oracle_reduced_f32.argtypes = [fp, C.c_size_t, fp, C.c_size_t, dp]
@@ -62,6 +64,24 @@ def diffraction_f64(rays: dict, px, py, pz, path_up=None, stride=1, threads=None
     threads = threads or os.cpu_count() or 1
     lib.oracle_diffraction_f64(arr, _d(pu) if pu is not None else None, n, _d(px), _d(py),
                                _d(pz), npx, stride, int(threads), _d(out))
+    return out.view(np.complex128).reshape(-1, 2)
+
+
+def diffraction_ld(rays: dict, px, py, pz, path_up=None, stride=1, threads=None):
+    """The same sum with path and phase in x87 extended precision (checker
+    for phases ~1e12 rad); returns (n_out, 2) complex128."""
+    cols = [np.ascontiguousarray(np.asarray(rays[f], dtype=np.float64)) for f in FIELDS]
+    n = cols[0].shape[0]
+    px, py, pz = (np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (px, py, pz))
+    npx = px.shape[0]
+    nout = (npx + stride - 1) // stride
+    out = np.zeros((nout, 4))
+    arr = (C.POINTER(C.c_double) * 12)(*[_d(c) for c in cols])
+    pu = (np.ascontiguousarray(np.asarray(path_up, dtype=np.float64))
+          if path_up is not None else None)
+    threads = threads or os.cpu_count() or 1
+    lib.oracle_diffraction_ld(arr, _d(pu) if pu is not None else None, n, _d(px), _d(py),
+                              _d(pz), npx, stride, int(threads), _d(out))
     return out.view(np.complex128).reshape(-1, 2)
 
 

@@ -5,7 +5,9 @@ the sum), the brute-force check of the oracle.
 The formula is the one of physwkim/xrt-rs src/waves/diffraction.rs:75-98:
   U = i k/(4 pi) (nl + n.(p - r)/|p - r|) exp(i k |p - r|) / |p - r|,
   Es += es U, Ep += ep U,  k = E / CHBAR * 1e7 [mm^-1].
-Inputs are float64 values; CHBAR is the float64 constant of consts.rs:74.
+Inputs are float64 values; k is the float64 value E / CHBAR * 1e7 evaluated
+in IEEE double in that order (the reference's k, diffraction.rs:89), then
+taken as exact.
 
 Run:  python tests/golden/make_mp_vectors.py
 """
@@ -33,7 +35,7 @@ def integral(rays, pix):
                 continue
             ns = (mp.mpf(rays["nx"][j]) * dx + mp.mpf(rays["ny"][j]) * dy
                   + mp.mpf(rays["nz"][j]) * dz) / path
-            k = mp.mpf(rays["energy"][j]) / mp.mpf(CHBAR) * mp.mpf(10) ** 7
+            k = mp.mpf(float(rays["energy"][j]) / CHBAR * 1e7)
             amp = k / (4 * mp.pi) * (mp.mpf(rays["nl"][j]) + ns) / path
             u = mp.mpc(0, 1) * amp * mp.expj(k * path)
             es += mp.mpc(rays["es_re"][j], rays["es_im"][j]) * u

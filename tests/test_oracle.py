@@ -108,7 +108,7 @@ def test_single_ray_closed_form():
     d = [mp.mpf(p[0]) - mp.mpf(r["x"]), mp.mpf(p[1]) - mp.mpf(r["y"]), mp.mpf(p[2]) - mp.mpf(r["z"])]
     path = mp.sqrt(sum(v * v for v in d))
     ns = (mp.mpf(r["nx"]) * d[0] + mp.mpf(r["ny"]) * d[1] + mp.mpf(r["nz"]) * d[2]) / path
-    k = mp.mpf(2000.0) / mp.mpf(CHBAR) * 10 ** 7
+    k = mp.mpf(2000.0 / CHBAR * 1e7)  # the reference's f64 k (diffraction.rs:89)
     u = mp.mpc(0, 1) * k / (4 * mp.pi) * (mp.mpf(r["nl"]) + ns) * mp.expj(k * path) / path
     es = complex(mp.mpc(r["es_re"], r["es_im"]) * u)
     ep = complex(mp.mpc(r["ep_re"], r["ep_im"]) * u)
@@ -249,3 +249,44 @@ def test_prepare_wave_oracle_closed_form():
     assert abs(x[0] + 5.0) < 1e-10
     assert abs(x[(11 - 1) * 7] - 5.0) < 1e-10
     assert abs(z[6] - 3.0) < 1e-10 and abs(z[0] + 3.0) < 1e-10
+
+
+# ---- extended-precision checker -------------------------------------------------
+@pytest.mark.parametrize("case", mp_cases(), ids=lambda c: c[0])
+def test_ld_oracle_vs_mpmath(case):
+    """The long-double restatement against the 50-digit sums: its phase
+    rounding is 2^-11 of the FP64 one, so it meets the FP64 bound with room."""
+    name, rays, pix, exp = case
+    got = O.diffraction_ld(rays, *pix)
+    mag, eps = fp64_bound(rays, pix)
+    err = np.abs(got - exp).sum(axis=1)
+    assert np.all(err <= (eps / 64 + 1e-14) * mag.sum(axis=1))
+
+
+def test_ld_oracle_beats_fp64_at_large_phase():
+    """At 30 keV and 30 m (phase ~4.6e12 rad) the FP64 oracle's error is
+    ~1e-4 per term; the long-double checker agrees with mpmath to ~1e-7."""
+    mp.mp.dps = 50
+    rng = np.random.default_rng(8)
+    n = 12
+    rays = rays_of([ray(x=rng.uniform(-.1, .1), y=0.0, z=rng.uniform(-.1, .1), nx=0.0, ny=1.0,
+                        nz=0.0, energy=30000.0) for _ in range(n)])
+    pix = ([0.01, -0.02], [30000.0, 30000.0], [0.0, 0.005])
+    exp = []
+    for p in range(2):
+        es = mp.mpc(0)
+        for j in range(n):
+            d = [mp.mpf(pix[0][p]) - mp.mpf(rays["x"][j]), mp.mpf(pix[1][p]) - mp.mpf(rays["y"][j]),
+                 mp.mpf(pix[2][p]) - mp.mpf(rays["z"][j])]
+            path = mp.sqrt(sum(v * v for v in d))
+            ns = d[1] / path
+            k = mp.mpf(30000.0 / CHBAR * 1e7)  # the reference's f64 k
+            es += mp.mpc(0, 1) * k / (4 * mp.pi) * (1 + ns) * mp.expj(k * path) / path
+        exp.append(complex(es))
+    exp = np.array(exp)
+    ld = O.diffraction_ld(rays, *pix)[:, 0]
+    f64 = O.diffraction_f64(rays, *pix)[:, 0]
+    e_ld = np.abs(ld - exp) / np.abs(exp)
+    e_64 = np.abs(f64 - exp) / np.abs(exp)
+    assert np.all(e_ld < 1e-6)
+    assert np.max(e_64) > 10 * np.max(e_ld)

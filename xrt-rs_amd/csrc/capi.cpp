@@ -1,7 +1,9 @@
-// extern "C" entry points of libxrtk (declared in include/xrtk.h).
+// extern "C" entry points of libxrtk (declared in include/xrtk.h): context,
+// one-shot solves, host helpers, resident ray sets, timing and plan queries.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -17,6 +19,7 @@ using namespace xrtk;
 namespace xrtk {
 int cpu_fallback_f32(const XrtkGpuRay* rays, size_t nr, const XrtkGpuPixel* px, size_t np,
                      double* out4);
+void load_tune_cache(xrtk_ctx* ctx);
 }
 
 namespace {
@@ -42,12 +45,15 @@ int create_ctx(int device, xrtk_ctx** out) {
   std::unique_ptr<xrtk_ctx> c(new (std::nothrow) xrtk_ctx());
   if (!c) return XRTK_ENOMEM;
   c->device = device;
-  c->name = prop.name;
+  c->name = prop.name[0] ? prop.name : "AMD gfx950";
+  c->name += std::string(" (") + prop.gcnArchName + ", " +
+             std::to_string(prop.multiProcessorCount) + " CUs)";
   c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return XRTK_ELAUNCH;
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return XRTK_ELAUNCH;
+  load_tune_cache(c.get());
   *out = c.release();
   return XRTK_OK;
 }
@@ -66,33 +72,10 @@ struct DeviceGuard {
 
 std::atomic<int> g_auto_backend{-1};
 
-void record_timing(xrtk_ctx* ctx) {
-  float a = 0, b = 0;
-  if (hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]) != hipSuccess) a = 0;
-  if (hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]) != hipSuccess) b = 0;
-  (void)hipGetLastError();
-  ctx->last_pair_ms = a;
-  ctx->last_solve_ms = b;
-}
-
-void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
-  XrtkPlan& q = ctx->last_plan;
-  q.mode = (int32_t)p.mode;
-  q.pixels_per_lane = p.cfg.P;
-  q.waves_per_group = p.cfg.W;
-  q.tile_rays = p.cfg.tile;
-  q.split = p.cfg.split;
-  q.ray_src = p.cfg.ray_src;
-  q.persistent = p.cfg.persistent;
-  q.n_items = (int32_t)p.n_items;
-  q.checked = p.checked;
-  q.backend = backend;
-}
-
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-// Groups ray indices by energy value (monochromatic sub-sets).  Returns the
-// groups in order of first appearance.
+// Groups ray indices by energy value (monochromatic sub-sets) in order of
+// first appearance; stops at 65 groups.
 template <class GetE>
 std::vector<std::vector<uint32_t>> group_by_energy(size_t nr, GetE get) {
   std::vector<std::vector<uint32_t>> groups;
@@ -112,28 +95,279 @@ std::vector<std::vector<uint32_t>> group_by_energy(size_t nr, GetE get) {
   return groups;
 }
 
-int mode_from(const XrtkOpts* o) {
+uint32_t mode_from(const XrtkOpts* o) {
   if (!o) return MODE_F32R;
   return o->mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
 }
 
-// Solves instances whose rays already sit in ctx->rays64 and pixels in
-// ctx->pix64, then copies the result to out4.
-int finish_solve(xrtk_ctx* ctx, uint32_t mode, const XrtkOpts* opts,
-                 std::vector<InstSpec>& specs, size_t nr_total, size_t np, double* out4) {
-  LaunchCfg cfg = choose_cfg(ctx, mode, nr_total, np, opts);
+// Per-axis half-extent bound of slots around their tile mean, for an
+// explicit pixel list laid out in tiles of `tsz` consecutive slots (the tail
+// padded with the last pixel).
+template <class Get>
+void explicit_tile_dq(size_t np, uint32_t tsz, Get get, double dq[3]) {
+  dq[0] = dq[1] = dq[2] = 0.0;
+  for (size_t t0 = 0; t0 < np; t0 += tsz) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t s = t0; s < t0 + tsz; ++s) {
+      double v[3];
+      get(std::min(s, np - 1), v);
+      for (int i = 0; i < 3; ++i) { lo[i] = std::min(lo[i], v[i]); hi[i] = std::max(hi[i], v[i]); }
+    }
+    for (int i = 0; i < 3; ++i) dq[i] = std::max(dq[i], hi[i] - lo[i]);
+  }
+}
+
+Box grid_box(const XrtkWaveParams& w) {
+  Box b;
+  b.add(w.center[0] - std::fabs(w.dx), w.center[1], w.center[2] - std::fabs(w.dz));
+  b.add(w.center[0] + std::fabs(w.dx), w.center[1], w.center[2] + std::fabs(w.dz));
+  return b;
+}
+
+void grid_dq(const XrtkWaveParams& w, int P, double dq[3]) {
+  uint32_t th, tw;
+  tile_shape(P, &th, &tw);
+  const double px = w.nx > 1 ? 2.0 * std::fabs(w.dx) / (w.nx - 1) : 0.0;
+  const double pz = w.nz > 1 ? 2.0 * std::fabs(w.dz) / (w.nz - 1) : 0.0;
+  dq[0] = px * th;
+  dq[1] = 0.0;
+  dq[2] = pz * tw;
+}
+
+void record_timing(xrtk_ctx* ctx) {
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]) != hipSuccess) a = 0;
+  if (hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]) != hipSuccess) b = 0;
+  (void)hipGetLastError();
+  ctx->last_pair_ms = a;
+  ctx->last_solve_ms = b;
+}
+
+void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
+  XrtkPlan& q = ctx->last_plan;
+  const bool all64 = p.n_inst && p.n_f64 == p.n_inst;
+  const LaunchCfg& c = all64 ? p.cfg64 : p.cfg32;
+  q.mode = all64 ? XRTK_MODE_F64 : XRTK_MODE_F32R;
+  q.pixels_per_lane = c.P;
+  q.waves_per_group = c.W;
+  q.tile_rays = c.tile;
+  q.split = c.split;
+  q.ray_src = c.ray_src;
+  q.persistent = c.persistent;
+  q.n_items = (int32_t)(p.n_items32 + p.n_items64);
+  q.n_fp64 = (int32_t)p.n_f64;
+  q.backend = backend;
+}
+
+// Pixel source of a one-shot solve.
+struct PixSrc {
+  const XrtkGpuPixel* f32 = nullptr;        // GpuPixel array
+  const double *x = nullptr, *y = nullptr, *z = nullptr;  // f64 columns
+  const XrtkWaveParams* grid = nullptr;     // prepare_wave grid
+  size_t np = 0;
+  void get(size_t i, double v[3]) const {
+    if (f32) { v[0] = f32[i].x; v[1] = f32[i].y; v[2] = f32[i].z; }
+    else { v[0] = x[i]; v[1] = y[i]; v[2] = z[i]; }
+  }
+};
+
+// Ray source of a one-shot solve: f32 GpuRay records or f64 columns.
+struct RaySrc {
+  const XrtkGpuRay* f32 = nullptr;
+  const XrtkRaysF64* f64 = nullptr;
+  size_t nr = 0;
+  double energy(size_t i) const { return f32 ? (double)f32[i].energy : f64->energy[i]; }
+  void pos(size_t i, double v[3]) const {
+    if (f32) { v[0] = f32[i].x; v[1] = f32[i].y; v[2] = f32[i].z; }
+    else { v[0] = f64->x[i]; v[1] = f64->y[i]; v[2] = f64->z[i]; }
+  }
+};
+
+// Uploads 13 f64 columns (path_up may be null: zeros) into ctx->stage at
+// offset 0, optionally permuted, and converts them into dst.
+int upload_cols(xrtk_ctx* ctx, const XrtkRaysF64* r, const double* path_up,
+                const std::vector<uint32_t>* perm, uint32_t nr, RayRec64* dst) {
+  if (!nr) return XRTK_OK;
+  const double* cols[13] = {r->x,     r->y,     r->z,     r->nx,    r->ny,
+                            r->nz,    r->nl,    r->es_re, r->es_im, r->ep_re,
+                            r->ep_im, r->energy, path_up};
+  for (int c = 0; c < 12; ++c)
+    if (!cols[c]) return XRTK_EINVAL;
+  const size_t cb = sizeof(double) * nr;
+  if (ctx->stage.cap < 13 * cb) return XRTK_EINVAL;  // caller sizes the stage
+  char* S = ctx->stage.as<char>();
+  std::vector<double> tmp;
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < 13 && e == hipSuccess; ++c) {
+    double* d = reinterpret_cast<double*>(S + c * cb);
+    if (!cols[c]) {
+      e = launch_zero_f64(d, nr, ctx->stream);
+    } else if (perm) {
+      tmp.resize(nr);
+      for (uint32_t i = 0; i < nr; ++i) tmp[i] = cols[c][(*perm)[i]];
+      e = hipMemcpyAsync(d, tmp.data(), cb, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp is reused
+    } else {
+      e = hipMemcpyAsync(d, cols[c], cb, hipMemcpyHostToDevice, ctx->stream);
+    }
+  }
+  if (e == hipSuccess) e = launch_rays_from_cols(reinterpret_cast<const double*>(S), dst, nr,
+                                                 ctx->stream);
+  return map_err(e);
+}
+
+// The one-shot pipeline: group, route, upload, solve, copy back.
+int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip, double* out4,
+                  const XrtkOpts* opts) {
+  const uint32_t want = mode_from(opts);
+  const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
+  LaunchCfg c32 = choose_cfg(ctx, MODE_F32R, nr, np, opts);
+  LaunchCfg c64 = cfg64_for(c32);
+  if (want == MODE_F64) c64 = choose_cfg(ctx, MODE_F64, nr, np, opts);
+  if (!pair_cfg_supported(MODE_F32R, c32) || !pair_cfg_supported(MODE_F64, c64))
+    return XRTK_EINVAL;
+  // monochromatic groups for the reduced kernel
+  std::vector<std::vector<uint32_t>> groups;
+  bool broadband = false;
+  if (nr) {
+    groups = group_by_energy(nr, [&](size_t i) { return rs.energy(i); });
+    broadband = groups.size() > 64;
+  }
+  // pixel slots and the routing bound inputs
+  Box pbox;
+  double dq[3] = {0, 0, 0};
+  uint32_t npad;
+  if (ps.grid) {
+    npad = grid_slots(ps.grid->nx, ps.grid->nz, c32, c64);
+    pbox = grid_box(*ps.grid);
+    grid_dq(*ps.grid, c32.P, dq);
+  } else {
+    npad = padded_slots(np, c32, c64);
+    for (size_t i = 0; i < np; ++i) {
+      double v[3];
+      ps.get(i, v);
+      pbox.add(v[0], v[1], v[2]);
+    }
+    if (want == MODE_F32R && !broadband)
+      explicit_tile_dq(np, 64u * c32.P, [&](size_t i, double v[3]) { ps.get(i, v); }, dq);
+  }
+  // instances: one per energy group (F32R), routed by the residual bound;
+  // broadband or F64 requests: one FP64 instance with per-ray energies
+  struct G { uint32_t off, n, mode; double energy; };
+  std::vector<G> gs;
+  std::vector<uint32_t> perm;
+  if (nr && want == MODE_F32R && !broadband) {
+    uint32_t off = 0;
+    for (auto& gr : groups) {
+      Box rb;
+      for (uint32_t i : gr) {
+        double v[3];
+        rs.pos(i, v);
+        rb.add(v[0], v[1], v[2]);
+      }
+      const double e = rs.energy(gr[0]);
+      gs.push_back(G{off, (uint32_t)gr.size(), route_mode(MODE_F32R, rb, pbox, dq, e), e});
+      off += (uint32_t)gr.size();
+    }
+    if (groups.size() > 1)
+      for (auto& gr : groups) perm.insert(perm.end(), gr.begin(), gr.end());
+  } else if (nr) {
+    gs.push_back(G{0, nr, MODE_F64, 0.0});
+  }
+  // ---- buffers
+  const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
+                                  : align256(13 * sizeof(double) * (size_t)nr);
+  const size_t pix_stage = ps.grid ? 0 : (ps.f32 ? sizeof(XrtkGpuPixel) * np
+                                                 : 3 * sizeof(double) * (size_t)np);
+  hipError_t e = ctx->stage.ensure(ray_stage + pix_stage + 256);
+  if (e == hipSuccess) e = ctx->rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
+  if (e == hipSuccess) e = ctx->pix64.ensure(sizeof(PixF64) * npad);
+  if (e == hipSuccess) e = ctx->perm.ensure(sizeof(uint32_t) * npad);
+  if (e == hipSuccess) e = ctx->outbuf.ensure(sizeof(double) * 4 * np);
+  if (e != hipSuccess) return XRTK_ENOMEM;
+  char* S = ctx->stage.as<char>();
+  // ---- rays
+  if (rs.f32 && nr) {
+    std::vector<XrtkGpuRay> tmp;
+    const XrtkGpuRay* src = rs.f32;
+    if (!perm.empty()) {
+      tmp.reserve(nr);
+      for (uint32_t i : perm) tmp.push_back(rs.f32[i]);
+      src = tmp.data();
+    }
+    e = hipMemcpyAsync(S, src, sizeof(XrtkGpuRay) * nr, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ctx->rays64.as<RayRec64>(),
+                                  nr, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp goes out of scope
+    if (e != hipSuccess) return map_err(e);
+  } else if (nr) {
+    const int rc = upload_cols(ctx, rs.f64, nullptr, perm.empty() ? nullptr : &perm, nr,
+                               ctx->rays64.as<RayRec64>());
+    if (rc != XRTK_OK) return rc;
+  }
+  // ---- pixels
+  const uint32_t* d_perm = nullptr;
+  if (ps.grid) {
+    uint32_t th, tw;
+    tile_shape(c32.P, &th, &tw);
+    e = launch_pix_grid(ps.grid->center, ps.grid->dx, ps.grid->dz, ps.grid->nx, ps.grid->nz, th,
+                        tw, npad, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(), ctx->stream);
+    d_perm = ctx->perm.as<uint32_t>();
+  } else if (ps.f32) {
+    e = hipMemcpyAsync(S + ray_stage, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice,
+                       ctx->stream);
+    if (e == hipSuccess)
+      e = launch_pix_from_f32aos(S + ray_stage, ctx->pix64.as<PixF64>(), np, npad, ctx->stream);
+  } else {
+    double* P3 = reinterpret_cast<double*>(S + ray_stage);
+    const size_t cb = sizeof(double) * np;
+    e = hipMemcpyAsync(P3, ps.x, cb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(P3 + np, ps.y, cb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(P3 + 2 * np, ps.z, cb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_pix_from_cols(P3, ctx->pix64.as<PixF64>(), np, npad, ctx->stream);
+  }
+  if (e != hipSuccess) return map_err(e);
+  // ---- instances
+  std::vector<InstSpec> specs;
+  for (size_t gi = 0; gi < gs.size(); ++gi) {
+    InstSpec s;
+    s.mode = gs[gi].mode;
+    s.rays = ctx->rays64.as<RayRec64>() + gs[gi].off;
+    s.nr = gs[gi].n;
+    s.pix = ctx->pix64.as<PixF64>();
+    s.perm = d_perm;
+    s.np = np;
+    s.npad = npad;
+    s.energy = s.mode == MODE_F32R ? gs[gi].energy : 0.0;  // F64: per-ray energies
+    s.skip = skip;
+    s.out = ctx->outbuf.as<double>();
+    s.accumulate = gi > 0;
+    specs.push_back(s);
+  }
+  if (specs.empty()) {  // no rays: zeros
+    InstSpec s;
+    s.mode = want;
+    s.pix = ctx->pix64.as<PixF64>();
+    s.np = np;
+    s.npad = npad;
+    s.out = ctx->outbuf.as<double>();
+    specs.push_back(s);
+  }
   Plan plan;
   const int split = opts && opts->split > 0 ? opts->split : 0;
-  int rc = run_instances(ctx, mode, cfg, specs, ctx->work, ctx->meta, true, &plan, split, 0,
+  uint32_t force = 0;
+  if (const char* f = std::getenv("XRTK_FORCE_FLAGS")) force = (uint32_t)std::strtoul(f, nullptr, 0);
+  int rc = run_instances(ctx, c32, c64, specs, ctx->work, ctx->meta, true, &plan, split, force,
                          false);
   if (rc != XRTK_OK) return rc;
-  hipError_t e = hipMemcpyAsync(out4, ctx->outbuf.p, sizeof(double) * 4 * np,
-                                hipMemcpyDeviceToHost, ctx->stream);
+  e = hipMemcpyAsync(out4, ctx->outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
+                     ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return map_err(e);
   record_timing(ctx);
-  // report whether any instance took the near-singular path
-  plan.checked = 0;
   record_plan(ctx, plan, 1);
   return XRTK_OK;
 }
@@ -156,9 +390,13 @@ xrtk_ctx* xrtk_ctx_shared(void) {
     int dev = 0;
     if (const char* d = std::getenv("XRTK_DEVICE")) dev = std::atoi(d);
     xrtk_ctx* c = nullptr;
-    if (create_ctx(dev, &c) == XRTK_OK) {
-      c->shared = true;
-      shared = c;
+    try {
+      if (create_ctx(dev, &c) == XRTK_OK) {
+        c->shared = true;
+        shared = c;
+      }
+    } catch (...) {
+      shared = nullptr;
     }
   });
   return shared;
@@ -179,10 +417,11 @@ void xrtk_ctx_destroy(xrtk_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto* ge : ctx->graphs) {
     if (ge->exec) (void)hipGraphExecDestroy(ge->exec);
-    ge->work.release(); ge->meta.release(); ge->pix.release(); ge->outd.release();
+    ge->work.release(); ge->meta.release(); ge->pix.release(); ge->perm.release();
+    ge->outd.release();
     delete ge;
   }
-  ctx->stage.release(); ctx->rays64.release(); ctx->pix64.release();
+  ctx->stage.release(); ctx->rays64.release(); ctx->pix64.release(); ctx->perm.release();
   ctx->work.release(); ctx->meta.release(); ctx->outbuf.release();
   for (auto& kv : ctx->raysets) kv.second.rays.release();
   for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
@@ -201,71 +440,18 @@ int xrtk_kirchhoff_gpu_f32(xrtk_ctx* ctx, const XrtkGpuRay* rays, size_t n_rays,
                            const XrtkOpts* opts) {
   if (!ctx) return XRTK_ENODEV;
   if ((n_rays && !rays) || (n_pixels && (!pixels || !out4))) return XRTK_EINVAL;
-  if (n_rays > 0xffffffffull || n_pixels > 0x7fffffffull) return XRTK_EINVAL;
+  if (n_rays > 0xffffffffull || n_pixels > 0x7fff0000ull) return XRTK_EINVAL;
   if (n_pixels == 0) return XRTK_OK;
   try {
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
-    uint32_t mode = mode_from(opts);
-    const uint32_t nr = (uint32_t)n_rays, np = (uint32_t)n_pixels;
-    // monochromatic groups for the reduced kernel
-    std::vector<std::vector<uint32_t>> groups;
-    if (mode == MODE_F32R && nr) {
-      groups = group_by_energy(nr, [&](size_t i) { return (double)rays[i].energy; });
-      if (groups.size() > 64) mode = MODE_F64;  // broadband: plain FP64 path
-    }
-    std::vector<XrtkGpuRay> perm;
-    const XrtkGpuRay* src = rays;
-    if (mode == MODE_F32R && groups.size() > 1) {
-      perm.reserve(nr);
-      for (auto& gr : groups)
-        for (uint32_t i : gr) perm.push_back(rays[i]);
-      src = perm.data();
-    }
-    const size_t rb = align256(sizeof(XrtkGpuRay) * nr), pb = sizeof(XrtkGpuPixel) * np;
-    hipError_t e = ctx->stage.ensure(rb + pb);
-    if (e == hipSuccess) e = ctx->rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
-    if (e == hipSuccess) e = ctx->pix64.ensure(sizeof(PixF64) * np);
-    if (e == hipSuccess) e = ctx->outbuf.ensure(sizeof(double) * 4 * np);
-    if (e != hipSuccess) return XRTK_ENOMEM;
-    char* S = ctx->stage.as<char>();
-    if (nr) e = hipMemcpyAsync(S, src, sizeof(XrtkGpuRay) * nr, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(S + rb, pixels, pb, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && nr)
-      e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ctx->rays64.as<RayRec64>(),
-                                  nr, ctx->stream);
-    if (e == hipSuccess) e = launch_pix_from_f32aos(S + rb, ctx->pix64.as<PixF64>(), np, ctx->stream);
-    if (e != hipSuccess) return map_err(e);
-    std::vector<InstSpec> specs;
-    const double skip = 1e-20;  // kirchhoff.wgsl:85, fallback.rs:48
-    if (mode == MODE_F32R && nr) {
-      uint32_t off = 0;
-      for (size_t gi = 0; gi < groups.size(); ++gi) {
-        InstSpec s;
-        s.rays = ctx->rays64.as<RayRec64>() + off;
-        s.nr = (uint32_t)groups[gi].size();
-        s.pix = ctx->pix64.as<PixF64>();
-        s.np = np;
-        s.energy = (double)src[off].energy;
-        s.skip = skip;
-        s.out = ctx->outbuf.as<double>();
-        s.accumulate = gi > 0;
-        specs.push_back(s);
-        off += s.nr;
-      }
-    } else {
-      InstSpec s;
-      s.rays = ctx->rays64.as<RayRec64>();
-      s.nr = nr;
-      s.pix = ctx->pix64.as<PixF64>();
-      s.np = np;
-      s.energy = 0.0;  // per-ray energies
-      s.skip = skip;
-      s.out = ctx->outbuf.as<double>();
-      specs.push_back(s);
-    }
-    return finish_solve(ctx, mode, opts, specs, nr, np, out4);
+    RaySrc rs;
+    rs.f32 = rays;
+    rs.nr = n_rays;
+    PixSrc ps;
+    ps.f32 = pixels;
+    ps.np = n_pixels;
+    return solve_oneshot(ctx, rs, ps, 1e-20 /* kirchhoff.wgsl:85, fallback.rs:48 */, out4, opts);
   } catch (...) {
     return XRTK_ENOMEM;
   }
@@ -282,145 +468,68 @@ int xrtk_kirchhoff_auto_f32(const XrtkGpuRay* rays, size_t n_rays, const XrtkGpu
   }
   if (env_flag("XRTK_REQUIRE_GPU")) return XRTK_ENODEV;
   g_auto_backend.store(0);
-  return cpu_fallback_f32(rays, n_rays, pixels, n_pixels, out4);
-}
-
-int xrtk_auto_last_backend(void) { return g_auto_backend.load(); }
-
-static int upload_cols(xrtk_ctx* ctx, const XrtkRaysF64* r, const double* path_up,
-                       const std::vector<uint32_t>* perm, uint32_t nr, RayRec64* dst) {
-  if (!nr) return XRTK_OK;
-  const double* cols[13] = {r->x,     r->y,     r->z,     r->nx,    r->ny,
-                            r->nz,    r->nl,    r->es_re, r->es_im, r->ep_re,
-                            r->ep_im, r->energy, path_up};
-  for (int c = 0; c < 12; ++c)
-    if (!cols[c]) return XRTK_EINVAL;
-  const size_t cb = sizeof(double) * nr;
-  hipError_t e = ctx->stage.ensure(13 * cb);
-  if (e != hipSuccess) return XRTK_ENOMEM;
-  char* S = ctx->stage.as<char>();
-  std::vector<double> tmp;
-  for (int c = 0; c < 13; ++c) {
-    double* d = reinterpret_cast<double*>(S + c * cb);
-    if (!cols[c]) {
-      e = launch_zero_f64(d, nr, ctx->stream);
-    } else if (perm) {
-      tmp.resize(nr);
-      for (uint32_t i = 0; i < nr; ++i) tmp[i] = cols[c][(*perm)[i]];
-      e = hipMemcpyAsync(d, tmp.data(), cb, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp is reused
-    } else {
-      e = hipMemcpyAsync(d, cols[c], cb, hipMemcpyHostToDevice, ctx->stream);
-    }
-    if (e != hipSuccess) return map_err(e);
-  }
-  e = launch_rays_from_cols(reinterpret_cast<const double*>(S), dst, nr, ctx->stream);
-  return map_err(e);
-}
-
-static int diffraction_common(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
-                              const double* px, const double* py, const double* pz,
-                              const XrtkWaveParams* grid, size_t n_pixels, double* out4,
-                              const XrtkOpts* opts) {
-  if (!ctx) return XRTK_ENODEV;
-  if ((n_rays && !rays) || (n_pixels && !out4)) return XRTK_EINVAL;
-  if (!grid && n_pixels && (!px || !py || !pz)) return XRTK_EINVAL;
-  if (n_rays > 0xffffffffull || n_pixels > 0x7fffffffull) return XRTK_EINVAL;
-  if (n_pixels == 0) return XRTK_OK;
   try {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    DeviceGuard g(ctx->device);
-    uint32_t mode = mode_from(opts);
-    const uint32_t nr = (uint32_t)n_rays, np = (uint32_t)n_pixels;
-    std::vector<std::vector<uint32_t>> groups;
-    std::vector<uint32_t> perm;
-    if (mode == MODE_F32R && nr) {
-      if (!rays->energy) return XRTK_EINVAL;
-      groups = group_by_energy(nr, [&](size_t i) { return rays->energy[i]; });
-      if (groups.size() > 64) mode = MODE_F64;
-      else if (groups.size() > 1)
-        for (auto& gr : groups) perm.insert(perm.end(), gr.begin(), gr.end());
-    }
-    // stage: 13 ray columns, then 3 pixel columns (sized once: a reallocation
-    // would free memory that queued kernels still read)
-    const size_t rb = align256(13 * sizeof(double) * (size_t)nr);
-    hipError_t e = ctx->stage.ensure(rb + 3 * sizeof(double) * np);
-    if (e == hipSuccess) e = ctx->rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
-    if (e == hipSuccess) e = ctx->pix64.ensure(sizeof(PixF64) * np);
-    if (e == hipSuccess) e = ctx->outbuf.ensure(sizeof(double) * 4 * np);
-    if (e != hipSuccess) return XRTK_ENOMEM;
-    int rc = upload_cols(ctx, rays, nullptr, perm.empty() ? nullptr : &perm, nr,
-                         ctx->rays64.as<RayRec64>());
-    if (rc != XRTK_OK) return rc;
-    if (grid) {
-      e = launch_pix_grid(grid->center, grid->dx, grid->dz, grid->nx, grid->nz,
-                          ctx->pix64.as<PixF64>(), ctx->stream);
-    } else {
-      char* S = ctx->stage.as<char>() + rb;
-      const size_t cb = sizeof(double) * np;
-      e = hipMemcpyAsync(S, px, cb, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess) e = hipMemcpyAsync(S + cb, py, cb, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(S + 2 * cb, pz, cb, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess)
-        e = launch_pix_from_cols(reinterpret_cast<const double*>(S), ctx->pix64.as<PixF64>(), np,
-                                 ctx->stream);
-    }
-    if (e != hipSuccess) return map_err(e);
-    std::vector<InstSpec> specs;
-    const double skip = 1e-30;  // diffraction.rs:81
-    if (mode == MODE_F32R && nr) {
-      uint32_t off = 0;
-      for (size_t gi = 0; gi < groups.size(); ++gi) {
-        InstSpec s;
-        s.rays = ctx->rays64.as<RayRec64>() + off;
-        s.nr = (uint32_t)groups[gi].size();
-        s.pix = ctx->pix64.as<PixF64>();
-        s.np = np;
-        s.energy = rays->energy[groups[gi][0]];
-        s.skip = skip;
-        s.out = ctx->outbuf.as<double>();
-        s.accumulate = gi > 0;
-        specs.push_back(s);
-        off += s.nr;
-      }
-    } else {
-      InstSpec s;
-      s.rays = ctx->rays64.as<RayRec64>();
-      s.nr = nr;
-      s.pix = ctx->pix64.as<PixF64>();
-      s.np = np;
-      s.energy = 0.0;
-      s.skip = skip;
-      s.out = ctx->outbuf.as<double>();
-      specs.push_back(s);
-    }
-    return finish_solve(ctx, mode, opts, specs, nr, np, out4);
+    return cpu_fallback_f32(rays, n_rays, pixels, n_pixels, out4);
   } catch (...) {
     return XRTK_ENOMEM;
   }
 }
 
+int xrtk_auto_last_backend(void) { return g_auto_backend.load(); }
+
 int xrtk_diffraction_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
                          const double* pix_x, const double* pix_y, const double* pix_z,
                          size_t n_pixels, double* out4, const XrtkOpts* opts) {
-  return diffraction_common(ctx, rays, n_rays, pix_x, pix_y, pix_z, nullptr, n_pixels, out4,
-                            opts);
+  if (!ctx) return XRTK_ENODEV;
+  if ((n_rays && !rays) || (n_pixels && (!out4 || !pix_x || !pix_y || !pix_z)))
+    return XRTK_EINVAL;
+  if (n_rays && (!rays->x || !rays->y || !rays->z || !rays->energy)) return XRTK_EINVAL;
+  if (n_rays > 0xffffffffull || n_pixels > 0x7fff0000ull) return XRTK_EINVAL;
+  if (n_pixels == 0) return XRTK_OK;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    RaySrc rs;
+    rs.f64 = rays;
+    rs.nr = n_rays;
+    PixSrc ps;
+    ps.x = pix_x; ps.y = pix_y; ps.z = pix_z;
+    ps.np = n_pixels;
+    return solve_oneshot(ctx, rs, ps, 1e-30 /* diffraction.rs:81 */, out4, opts);
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
 }
 
 int xrtk_diffraction_grid_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
                               const XrtkWaveParams* screen, double* out4, const XrtkOpts* opts) {
-  if (!screen) return XRTK_EINVAL;
+  if (!ctx) return XRTK_ENODEV;
+  if (!screen || (n_rays && !rays)) return XRTK_EINVAL;
+  if (n_rays && (!rays->x || !rays->y || !rays->z || !rays->energy)) return XRTK_EINVAL;
   const size_t np = (size_t)screen->nx * screen->nz;
-  return diffraction_common(ctx, rays, n_rays, nullptr, nullptr, nullptr, screen, np, out4,
-                            opts);
+  if (np && !out4) return XRTK_EINVAL;
+  if (n_rays > 0xffffffffull || np > 0x7fff0000ull) return XRTK_EINVAL;
+  if (np == 0) return XRTK_OK;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    RaySrc rs;
+    rs.f64 = rays;
+    rs.nr = n_rays;
+    PixSrc ps;
+    ps.grid = screen;
+    ps.np = np;
+    return solve_oneshot(ctx, rs, ps, 1e-30, out4, opts);
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
 }
 
 int xrtk_prepare_wave(const XrtkWaveParams* w, double* px, double* py, double* pz) {
   if (!w) return XRTK_EINVAL;
   const size_t n = (size_t)w->nx * w->nz;
   if (n && (!px || !py || !pz)) return XRTK_EINVAL;
-  // prepare_wave.rs:43-65, same operation order (no contraction: see Makefile)
+  // prepare_wave.rs:43-65, same operation order (built without contraction)
   size_t k = 0;
   for (uint32_t ix = 0; ix < w->nx; ++ix) {
     const double fx = w->nx > 1 ? (double)ix / (double)(w->nx - 1) - 0.5 : 0.0;
@@ -448,8 +557,10 @@ int xrtk_beam_to_rays(const double* bx, const double* by, const double* bz, cons
     if (n_good && !out_cols[c]) return XRTK_EINVAL;
   const bool amps = es_re && es_im && ep_re && ep_im;
   for (size_t k = 0; k < n_good; ++k) {
+    if (good[k] >= n_beam) return XRTK_EINVAL;
+  }
+  for (size_t k = 0; k < n_good; ++k) {
     const size_t i = good[k];
-    if (i >= n_beam) return XRTK_EINVAL;
     const double* n = normals3 + 3 * i;
     // prepare_wave.rs:86-87: nl = n . (a, b, c)
     const double nl = n[0] * ba[i] + n[1] * bb[i] + n[2] * bc[i];
@@ -477,12 +588,15 @@ int xrtk_rayset_upload(xrtk_ctx* ctx, const XrtkRaysF64* rays, const double* pat
     rs.nr = (uint32_t)n_rays;
     rs.has_path = path_up != nullptr;
     if (n_rays) {
-      if (!rays->energy) return XRTK_EINVAL;
+      if (!rays->x || !rays->y || !rays->z || !rays->energy) return XRTK_EINVAL;
       rs.energy = rays->energy[0];
-      for (size_t i = 1; i < n_rays; ++i)
-        if (rays->energy[i] != rs.energy) { rs.mono = false; break; }
+      for (size_t i = 0; i < n_rays; ++i) {
+        if (rays->energy[i] != rs.energy) rs.mono = false;
+        rs.box.add(rays->x[i], rays->y[i], rays->z[i]);
+      }
     }
-    if (rs.rays.ensure(sizeof(RayRec64) * std::max<size_t>(n_rays, 1)) != hipSuccess)
+    if (ctx->stage.ensure(13 * sizeof(double) * std::max<size_t>(n_rays, 1) + 256) != hipSuccess ||
+        rs.rays.ensure(sizeof(RayRec64) * std::max<size_t>(n_rays, 1)) != hipSuccess)
       return XRTK_ENOMEM;
     int rc = upload_cols(ctx, rays, path_up, nullptr, rs.nr, rs.rays.as<RayRec64>());
     if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
@@ -506,6 +620,14 @@ int xrtk_rayset_release(xrtk_ctx* ctx, xrtk_rayset rs) {
   if (it == ctx->raysets.end()) return XRTK_EINVAL;
   DeviceGuard g(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  // cached batch graphs may reference this ray set: drop them
+  for (auto*& ge : ctx->graphs) {
+    if (ge->exec) (void)hipGraphExecDestroy(ge->exec);
+    ge->work.release(); ge->meta.release(); ge->pix.release(); ge->perm.release();
+    ge->outd.release();
+    delete ge;
+  }
+  ctx->graphs.clear();
   it->second.rays.release();
   ctx->raysets.erase(it);
   return XRTK_OK;

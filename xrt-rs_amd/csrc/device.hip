@@ -4,18 +4,24 @@
 //   U = i k/(4 pi) (nl + n.d/path) exp(i k path) / path,  Es += es U, Ep += ep U
 // evaluated two ways:
 //   * k_pair_f64   plain FP64 with Kahan sums, the operation order of
-//                  diffraction.rs:76-110 (the FP64 device oracle);
+//                  diffraction.rs:76-110 (the FP64 device oracle and the FP64
+//                  speed baseline);
 //   * k_pair_f32r  FP32 with the phase reduced against the FP64 ray centroid
-//                  c (the method of src/gpu/kirchhoff.rs:96-104,276-297 and
-//                  src/gpu/shaders/kirchhoff.wgsl:95-113), redesigned: the
-//                  offsets o = r - c and q = p - c are formed in FP64 and
-//                  rounded once, the phase is kept in revolutions for the
-//                  hardware v_sin/v_cos, FP32 partial sums are folded into
-//                  FP64 every `tile` rays, and an optional quadratic fold
-//                  moves the per-ray term k|o|^2/(2 R0) into the amplitudes.
-// Work is split into items (instance, pixel block, ray chunk); each chunk's
-// FP64 partial goes to HBM and k_merge sums the chunks in a fixed order with
-// TwoSum, then applies k/(4 pi) exp(i k |p - c|) in FP64.
+//                  c — the method of src/gpu/kirchhoff.rs:96-104,276-297 and
+//                  src/gpu/shaders/kirchhoff.wgsl:95-113, redesigned:
+//                  - o = r - c and q = p - c are formed in FP64, rounded once;
+//                  - phases are in revolutions for the hardware v_sin/v_cos;
+//                  - |p - r|^2 = |q|^2 + kr^-1 * kr(|o|^2 - 2 o.q) reuses the
+//                    phase numerator (no difference vector per pair);
+//                  - TILE variant: each wave's 64*P pixels are one tile with
+//                    an FP64 reference point q_t; the per-(ray, tile) phase
+//                    kr(|o|^2 - 2 o.q_t)/(2|q_t|) is folded into the ray
+//                    amplitudes in FP64 (lane-parallel, staged in LDS), so the
+//                    FP32 pair phase only carries the small in-tile residual;
+//                  - FP32 partial sums are folded into FP64 every `tile` rays.
+// Work is split into items (instance, block of pixel slots, ray chunk); each
+// chunk's FP64 partial goes to HBM and k_merge sums the chunks in a fixed
+// order with TwoSum, then applies k/(4 pi) exp(i k |p - c|) in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,6 +78,19 @@ __device__ __forceinline__ void cis_rev_dd(double a_hi, double a_lo, double x,
   sincospi(2.0 * f, s, c);
 }
 
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // staging: caller layouts -> resident f64 records
 // ---------------------------------------------------------------------------
@@ -108,36 +127,49 @@ __global__ void k_rays_from_cols(const double* __restrict__ cols,
   dst[i] = r;
 }
 
+// Explicit pixel lists keep their order; slots [n, npad) repeat pixel n-1.
 __global__ void k_pix_from_f32aos(const float4* __restrict__ src,
-                                  PixF64* __restrict__ dst, uint32_t n) {
+                                  PixF64* __restrict__ dst, uint32_t n, uint32_t npad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = src[i];
+  if (i >= npad) return;
+  const float4 p = src[i < n ? i : n - 1u];
   dst[i] = PixF64{(double)p.x, (double)p.y, (double)p.z, 0.0};
 }
 
 __global__ void k_pix_from_cols(const double* __restrict__ cols,
-                                PixF64* __restrict__ dst, uint32_t n) {
+                                PixF64* __restrict__ dst, uint32_t n, uint32_t npad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= npad) return;
+  const uint32_t j = i < n ? i : n - 1u;
   const size_t N = n;
-  dst[i] = PixF64{cols[i], cols[N + i], cols[2 * N + i], 0.0};
+  dst[i] = PixF64{cols[j], cols[N + j], cols[2 * N + j], 0.0};
 }
 
 // prepare_wave (src/waves/prepare_wave.rs:40-68), evaluated in the same
-// operation order without contraction so the points equal the host's.
-__global__ void k_pix_grid(double cx, double cy, double cz, double dx,
-                           double dz, uint32_t nx, uint32_t nz,
-                           PixF64* __restrict__ dst) {
+// operation order without contraction so the points equal the host's, and
+// laid out tile by tile: tiles of th x tw pixels (th rows of ix, tw columns of
+// iz), tiles in row-major order over (ix, iz).  perm[slot] is the
+// prepare_wave index ix*nz + iz, or ~0u for a padding slot, which repeats the
+// nearest pixel of the grid (so a tile's mean stays inside the tile).
+__global__ void k_pix_grid(double cx, double cy, double cz, double dx, double dz, uint32_t nx,
+                           uint32_t nz, uint32_t th, uint32_t tw, uint32_t npad,
+                           PixF64* __restrict__ dst, uint32_t* __restrict__ perm) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nx * nz) return;
-  const uint32_t ix = i / nz, iz = i - ix * nz;
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  const uint32_t tsz = th * tw;
+  const uint32_t ntz = (nz + tw - 1) / tw;
+  const uint32_t t = s / tsz, r = s - t * tsz;
+  const uint32_t tix = t / ntz, tiz = t - tix * ntz;
+  uint32_t ix = tix * th + r / tw, iz = tiz * tw + r % tw;
+  const bool real = ix < nx && iz < nz;
+  if (!real) { ix = min(ix, nx - 1u); iz = min(iz, nz - 1u); }
   const double fx = nx > 1 ? (double)ix / (double)(nx - 1) - 0.5 : 0.0;
   const double fz = nz > 1 ? (double)iz / (double)(nz - 1) - 0.5 : 0.0;
   const double px = cx + fx * 2.0 * dx;
   const double pz = cz + fz * 2.0 * dz;
-  dst[i] = PixF64{px, cy, pz, 0.0};
+  dst[s] = PixF64{px, cy, pz, 0.0};
+  perm[s] = real ? ix * nz + iz : ~0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -173,7 +205,7 @@ __global__ __launch_bounds__(256) void k_centroid_partial(
 // Also resets the atomically reduced bounds of the instance.
 __global__ __launch_bounds__(256) void k_centroid_final(
     const double* __restrict__ part, uint32_t nblk, uint32_t n,
-    InstScratch* __restrict__ scr) {
+    const RayRec64* __restrict__ rays, InstScratch* __restrict__ scr) {
   __shared__ double sx[256], sy[256], sz[256];
   double ax = 0, ay = 0, az = 0;
   for (uint32_t b = threadIdx.x; b < nblk; b += 256) {
@@ -192,63 +224,127 @@ __global__ __launch_bounds__(256) void k_centroid_final(
   if (threadIdx.x == 0) {
     const double inv = n ? 1.0 / (double)n : 0.0;
     scr->cx = sx[0] * inv; scr->cy = sy[0] * inv; scr->cz = sz[0] * inv;
-    scr->R0 = 0.0;
-    scr->omax_bits = 0u;
+    scr->pad0 = 0.0;
+    for (int k = 0; k < 4; ++k) scr->omax_bits[k] = 0u;
+    for (int k = 0; k < 3; ++k) scr->qmax_bits[k] = scr->dqmax_bits[k] = 0u;
     scr->rmin_bits = 0x7f800000u;  // +inf
     scr->rmax_bits = 0u;
     scr->flags = 0u;
     scr->ep_nonzero = 0u;
+    scr->n_differs = 0u;
+    scr->n0x = n ? rays[0].nx : 0.0;
+    scr->n0y = n ? rays[0].ny : 0.0;
+    scr->n0z = n ? rays[0].nz : 0.0;
+    scr->v0_rev = scr->v2_rev = 0.0;
   }
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_min(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// max |r - c| over the rays (rounded up to float) and whether any Ep != 0.
+// Per-axis max |o| (o rounded to FP32 as the pair kernel sees it), max |o|,
+// whether any Ep != 0 and whether any normal differs from ray 0's.
 __global__ __launch_bounds__(256) void k_ray_bounds(
     const RayRec64* __restrict__ rays, uint32_t n, InstScratch* __restrict__ scr) {
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
-  float m = 0.f;
-  uint32_t ep = 0u;
+  const double n0x = scr->n0x, n0y = scr->n0y, n0z = scr->n0z;
+  float mx = 0.f, my = 0.f, mz = 0.f, mo = 0.f;
+  uint32_t ep = 0u, nd = 0u;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const double ox = rays[i].x - cx, oy = rays[i].y - cy, oz = rays[i].z - cz;
-    const float d = __double2float_ru(sqrt(ox * ox + oy * oy + oz * oz) * (1.0 + 1e-12));
-    m = fmaxf(m, d);
-    ep |= (rays[i].ep_re != 0.0 || rays[i].ep_im != 0.0) ? 1u : 0u;
+    const RayRec64 r = rays[i];
+    const float ox = (float)(r.x - cx), oy = (float)(r.y - cy), oz = (float)(r.z - cz);
+    mx = fmaxf(mx, fabsf(ox)); my = fmaxf(my, fabsf(oy)); mz = fmaxf(mz, fabsf(oz));
+    const double o2 = (double)ox * ox + (double)oy * oy + (double)oz * oz;
+    mo = fmaxf(mo, __double2float_ru(sqrt(o2)));
+    ep |= (r.ep_re != 0.0 || r.ep_im != 0.0) ? 1u : 0u;
+    nd |= (r.nx != n0x || r.ny != n0y || r.nz != n0z) ? 1u : 0u;
   }
-  m = wave_max(m);
-  const uint32_t epw = __any(ep) ? 1u : 0u;
+  mx = wave_max(mx); my = wave_max(my); mz = wave_max(mz); mo = wave_max(mo);
+  const bool epw = __any(ep), ndw = __any(nd);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&scr->omax_bits, __float_as_uint(m));
+    atomicMax(&scr->omax_bits[0], __float_as_uint(mx));
+    atomicMax(&scr->omax_bits[1], __float_as_uint(my));
+    atomicMax(&scr->omax_bits[2], __float_as_uint(mz));
+    atomicMax(&scr->omax_bits[3], __float_as_uint(mo));
     if (epw) atomicOr(&scr->ep_nonzero, 1u);
+    if (ndw) atomicOr(&scr->n_differs, 1u);
   }
 }
 
-// q = p - c and R = |q| in FP64, the FP32 pixel record, and min/max R.
-__global__ __launch_bounds__(256) void k_pix_prep(
-    const PixF64* __restrict__ pix, uint32_t n, double kr,
-    InstScratch* __restrict__ scr, PixRec32* __restrict__ out32,
-    double* __restrict__ R64) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per tile of tsz = 64 * P slots (P slots per lane, slot = t*tsz +
+// i*64 + lane): the tile reference point (mean q, FP64), the FP32 slot
+// records, R per slot in FP64, and the bounds of |q|, |dq| and |q|.
+template <int P>
+__global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix, double kr,
+                                                  InstScratch* __restrict__ scr,
+                                                  PixRec32* __restrict__ out32,
+                                                  TileRec* __restrict__ tiles,
+                                                  double* __restrict__ R64) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t base = blockIdx.x * 64u * P;
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
+  const bool uni = scr->n_differs == 0u;
+  const double n0x = scr->n0x, n0y = scr->n0y, n0z = scr->n0z;
+  double qx[P], qy[P], qz[P];
+  double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const PixF64 p = pix[base + i * 64u + lane];
+    qx[i] = p.x - cx; qy[i] = p.y - cy; qz[i] = p.z - cz;
+    sx += qx[i]; sy += qy[i]; sz += qz[i];
+  }
+  const double inv = 1.0 / (64.0 * P);
+  const double tx = wave_sum(sx) * inv, ty = wave_sum(sy) * inv, tz = wave_sum(sz) * inv;
+  const double Rt = sqrt(tx * tx + ty * ty + tz * tz);
+  if (lane == 0) tiles[blockIdx.x] = TileRec{tx, ty, tz, Rt};
+  float aqx = 0.f, aqy = 0.f, aqz = 0.f, adx = 0.f, ady = 0.f, adz = 0.f;
+  float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const double R2 = qx[i] * qx[i] + qy[i] * qy[i] + qz[i] * qz[i];
+    const double R = sqrt(R2);
+    const double dqx = qx[i] - tx, dqy = qy[i] - ty, dqz = qz[i] - tz;
+    PixRec32 r;
+    r.mx = (float)(-2.0 * kr * dqx);
+    r.my = (float)(-2.0 * kr * dqy);
+    r.mz = (float)(-2.0 * kr * dqz);
+    r.R2 = (float)R2;
+    r.R = (float)R;
+    r.g = (float)((Rt - R) / Rt);
+    r.nq = uni ? (float)(n0x * qx[i] + n0y * qy[i] + n0z * qz[i]) : 0.f;
+    r.pad = 0.f;
+    const uint32_t s = base + i * 64u + lane;
+    out32[s] = r;
+    R64[s] = R;
+    aqx = fmaxf(aqx, __double2float_ru(fabs(qx[i])));
+    aqy = fmaxf(aqy, __double2float_ru(fabs(qy[i])));
+    aqz = fmaxf(aqz, __double2float_ru(fabs(qz[i])));
+    adx = fmaxf(adx, __double2float_ru(fabs(dqx)));
+    ady = fmaxf(ady, __double2float_ru(fabs(dqy)));
+    adz = fmaxf(adz, __double2float_ru(fabs(dqz)));
+    rmin = fminf(rmin, __double2float_rd(R));
+    rmax = fmaxf(rmax, __double2float_ru(R));
+  }
+  aqx = wave_max(aqx); aqy = wave_max(aqy); aqz = wave_max(aqz);
+  adx = wave_max(adx); ady = wave_max(ady); adz = wave_max(adz);
+  rmin = wave_min(rmin); rmax = wave_max(rmax);
+  if (lane == 0) {
+    atomicMax(&scr->qmax_bits[0], __float_as_uint(aqx));
+    atomicMax(&scr->qmax_bits[1], __float_as_uint(aqy));
+    atomicMax(&scr->qmax_bits[2], __float_as_uint(aqz));
+    atomicMax(&scr->dqmax_bits[0], __float_as_uint(adx));
+    atomicMax(&scr->dqmax_bits[1], __float_as_uint(ady));
+    atomicMax(&scr->dqmax_bits[2], __float_as_uint(adz));
+    atomicMin(&scr->rmin_bits, __float_as_uint(rmin));
+    atomicMax(&scr->rmax_bits, __float_as_uint(rmax));
+  }
+}
+
+// F64 instances only need min/max |q| (for the skip bound).
+__global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ pix, uint32_t n,
+                                                    InstScratch* __restrict__ scr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
   if (i < n) {
-    const double qx = pix[i].x - cx, qy = pix[i].y - cy, qz = pix[i].z - cz;
+    const double qx = pix[i].x - scr->cx, qy = pix[i].y - scr->cy, qz = pix[i].z - scr->cz;
     const double R = sqrt(qx * qx + qy * qy + qz * qz);
-    PixRec32 r;
-    r.qx = (float)qx; r.qy = (float)qy; r.qz = (float)qz; r.R = (float)R;
-    r.mx = (float)(-2.0 * kr * qx);
-    r.my = (float)(-2.0 * kr * qy);
-    r.mz = (float)(-2.0 * kr * qz);
-    r.pad = 0.f;
-    if (out32) out32[i] = r;
-    R64[i] = R;
     rmin = __double2float_rd(R);
     rmax = __double2float_ru(R);
   }
@@ -260,209 +356,246 @@ __global__ __launch_bounds__(256) void k_pix_prep(
   }
 }
 
-// Decides the per-instance flags from the device-computed bounds.
-//  CHECK: a pair could have path below the skip threshold (or be close enough
-//         to singular that FP32 loses the offset), so the skip test runs.
-//  QFOLD: the reduced phase would exceed qfold_rev revolutions, so the
-//         per-ray quadratic term is folded into the amplitudes.
-__global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr,
-                       uint32_t mode, double qfold_rev, uint32_t force_flags,
-                       uint32_t static_flags) {
-  const double omax = (double)__uint_as_float(scr->omax_bits);
-  const double rmin = (double)__uint_as_float(scr->rmin_bits);
-  const double rmax = (double)__uint_as_float(scr->rmax_bits);
+// Per-instance flags from the device-computed bounds.
+//   F64:   CHECK when some pair may fall below the skip threshold
+//          (min |q| - max |o| <= skip).
+//   F32R:  the bound of the centroid-referenced phase,
+//            v0 = kr (|o|^2 + 2 sum_i |o_i| |q_i|) / (2 min|q| - |o|),
+//          and of the tile-referenced residual,
+//            v2 = kr (2 sum_i |o_i| |dq_i| + |o|^3 / (2 min|q| - |o|)) / (...)
+//          TILE is used unless v0 is already small; NOFRACT when the phase
+//          the pair kernel sees stays well inside v_sin's +-256 rev domain.
+// force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK.
+__global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
+                       uint32_t force_flags, uint32_t static_flags) {
+  const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
+  const double oz = __uint_as_float(scr->omax_bits[2]), om = __uint_as_float(scr->omax_bits[3]);
+  const double qx = __uint_as_float(scr->qmax_bits[0]), qy = __uint_as_float(scr->qmax_bits[1]);
+  const double qz = __uint_as_float(scr->qmax_bits[2]);
+  const double dx = __uint_as_float(scr->dqmax_bits[0]), dy = __uint_as_float(scr->dqmax_bits[1]);
+  const double dz = __uint_as_float(scr->dqmax_bits[2]);
+  const double rmin = __uint_as_float(scr->rmin_bits);
   uint32_t f = static_flags;
-  const double slack = mode == MODE_F32R ? 1e-5 * (rmax + omax) : 0.0;
-  if (!(rmin - omax > skip * (1.0 + 1e-9) + slack)) f |= INST_F_CHECK;
-  if (mode == MODE_F32R && rmin > 0.0 && kr * omax * omax / (2.0 * rmin) > qfold_rev)
-    f |= INST_F_QFOLD;
-  if (force_flags & 0x100u) f = (f & ~INST_F_QFOLD) | (force_flags & INST_F_QFOLD);
+  if (scr->n_differs == 0u) f |= INST_F_UNIFORM_N;
+  if (!(rmin - om > skip * (1.0 + 1e-9))) f |= INST_F_CHECK;
+  const double den = 2.0 * rmin - om;
+  double v0 = 1e300, v2 = 1e300;
+  if (den > 0.0) {
+    v0 = kr * (om * om + 2.0 * (ox * qx + oy * qy + oz * qz)) / den;
+    v2 = kr * (2.0 * (ox * dx + oy * dy + oz * dz) + om * om * (om / den)) / den;
+  }
+  if (mode == MODE_F32R) {
+    bool tile = v0 > 2.0;
+    if (force_flags & 0x100u) tile = (force_flags & INST_F_TILE) != 0u;
+    if (tile) f |= INST_F_TILE;
+    if ((tile ? v2 : v0) < 128.0) f |= INST_F_NOFRACT;
+  }
   if (force_flags & 0x200u) f |= INST_F_CHECK;
   scr->flags = f;
-  scr->R0 = rmin > 0.0 ? rmin : 1.0;
+  scr->v0_rev = v0;
+  scr->v2_rev = v2;
 }
 
 // FP32 ray records of the phase-reduced kernel.
-__global__ __launch_bounds__(256) void k_ray_prep32(
-    const RayRec64* __restrict__ rays, uint32_t n, const InstScratch* __restrict__ scr,
-    double kr, double kr_lo, RayRec32* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__ rays,
+                                                    uint32_t n,
+                                                    const InstScratch* __restrict__ scr,
+                                                    double kr, double kr_lo,
+                                                    RayRec32* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
   const uint32_t flags = scr->flags;
   const RayRec64 r = rays[i];
-  const double ox = r.x - cx, oy = r.y - cy, oz = r.z - cz;
-  const double oo = ox * ox + oy * oy + oz * oz;
+  const float ox = (float)(r.x - cx), oy = (float)(r.y - cy), oz = (float)(r.z - cz);
+  const double oo = (double)ox * ox + (double)oy * oy + (double)oz * oz;
   double er = r.es_re, ei = r.es_im, pr = r.ep_re, pi = r.ep_im;
-  // Phase carried by the amplitude, in revolutions.
-  double rev = 0.0;
-  if (flags & INST_F_QFOLD) rev += kr * oo / (2.0 * scr->R0);
   if (flags & INST_F_PATHUP) {
     const double hi = kr * r.path_up;
     const double lo = fma(kr, r.path_up, -hi) + kr_lo * r.path_up;
-    rev += (hi - rint(hi)) + lo;
-  }
-  if (rev != 0.0) {
     double s, c;
-    sincospi(2.0 * rev, &s, &c);
+    sincospi(2.0 * ((hi - rint(hi)) + lo), &s, &c);
     const double er2 = er * c - ei * s, ei2 = er * s + ei * c;
     const double pr2 = pr * c - pi * s, pi2 = pr * s + pi * c;
     er = er2; ei = ei2; pr = pr2; pi = pi2;
   }
   RayRec32 o;
-  o.ox = (float)ox; o.oy = (float)oy; o.oz = (float)oz;
+  o.ox = ox; o.oy = oy; o.oz = oz;
   o.oo = (float)(kr * oo);
-  o.nx = (float)r.nx; o.ny = (float)r.ny; o.nz = (float)r.nz; o.nl = (float)r.nl;
+  // n' = -n / (2 kr): n . d = n' . m + n . (q_t - o) with m = -2 kr dq
+  const double nf = -0.5 / kr;
+  o.nx = (float)(r.nx * nf); o.ny = (float)(r.ny * nf); o.nz = (float)(r.nz * nf);
+  o.nl = (float)r.nl;
   // i * Es, i * Ep
   o.ar = (float)(-ei); o.ai = (float)er;
   o.br = (float)(-pi); o.bi = (float)pr;
+  o.no = (float)(r.nx * (double)ox + r.ny * (double)oy + r.nz * (double)oz);
+  o.pad0 = o.pad1 = o.pad2 = 0.f;
   out[i] = o;
 }
 
 // ---------------------------------------------------------------------------
 // FP32 phase-reduced pair kernel
 // ---------------------------------------------------------------------------
-struct SmemRays {
-  const XRTK_CONST RayRec32* p;
-  __device__ __forceinline__ RayRec32 get(uint32_t j) const { return p[j]; }
-  static constexpr bool kScalar = true;
+// Per-(ray, tile) values staged in LDS by the TILE variant.
+struct TileRay {
+  float B;       // kr (|o|^2 - 2 o.q_t)
+  float ar, ai;  // i Es rotated by exp(i 2 pi kr (|o|^2 - 2 o.q_t)/(2 R_t))
+  float br, bi;  // i Ep rotated likewise
+  float nqt;     // n . (q_t - o)
+  float pad0, pad1;
 };
-struct LdsRays {
-  const RayRec32* p;
-  __device__ __forceinline__ RayRec32 get(uint32_t j) const { return p[j]; }
-  static constexpr bool kScalar = false;
-};
+static_assert(sizeof(TileRay) == 32, "TileRay is 32 B");
+
+static constexpr uint32_t kBlockRays = 64;  // rays staged per LDS block (one per lane)
 
 template <int P>
-struct PixRegs {
-  float qx[P], qy[P], qz[P], R[P], mx[P], my[P], mz[P], g[P];
+struct SlotRegs {
+  float mx[P], my[P], mz[P], R2[P], R[P], g[P], nq[P];
 };
 
-// Accumulates rays [j0, j1) of `src` into a[P][4] (FP32).
-template <int P, bool CHECK, bool QFOLD, class Src>
-__device__ __forceinline__ void pair_span(const Src& src, uint32_t j0, uint32_t j1,
-                                          const PixRegs<P>& px, float h,
-                                          float skip2, float (&a)[P][4]) {
-#pragma unroll 2
-  for (uint32_t j = j0; j < j1; ++j) {
-    const RayRec32 r = src.get(j);
-    float oo = r.oo;
-    if constexpr (Src::kScalar && !QFOLD) oo = vcopy(oo);
+// Accumulates one ray into a[P][4] for the P slots of this lane.  TILE: B and
+// the rotated amplitudes come from the per-(ray, tile) LDS record.
+template <int P, bool TILE, bool UNIN, bool NOFRACT>
+__device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float tar, float tai,
+                                         float tbr, float tbi, float nqt, const SlotRegs<P>& s,
+                                         float ikr, float h, float (&a)[P][4]) {
+  float ar = r.ar, ai = r.ai, br = r.br, bi = r.bi;
+  float oo = r.oo, mno = -r.no;
+  if constexpr (TILE) {
+    ar = tar; ai = tai; br = tbr; bi = tbi;
+  } else {
+    oo = vcopy(oo);
+    if constexpr (!UNIN) mno = vcopy(mno);
+  }
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-      const float dx = px.qx[i] - r.ox;
-      const float dy = px.qy[i] - r.oy;
-      const float dz = px.qz[i] - r.oz;
-      const float p2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      const float inv = __builtin_amdgcn_rsqf(p2);
-      const float nd = fmaf(r.nz, dz, fmaf(r.ny, dy, r.nx * dx));
-      const float obl = fmaf(nd, inv, r.nl);
-      const float rden = __builtin_amdgcn_rcpf(fmaf(p2, inv, px.R[i]));
-      float rev;
-      if constexpr (QFOLD) {
-        // kr [|o|^2 t - 2 o.q] / (path + R),  t = (R0 - R)/R0 - eps/(2 R0)
-        const float lin = fmaf(r.oz, px.mz[i], fmaf(r.oy, px.my[i], r.ox * px.mx[i]));
-        const float eps = (lin + oo) * rden;  // kr (path - R)
-        const float t = fmaf(-eps, h, px.g[i]);
-        rev = fmaf(oo, t, lin) * rden;
-      } else {
-        // kr (|o|^2 - 2 o.q) / (path + R) = kr (path - R)
-        const float num = fmaf(r.oz, px.mz[i], fmaf(r.oy, px.my[i], fmaf(r.ox, px.mx[i], oo)));
-        rev = num * rden;
-      }
-      const float f = __builtin_amdgcn_fractf(rev);
-      const float s = __builtin_amdgcn_sinf(f);
-      const float c = __builtin_amdgcn_cosf(f);
-      const float amp = obl * inv;
-      float wr = amp * c, wi = amp * s;
-      if constexpr (CHECK) {
-        const bool skip = !(p2 > skip2);
-        wr = skip ? 0.f : wr;
-        wi = skip ? 0.f : wi;
-      }
-      a[i][0] = fmaf(r.ar, wr, a[i][0]);
-      a[i][0] = fmaf(-r.ai, wi, a[i][0]);
-      a[i][1] = fmaf(r.ar, wi, a[i][1]);
-      a[i][1] = fmaf(r.ai, wr, a[i][1]);
-      a[i][2] = fmaf(r.br, wr, a[i][2]);
-      a[i][2] = fmaf(-r.bi, wi, a[i][2]);
-      a[i][3] = fmaf(r.br, wi, a[i][3]);
-      a[i][3] = fmaf(r.bi, wr, a[i][3]);
+  for (int i = 0; i < P; ++i) {
+    // kr (|o|^2 - 2 o.q) split as base + o.m
+    const float lin = fmaf(r.oz, s.mz[i], fmaf(r.oy, s.my[i], r.ox * s.mx[i]));
+    const float num = lin + (TILE ? B : oo);
+    const float p2 = fmaf(num, ikr, s.R2[i]);                         // |p - r|^2
+    const float inv = __builtin_amdgcn_rsqf(p2);                       // 1 / path
+    const float rden = __builtin_amdgcn_rcpf(fmaf(p2, inv, s.R[i]));  // 1 / (path + R)
+    float rev;
+    if constexpr (TILE) {
+      const float eps = num * rden;            // kr (path - R)
+      const float u = fmaf(-eps, h, s.g[i]);   // (S_t - S) / S_t
+      rev = fmaf(B, u, lin) * rden;
+    } else {
+      rev = num * rden;
     }
+    float nd;
+    if constexpr (UNIN) {
+      nd = s.nq[i] + mno;  // n.q - n.o
+    } else {
+      nd = fmaf(r.nz, s.mz[i], fmaf(r.ny, s.my[i], fmaf(r.nx, s.mx[i], TILE ? nqt : mno)));
+    }
+    const float obl = fmaf(nd, inv, r.nl);     // nl + n.d / path
+    const float f = NOFRACT ? rev : __builtin_amdgcn_fractf(rev);
+    const float sn = __builtin_amdgcn_sinf(f);
+    const float cs = __builtin_amdgcn_cosf(f);
+    const float amp = obl * inv;
+    const float wr = amp * cs, wi = amp * sn;
+    a[i][0] = fmaf(ar, wr, a[i][0]);
+    a[i][0] = fmaf(-ai, wi, a[i][0]);
+    a[i][1] = fmaf(ar, wi, a[i][1]);
+    a[i][1] = fmaf(ai, wr, a[i][1]);
+    a[i][2] = fmaf(br, wr, a[i][2]);
+    a[i][2] = fmaf(-bi, wi, a[i][2]);
+    a[i][3] = fmaf(br, wi, a[i][3]);
+    a[i][3] = fmaf(bi, wr, a[i][3]);
   }
 }
 
-static constexpr uint32_t kLdsRays = 512;  // rays staged per LDS tile
-
-template <int P, int W, int SRC, bool CHECK, bool QFOLD>
-__device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w,
-                                          RayRec32* lds, uint32_t lane,
-                                          uint32_t wave) {
-  const uint32_t np = I.np;
-  const uint32_t base = w.pix0 + wave * 64u * P;
-  const double R0 = ((const XRTK_CONST InstScratch*)I.scr)->R0;
-  const float h = (float)(1.0 / (2.0 * R0 * I.kr_hi));
-  const float skip2 = (float)(I.skip * I.skip);
-  PixRegs<P> px;
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT>
+__device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
+                                          uint32_t lane, uint32_t wave) {
+  const uint32_t npad = I.npad;
+  const uint32_t base = w.pix0 + wave * 64u * P;  // first slot of this wave's tile
+  const double kr = I.kr_hi;
+  const TileRec T = ((const XRTK_CONST TileRec*)I.tiles)[base / (64u * P)];
+  const float ikr = (float)(1.0 / kr);
+  const float h = (float)(1.0 / (2.0 * T.R * kr));
+  SlotRegs<P> s;
+  float mtx = 0.f, mty = 0.f, mtz = 0.f;
+  if constexpr (!TILE) {
+    mtx = (float)(-2.0 * kr * T.qx); mty = (float)(-2.0 * kr * T.qy);
+    mtz = (float)(-2.0 * kr * T.qz);
+  }
 #pragma unroll
   for (int i = 0; i < P; ++i) {
-    uint32_t p = base + i * 64u + lane;
-    p = p < np ? p : np - 1u;
-    const PixRec32 pr = I.pix32[p];
-    px.qx[i] = pr.qx; px.qy[i] = pr.qy; px.qz[i] = pr.qz; px.R[i] = pr.R;
-    px.mx[i] = pr.mx; px.my[i] = pr.my; px.mz[i] = pr.mz;
-    px.g[i] = QFOLD ? (float)((R0 - I.R64[p]) / R0) : 0.f;
+    const PixRec32 pr = I.pix32[base + i * 64u + lane];
+    s.mx[i] = TILE ? pr.mx : pr.mx + mtx;
+    s.my[i] = TILE ? pr.my : pr.my + mty;
+    s.mz[i] = TILE ? pr.mz : pr.mz + mtz;
+    s.R2[i] = pr.R2; s.R[i] = pr.R; s.g[i] = pr.g; s.nq[i] = pr.nq;
   }
-  double acc[P][4];
+  // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
+  // partials are summed in FP64 by k_merge.
+  float a[P][4];
 #pragma unroll
-  for (int i = 0; i < P; ++i) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0.0;
+  for (int i = 0; i < P; ++i) a[i][0] = a[i][1] = a[i][2] = a[i][3] = 0.f;
 
+  const XRTK_CONST RayRec32* rays = (const XRTK_CONST RayRec32*)I.rays32;
   const uint32_t r0 = w.chunk * I.chunk_len;
   const uint32_t r1 = min(r0 + I.chunk_len, I.nr);
-  const uint32_t tile = I.tile;
-  for (uint32_t t0 = r0; t0 < r1; t0 += tile) {
-    const uint32_t t1 = min(t0 + tile, r1);
-    float a[P][4];
-#pragma unroll
-    for (int i = 0; i < P; ++i) a[i][0] = a[i][1] = a[i][2] = a[i][3] = 0.f;
-    if constexpr (SRC == 1) {
-      SmemRays src{(const XRTK_CONST RayRec32*)I.rays32};
-      pair_span<P, CHECK, QFOLD>(src, t0, t1, px, h, skip2, a);
-    } else {
-      for (uint32_t s0 = t0; s0 < t1; s0 += kLdsRays) {
-        const uint32_t ns = min(kLdsRays, t1 - s0);
-        __syncthreads();
-        const float4* g4 = reinterpret_cast<const float4*>(I.rays32 + s0);
-        float4* l4 = reinterpret_cast<float4*>(lds);
-        for (uint32_t k = threadIdx.x; k < ns * 3u; k += 64u * W) l4[k] = g4[k];
-        __syncthreads();
-        LdsRays src{lds};
-        pair_span<P, CHECK, QFOLD>(src, 0u, ns, px, h, skip2, a);
+  if constexpr (TILE) {
+    TileRay* my = lds + wave * kBlockRays;
+    for (uint32_t b0 = r0; b0 < r1; b0 += kBlockRays) {
+      const uint32_t nb = min(kBlockRays, r1 - b0);
+      // lane-parallel per-(ray, tile) values, one ray per lane: the phase
+      // A = kr (|o|^2 - 2 o.q_t) / (2 |q_t|) is reduced in FP64, its
+      // fractional revolution rotates the amplitudes
+      __syncthreads();
+      if (lane < nb) {
+        const RayRec32 rr = I.rays32[b0 + lane];
+        const double ox = rr.ox, oy = rr.oy, oz = rr.oz;
+        const double Bt = (ox * ox + oy * oy + oz * oz) -
+                          2.0 * (ox * T.qx + oy * T.qy + oz * T.qz);
+        const double A = kr * Bt / (2.0 * T.R);  // revolutions
+        const float fa = (float)(A - rint(A));
+        const float sn = __builtin_amdgcn_sinf(fa), cs = __builtin_amdgcn_cosf(fa);
+        TileRay tr;
+        tr.B = (float)(kr * Bt);
+        tr.ar = fmaf(rr.ar, cs, -rr.ai * sn);
+        tr.ai = fmaf(rr.ar, sn, rr.ai * cs);
+        tr.br = fmaf(rr.br, cs, -rr.bi * sn);
+        tr.bi = fmaf(rr.br, sn, rr.bi * cs);
+        // n.(q_t - o), with n = -2 kr n' (the record stores n' = -n/(2 kr))
+        tr.nqt = (float)(-2.0 * kr * ((double)rr.nx * T.qx + (double)rr.ny * T.qy +
+                                      (double)rr.nz * T.qz) - (double)rr.no);
+        tr.pad0 = tr.pad1 = 0.f;
+        my[lane] = tr;
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (uint32_t jj = 0; jj < nb; ++jj) {
+        const RayRec32 r = rays[b0 + jj];
+        const TileRay tr = my[jj];
+        pair_ray<P, true, UNIN, NOFRACT>(r, tr.B, tr.ar, tr.ai, tr.br, tr.bi, tr.nqt, s, ikr, h,
+                                         a);
       }
     }
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-      acc[i][0] += (double)a[i][0]; acc[i][1] += (double)a[i][1];
-      acc[i][2] += (double)a[i][2]; acc[i][3] += (double)a[i][3];
+  } else {
+#pragma unroll 2
+    for (uint32_t j = r0; j < r1; ++j) {
+      const RayRec32 r = rays[j];
+      pair_ray<P, false, UNIN, NOFRACT>(r, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
-  double* part = I.partial + (size_t)w.chunk * np * 4u;
+  float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
-    if (p < np) {
-      double2* d = reinterpret_cast<double2*>(part + (size_t)p * 4u);
-      d[0] = make_double2(acc[i][0], acc[i][1]);
-      d[1] = make_double2(acc[i][2], acc[i][3]);
-    }
+    part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
   }
 }
 
-template <int P, int W, int SRC, bool PERSIST>
+template <int P, int W, bool PERSIST>
 __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     const KInst* __restrict__ insts, const WorkItem* __restrict__ items,
     uint32_t n_items, uint32_t* __restrict__ queue, uint32_t remap) {
-  __shared__ RayRec32 lds[SRC == 2 ? kLdsRays : 1];
+  __shared__ TileRay lds[W * kBlockRays];
   __shared__ uint32_t s_item;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = uniform(threadIdx.x >> 6);
@@ -479,14 +612,24 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     if (it >= n_items) break;
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
-    const uint32_t flags = ((const XRTK_CONST InstScratch*)I.scr)->flags;
-    const bool chk = flags & INST_F_CHECK, qf = flags & INST_F_QFOLD;
-    if (chk) {
-      if (qf) f32r_item<P, W, SRC, true, true>(I, w, lds, lane, wave);
-      else    f32r_item<P, W, SRC, true, false>(I, w, lds, lane, wave);
+    const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
+    const bool tile = f & INST_F_TILE, uni = f & INST_F_UNIFORM_N, nofr = f & INST_F_NOFRACT;
+    if (tile) {
+      if (uni) {
+        if (nofr) f32r_item<P, W, true, true, true>(I, w, lds, lane, wave);
+        else      f32r_item<P, W, true, true, false>(I, w, lds, lane, wave);
+      } else {
+        if (nofr) f32r_item<P, W, true, false, true>(I, w, lds, lane, wave);
+        else      f32r_item<P, W, true, false, false>(I, w, lds, lane, wave);
+      }
     } else {
-      if (qf) f32r_item<P, W, SRC, false, true>(I, w, lds, lane, wave);
-      else    f32r_item<P, W, SRC, false, false>(I, w, lds, lane, wave);
+      if (uni) {
+        if (nofr) f32r_item<P, W, false, true, true>(I, w, lds, lane, wave);
+        else      f32r_item<P, W, false, true, false>(I, w, lds, lane, wave);
+      } else {
+        if (nofr) f32r_item<P, W, false, false, true>(I, w, lds, lane, wave);
+        else      f32r_item<P, W, false, false, false>(I, w, lds, lane, wave);
+      }
     }
     if constexpr (!PERSIST) break;
   }
@@ -499,14 +642,12 @@ template <int P, bool CHECK>
 __device__ __forceinline__ void f64_item(const KInst& I, const WorkItem& w,
                                          uint32_t lane, uint32_t wave) {
 #pragma clang fp contract(off)
-  const uint32_t np = I.np;
+  const uint32_t npad = I.npad;
   const uint32_t base = w.pix0 + wave * 64u * P;
   double px[P], py[P], pz[P];
 #pragma unroll
   for (int i = 0; i < P; ++i) {
-    uint32_t p = base + i * 64u + lane;
-    p = p < np ? p : np - 1u;
-    const PixF64 q = I.pix64[p];
+    const PixF64 q = I.pix64[base + i * 64u + lane];
     px[i] = q.x; py[i] = q.y; pz[i] = q.z;
   }
   double sum[P][4], cmp[P][4];
@@ -561,15 +702,13 @@ __device__ __forceinline__ void f64_item(const KInst& I, const WorkItem& w,
       sum[i][0] = n0; sum[i][1] = n1; sum[i][2] = n2; sum[i][3] = n3;
     }
   }
-  double* part = I.partial + (size_t)w.chunk * np * 4u;
+  double* part = reinterpret_cast<double*>(I.partial) + (size_t)w.chunk * npad * 4u;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
-    if (p < np) {
-      double2* d = reinterpret_cast<double2*>(part + (size_t)p * 4u);
-      d[0] = make_double2(sum[i][0], sum[i][1]);
-      d[1] = make_double2(sum[i][2], sum[i][3]);
-    }
+    double2* d = reinterpret_cast<double2*>(part + (size_t)p * 4u);
+    d[0] = make_double2(sum[i][0], sum[i][1]);
+    d[1] = make_double2(sum[i][2], sum[i][3]);
   }
 }
 
@@ -593,29 +732,40 @@ __global__ __launch_bounds__(64 * W) void k_pair_f64(
     if (it >= n_items) break;
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
-    if (((const XRTK_CONST InstScratch*)I.scr)->flags & INST_F_CHECK) f64_item<P, true>(I, w, lane, wave);
-    else                             f64_item<P, false>(I, w, lane, wave);
+    if (((const XRTK_CONST InstScratch*)I.scr)->flags & INST_F_CHECK)
+      f64_item<P, true>(I, w, lane, wave);
+    else
+      f64_item<P, false>(I, w, lane, wave);
     if constexpr (!PERSIST) break;
   }
 }
 
 // ---------------------------------------------------------------------------
-// merge: compensated FP64 sum of the chunk partials in chunk order, then the
-// base phase (FP32R): out = k/(4 pi) exp(i k |p - c|) * sum.
-// grid = (ceil(max np / 256), n_list); list[y] is an instance index.
+// merge: compensated FP64 sum of the chunk partials (FP32 partials of the
+// reduced kernel, FP64 of the plain kernel) in chunk order, then the
+// base phase (FP32R): out = k/(4 pi) exp(i k |p - c|) * sum, written to the
+// output index of the slot.  grid = (ceil(max npad / 256), n_list).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
                                                const uint32_t* __restrict__ list) {
   const KInst& I = insts[list[blockIdx.y]];
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  if (p >= I.np) return;
+  if (p >= I.npad) return;
+  const uint32_t dst = I.perm ? I.perm[p] : (p < I.np ? p : ~0u);
+  if (dst == ~0u) return;
   double s[4] = {0, 0, 0, 0}, e[4] = {0, 0, 0, 0};
-  const size_t stride = (size_t)I.np * 4u;
-  const double* src = I.partial + (size_t)p * 4u;
+  const bool f32 = I.mode == MODE_F32R;
   for (uint32_t c = 0; c < I.n_chunks; ++c) {
-    const double2 v01 = reinterpret_cast<const double2*>(src + c * stride)[0];
-    const double2 v23 = reinterpret_cast<const double2*>(src + c * stride)[1];
-    const double v[4] = {v01.x, v01.y, v23.x, v23.y};
+    double v[4];
+    if (f32) {
+      const float4 q = reinterpret_cast<const float4*>(I.partial)[(size_t)c * I.npad + p];
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+      const double2* q = reinterpret_cast<const double2*>(
+          reinterpret_cast<const double*>(I.partial) + ((size_t)c * I.npad + p) * 4u);
+      const double2 v01 = q[0], v23 = q[1];
+      v[0] = v01.x; v[1] = v01.y; v[2] = v23.x; v[3] = v23.y;
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {  // TwoSum (Knuth)
       const double t = s[k] + v[k];
@@ -636,7 +786,7 @@ __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
     const double a2 = o[2] * c - o[3] * sn, a3 = o[2] * sn + o[3] * c;
     o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3;
   }
-  double2* d = reinterpret_cast<double2*>(I.out + (size_t)p * 4u);
+  double2* d = reinterpret_cast<double2*>(I.out + (size_t)dst * 4u);
   if (I.accumulate) {
     const double2 d0 = d[0], d1 = d[1];
     o[0] += d0.x; o[1] += d0.y; o[2] += d1.x; o[3] += d1.y;
@@ -670,74 +820,92 @@ hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, 
   hipLaunchKernelGGL(k_rays_from_cols, dim3(cdiv(n, 256)), dim3(256), 0, st, cols, dst, n);
   return hipGetLastError();
 }
-hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, hipStream_t st) {
+hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, uint32_t npad,
+                                  hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_pix_from_f32aos, dim3(cdiv(n, 256)), dim3(256), 0, st,
-                     (const float4*)src, dst, n);
+  hipLaunchKernelGGL(k_pix_from_f32aos, dim3(cdiv(npad, 256)), dim3(256), 0, st,
+                     (const float4*)src, dst, n, npad);
   return hipGetLastError();
 }
-hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, hipStream_t st) {
+hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, uint32_t npad,
+                                hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_pix_from_cols, dim3(cdiv(n, 256)), dim3(256), 0, st, cols, dst, n);
+  hipLaunchKernelGGL(k_pix_from_cols, dim3(cdiv(npad, 256)), dim3(256), 0, st, cols, dst, n,
+                     npad);
   return hipGetLastError();
 }
-hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx,
-                           uint32_t nz, PixF64* dst, hipStream_t st) {
-  const size_t n = (size_t)nx * nz;
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_pix_grid, dim3(cdiv(n, 256)), dim3(256), 0, st, c[0], c[1], c[2],
-                     dx, dz, nx, nz, dst);
+hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,
+                           uint32_t th, uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
+                           hipStream_t st) {
+  if (!npad) return hipSuccess;
+  hipLaunchKernelGGL(k_pix_grid, dim3(cdiv(npad, 256)), dim3(256), 0, st, c[0], c[1], c[2], dx,
+                     dz, nx, nz, th, tw, npad, dst, perm);
   return hipGetLastError();
 }
 
 // Centroid, bounds, plan and the per-solve FP32 records of one instance.
-// part: kRedBlocks * 4 doubles of scratch.
-hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t np,
-                       InstScratch* scr, double* part, uint32_t mode, double kr,
-                       double kr_lo, double skip, double qfold_rev, uint32_t force_flags,
-                       uint32_t static_flags, RayRec32* rays32, PixRec32* pix32,
-                       double* R64, hipStream_t st) {
+// part: kRedBlocks * 4 doubles of scratch.  P is the pixels-per-lane of the
+// F32R launch (tile = 64 * P slots; npad a multiple of it).
+hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
+                       InstScratch* scr, double* part, uint32_t mode, double kr, double kr_lo,
+                       double skip, uint32_t force_flags, uint32_t static_flags, int P,
+                       RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
+                       hipStream_t st) {
   const unsigned nb = nr ? std::min<unsigned>(kRedBlocks, cdiv(nr, 256)) : 1u;
   hipLaunchKernelGGL(k_centroid_partial, dim3(nb), dim3(256), 0, st, rays, nr, part);
-  hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(256), 0, st, part, nb, nr, scr);
-  if (nr)
-    hipLaunchKernelGGL(k_ray_bounds, dim3(nb), dim3(256), 0, st, rays, nr, scr);
-  if (np)
-    hipLaunchKernelGGL(k_pix_prep, dim3(cdiv(np, 256)), dim3(256), 0, st, pix, np, kr, scr,
-                       mode == MODE_F32R ? pix32 : (PixRec32*)nullptr, R64);
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, st, scr, skip, kr, mode, qfold_rev,
-                     force_flags, static_flags);
+  hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(256), 0, st, part, nb, nr, rays, scr);
+  if (nr) hipLaunchKernelGGL(k_ray_bounds, dim3(nb), dim3(256), 0, st, rays, nr, scr);
+  if (npad) {
+    if (mode == MODE_F32R) {
+      const unsigned ntile = npad / (64u * P);
+      if (P == 2)
+        hipLaunchKernelGGL(k_tile_prep<2>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
+                           tiles, R64);
+      else if (P == 4)
+        hipLaunchKernelGGL(k_tile_prep<4>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
+                           tiles, R64);
+      else if (P == 8)
+        hipLaunchKernelGGL(k_tile_prep<8>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
+                           tiles, R64);
+      else
+        return hipErrorInvalidValue;
+    } else {
+      hipLaunchKernelGGL(k_pix_bounds, dim3(cdiv(npad, 256)), dim3(256), 0, st, pix, npad, scr);
+    }
+  }
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, st, scr, skip, kr, mode, force_flags,
+                     static_flags);
   if (mode == MODE_F32R && nr)
     hipLaunchKernelGGL(k_ray_prep32, dim3(cdiv(nr, 256)), dim3(256), 0, st, rays, nr, scr, kr,
                        kr_lo, rays32);
   return hipGetLastError();
 }
 
-template <int P, int W, int SRC, bool PERSIST>
+template <int P, int W, bool PERSIST>
 static void launch_f32r_t(const KInst* insts, const WorkItem* items, uint32_t n_items,
                           uint32_t* queue, uint32_t grid, uint32_t remap, hipStream_t st) {
-  hipLaunchKernelGGL((k_pair_f32r<P, W, SRC, PERSIST>), dim3(grid), dim3(64 * W), 0, st, insts,
-                     items, n_items, queue, remap);
+  hipLaunchKernelGGL((k_pair_f32r<P, W, PERSIST>), dim3(grid), dim3(64 * W), 0, st, insts, items,
+                     n_items, queue, remap);
 }
 template <int P, int W, bool PERSIST>
 static void launch_f64_t(const KInst* insts, const WorkItem* items, uint32_t n_items,
                          uint32_t* queue, uint32_t grid, uint32_t remap, hipStream_t st) {
-  hipLaunchKernelGGL((k_pair_f64<P, W, PERSIST>), dim3(grid), dim3(64 * W), 0, st, insts,
-                     items, n_items, queue, remap);
+  hipLaunchKernelGGL((k_pair_f64<P, W, PERSIST>), dim3(grid), dim3(64 * W), 0, st, insts, items,
+                     n_items, queue, remap);
 }
 
 // Returns hipErrorInvalidValue for a configuration that is not instantiated.
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
-                       const WorkItem* items, uint32_t n_items, uint32_t* queue,
-                       uint32_t grid, uint32_t remap, hipStream_t st) {
+                       const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
+                       uint32_t remap, hipStream_t st) {
   if (!n_items) return hipSuccess;
   if (cfg.persistent) hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 1u);
   const bool ps = cfg.persistent != 0;
-#define XRTK_F32(P_, W_, S_)                                                              \
-  if (cfg.P == P_ && cfg.W == W_ && cfg.ray_src == S_) {                                  \
-    if (ps) launch_f32r_t<P_, W_, S_, true>(insts, items, n_items, queue, grid, remap, st); \
-    else launch_f32r_t<P_, W_, S_, false>(insts, items, n_items, queue, grid, remap, st);   \
-    return hipGetLastError();                                                             \
+#define XRTK_F32(P_, W_)                                                                \
+  if (cfg.P == P_ && cfg.W == W_) {                                                     \
+    if (ps) launch_f32r_t<P_, W_, true>(insts, items, n_items, queue, grid, remap, st); \
+    else launch_f32r_t<P_, W_, false>(insts, items, n_items, queue, grid, remap, st);   \
+    return hipGetLastError();                                                           \
   }
 #define XRTK_F64(P_, W_)                                                               \
   if (cfg.P == P_ && cfg.W == W_) {                                                    \
@@ -746,9 +914,7 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
     return hipGetLastError();                                                          \
   }
   if (mode == MODE_F32R) {
-    XRTK_F32(2, 1, 1) XRTK_F32(2, 4, 1) XRTK_F32(4, 1, 1) XRTK_F32(4, 4, 1)
-    XRTK_F32(8, 1, 1) XRTK_F32(8, 4, 1)
-    XRTK_F32(2, 4, 2) XRTK_F32(4, 4, 2) XRTK_F32(8, 4, 2)
+    XRTK_F32(2, 1) XRTK_F32(2, 4) XRTK_F32(4, 1) XRTK_F32(4, 4) XRTK_F32(8, 1) XRTK_F32(8, 4)
   } else {
     XRTK_F64(1, 1) XRTK_F64(1, 4) XRTK_F64(2, 1) XRTK_F64(2, 4)
   }
@@ -758,18 +924,15 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 }
 
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c) {
-  if (mode == MODE_F32R) {
-    if (c.ray_src == 1) return (c.P == 2 || c.P == 4 || c.P == 8) && (c.W == 1 || c.W == 4);
-    if (c.ray_src == 2) return (c.P == 2 || c.P == 4 || c.P == 8) && c.W == 4;
-    return false;
-  }
+  if (c.ray_src != 1) return false;
+  if (mode == MODE_F32R) return (c.P == 2 || c.P == 4 || c.P == 8) && (c.W == 1 || c.W == 4);
   return (c.P == 1 || c.P == 2) && (c.W == 1 || c.W == 4);
 }
 
 hipError_t launch_merge(const KInst* insts, const uint32_t* list, uint32_t n_list,
-                        uint32_t max_np, hipStream_t st) {
-  if (!n_list || !max_np) return hipSuccess;
-  hipLaunchKernelGGL(k_merge, dim3(cdiv(max_np, 256), n_list), dim3(256), 0, st, insts, list);
+                        uint32_t max_npad, hipStream_t st) {
+  if (!n_list || !max_npad) return hipSuccess;
+  hipLaunchKernelGGL(k_merge, dim3(cdiv(max_npad, 256), n_list), dim3(256), 0, st, insts, list);
   return hipGetLastError();
 }
 

@@ -4,11 +4,15 @@
 // screen):
 //   RayRec64[nr]   resident f64 rays, 128 B each (AoS so that one wave reads a
 //                  whole record with two scalar loads)
-//   PixF64[np]     f64 screen points, 32 B each
-//   RayRec32[nr]   per-solve FP32 records of the phase-reduced kernel, 48 B
-//   PixRec32[np]   per-solve FP32 pixel records, 32 B
-//   double R64[np] FP64 |p - c| for the base phase
-//   double4 partial[n_chunks][np]  FP64 partial sums of each ray chunk
+//   PixF64[npad]   f64 screen points in slot order (tiles of 64*P slots; a
+//                  grid screen is laid out tile by tile, padding slots repeat a
+//                  real pixel), 32 B each, with perm[slot] -> output index
+//   RayRec32[nr]   per-solve FP32 ray records of the reduced kernel, 48 B
+//   PixRec32[npad] per-solve FP32 slot records, 32 B
+//   TileRec[ntile] per-solve FP64 tile reference points, 32 B
+//   double R64[npad]               FP64 |p - c| for the base phase
+//   partial[n_chunks][npad]        per-chunk partial sums (float4 from the
+//                                  FP32 kernel, double4 from the FP64 one)
 //   double4 out[np]                final {Es.re, Es.im, Ep.re, Ep.im}
 #pragma once
 #include <stdint.h>
@@ -34,43 +38,62 @@ struct PixF64 {
   double x, y, z, pad;
 };
 
-// FP32 record of one ray for the phase-reduced pair kernel.  o = r - c is
-// formed in FP64 and rounded once.  oo = kr |o|^2 (kr = k / 2 pi, so phases
-// are in revolutions).  (ar, ai) = i * Es' and (br, bi) = i * Ep', where Es'
-// and Ep' carry the upstream phase and, in the quadratic-fold variant, the
-// per-ray phase exp(i 2 pi kr |o|^2 / (2 R0)).  The factor k / (4 pi) is
-// applied once per pixel in the FP64 merge.
+// FP32 record of one ray for the phase-reduced pair kernel.
+//   o  = r - c, formed in FP64 and rounded once; every later use of the ray
+//        position (phase, path, fold) uses this rounded o, so the geometry is
+//        exactly that of a ray at c + o.
+//   oo = kr |o|^2 (kr = k / 2 pi: phases are in revolutions)
+//   no = n . o
+//   (ar, ai) = i Es', (br, bi) = i Ep', Es'/Ep' carrying the upstream phase.
+// The factor k/(4 pi) is applied once per pixel in the FP64 merge.
 struct RayRec32 {
   float ox, oy, oz, oo;
   float nx, ny, nz, nl;
   float ar, ai, br, bi;
+  float no, pad0, pad1, pad2;
 };
-static_assert(sizeof(RayRec32) == 48, "RayRec32 is 48 B");
+static_assert(sizeof(RayRec32) == 64, "RayRec32 is 64 B");
 
-// FP32 record of one pixel: q = p - c (rounded once from FP64), R = |q| in
-// FP32, m = -2 kr q (formed in FP64, rounded once).
+// FP32 record of one pixel slot.  With the slot's tile reference point t
+// (TileRec) and q = p - c, dq = q - q_t:
+//   m  = -2 kr dq   (FP64, rounded once)
+//   R2 = |q|^2, R = |q|
+//   g  = (R_t - R) / R_t
+//   nq = n . q   (only when every ray has the same normal n)
 struct PixRec32 {
-  float qx, qy, qz, R;
-  float mx, my, mz, pad;
+  float mx, my, mz, R2;
+  float R, g, nq, pad;
 };
 static_assert(sizeof(PixRec32) == 32, "PixRec32 is 32 B");
+
+// FP64 reference point of one tile of 64*P slots (the mean of its q).
+struct TileRec {
+  double qx, qy, qz, R;
+};
 
 // Per-instance values computed on the device (centroid, bounds, flags).
 struct InstScratch {
   double cx, cy, cz;       // FP64 ray centroid
-  double R0;               // reference distance of the quadratic fold
-  uint32_t omax_bits;      // max |r - c| as float bits (atomicMax)
-  uint32_t rmin_bits;      // min |p - c| as float bits (atomicMin)
-  uint32_t rmax_bits;      // max |p - c| as float bits (atomicMax)
+  double pad0;
+  uint32_t omax_bits[4];   // max |o_x|, |o_y|, |o_z|, |o| as float bits
+  uint32_t qmax_bits[3];   // max |q_x|, |q_y|, |q_z|
+  uint32_t dqmax_bits[3];  // max |dq_x|, |dq_y|, |dq_z| (slot to tile reference)
+  uint32_t rmin_bits;      // min |q|
+  uint32_t rmax_bits;      // max |q|
   uint32_t flags;          // INST_F_* decided by the plan kernel
   uint32_t ep_nonzero;     // 1 if any ray has Ep != 0
-  uint32_t pad[5];
+  uint32_t n_differs;      // 1 if some ray's normal differs from ray 0's
+  uint32_t pad1;
+  double n0x, n0y, n0z;    // normal of ray 0
+  double v0_rev, v2_rev;   // bounds of the reduced phase of the two variants
 };
 
 enum : uint32_t {
-  INST_F_CHECK = 1u,   // some pair may be near-singular: run the skip test
-  INST_F_QFOLD = 2u,   // quadratic phase folded into the ray amplitudes
-  INST_F_PATHUP = 4u,  // rays carry an upstream path phase
+  INST_F_CHECK = 1u,      // F64 kernel: some pair may fall below the skip threshold
+  INST_F_TILE = 2u,       // F32R: per-(ray, tile) phase folded into the amplitudes
+  INST_F_PATHUP = 4u,     // rays carry an upstream path phase
+  INST_F_UNIFORM_N = 8u,  // every ray has the same normal
+  INST_F_NOFRACT = 16u,   // |reduced phase| < 128 rev: v_sin/v_cos take it as is
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };
@@ -80,27 +103,28 @@ enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };
 struct KInst {
   const RayRec64* rays64;   // resident rays (F64 kernel, prep kernels)
   const RayRec32* rays32;   // prepared FP32 rays (F32R kernel)
-  const PixF64* pix64;      // f64 pixels
-  const PixRec32* pix32;    // prepared FP32 pixels
-  const double* R64;        // FP64 |p - c|
-  double* partial;          // [n_chunks][np][4]
+  const PixF64* pix64;      // f64 pixels in slot order
+  const PixRec32* pix32;    // prepared FP32 slots
+  const TileRec* tiles;     // tile reference points
+  const uint32_t* perm;     // slot -> output index (~0u: padding); null = identity
+  const double* R64;        // FP64 |p - c| per slot
+  void* partial;            // [n_chunks][npad] float4 (F32R) or double4 (F64)
   double* out;              // [np][4]
   InstScratch* scr;         // device-computed values
-  uint32_t nr, np, n_chunks, chunk_len;
-  uint32_t mode;            // MODE_*
-  uint32_t tile;            // rays per FP32 partial (F32R)
-  uint32_t accumulate;      // merge adds into out (energy groups)
-  uint32_t pad0;
-  double energy;            // > 0: overrides the rays' energy column
+  uint32_t nr, np, npad, n_chunks;
+  uint32_t chunk_len, mode, tile, accumulate;  // tile: unused by the kernels
+  uint32_t tile_slots;      // slots per tile (64 * P of the F32R launch)
+  uint32_t pad0, pad1, pad2;
+  double energy;            // > 0: overrides the rays' energy column (F64)
   double k;                 // E / CHBAR * 1e7 of this instance [mm^-1]
   double kr_hi, kr_lo;      // k / (2 pi) as a double-double [rev / mm]
   double skip;              // skip pairs with path < skip [mm]
-  double pad1;
+  double pad3;
 };
 static_assert(sizeof(KInst) % 16 == 0, "KInst size");
 
-// One unit of pair-kernel work: a block of pixels of one instance against one
-// chunk of its rays.  A workgroup of W waves handles W * 64 * P pixels.
+// One unit of pair-kernel work: a block of slots of one instance against one
+// chunk of its rays.  A workgroup of W waves handles W * 64 * P slots.
 struct WorkItem {
   uint32_t inst, pix0, chunk, pad;
 };
@@ -109,9 +133,9 @@ struct WorkItem {
 struct LaunchCfg {
   int P = 4;          // pixels per lane
   int W = 4;          // waves per workgroup
-  int ray_src = 1;    // 1 = SMEM, 2 = LDS
-  int persistent = 0; // 1 = persistent groups on an atomic queue
-  int tile = 1024;    // rays per FP32 partial
+  int ray_src = 1;    // 1 = SMEM (the only source of this build)
+  int persistent = 1; // 1 = persistent groups on an atomic queue
+  int tile = 8192;    // F32R: max rays per FP32 chunk partial
   int split = 0;      // ray chunks per pixel block (0 = planner)
 };
 

@@ -44,13 +44,13 @@ LaunchCfg batch_cfg(uint32_t mode, const XrtkOpts* o) {
   if (mode == MODE_F32R) { c.P = 4; c.W = 4; } else { c.P = 1; c.W = 4; }
   c.ray_src = 1;
   c.persistent = 1;
-  c.tile = 1024;
+  c.tile = mode == MODE_F32R ? 8192 : 0;
   if (o) {
     if (o->pixels_per_lane > 0) c.P = o->pixels_per_lane;
     if (o->waves_per_group > 0) c.W = o->waves_per_group;
     if (o->ray_src > 0) c.ray_src = o->ray_src;
     if (o->persistent == 0) c.persistent = 0;
-    if (o->tile_rays > 0) c.tile = (o->tile_rays + 63) / 64 * 64;
+    if (o->tile_rays > 0 && mode == MODE_F32R) c.tile = (o->tile_rays + 63) / 64 * 64;
   }
   return c;
 }
@@ -59,39 +59,59 @@ LaunchCfg batch_cfg(uint32_t mode, const XrtkOpts* o) {
 
 namespace xrtk {
 
-// Builds the instance specs of a batch inside `pixbuf` / `outbuf` (device).
-int build_job_specs(xrtk_ctx* ctx, uint32_t mode, const XrtkJob* jobs, size_t n,
-                    DBuf& pixbuf, DBuf& outbuf, std::vector<InstSpec>& specs,
-                    std::vector<size_t>& out_off) {
-  size_t pix_total = 0;
-  std::vector<size_t> pix_off(n);
-  out_off.assign(n, 0);
+// Builds the instance specs of a batch: pixel slots and permutations in
+// `pixbuf` / `permbuf`, outputs in `outbuf` (device).  Each job's F32R/FP64
+// choice comes from the residual bound of its ray set and screen.
+int build_job_specs(xrtk_ctx* ctx, uint32_t want, const LaunchCfg& c32, const LaunchCfg& c64,
+                    const XrtkJob* jobs, size_t n, DBuf& pixbuf, DBuf& permbuf, DBuf& outbuf,
+                    std::vector<InstSpec>& specs) {
+  size_t slots = 0, outs = 0;
+  std::vector<size_t> slot_off(n), out_off(n);
+  std::vector<uint32_t> npad(n);
   for (size_t i = 0; i < n; ++i) {
     const XrtkJob& j = jobs[i];
     const size_t np = (size_t)j.screen.nx * j.screen.nz;
-    if (np > 0x7fffffffull) return XRTK_EINVAL;
+    if (np > 0x7fff0000ull) return XRTK_EINVAL;
     if (np && !j.out4) return XRTK_EINVAL;
     auto it = ctx->raysets.find(j.rays);
     if (it == ctx->raysets.end()) return XRTK_EINVAL;
     const RaySet& rs = it->second;
-    if (mode == MODE_F32R && !(j.energy > 0) && !rs.mono) return XRTK_EINVAL;
-    pix_off[i] = pix_total;
-    out_off[i] = pix_total;
-    pix_total += a256(np);
+    if (want == MODE_F32R && !(j.energy > 0) && !rs.mono) return XRTK_EINVAL;
+    npad[i] = np ? grid_slots(j.screen.nx, j.screen.nz, c32, c64) : 0;
+    slot_off[i] = slots;
+    slots += npad[i];
+    out_off[i] = outs;
+    outs += a256(np);
   }
-  if (pixbuf.ensure(sizeof(PixF64) * std::max<size_t>(pix_total, 1)) != hipSuccess ||
-      outbuf.ensure(sizeof(double) * 4 * std::max<size_t>(pix_total, 1)) != hipSuccess)
+  if (pixbuf.ensure(sizeof(PixF64) * std::max<size_t>(slots, 1)) != hipSuccess ||
+      permbuf.ensure(sizeof(uint32_t) * std::max<size_t>(slots, 1)) != hipSuccess ||
+      outbuf.ensure(sizeof(double) * 4 * std::max<size_t>(outs, 1)) != hipSuccess)
     return XRTK_ENOMEM;
   specs.clear();
   for (size_t i = 0; i < n; ++i) {
     const XrtkJob& j = jobs[i];
     const RaySet& rs = ctx->raysets.find(j.rays)->second;
+    const double e = j.energy > 0 ? j.energy : rs.energy;
     InstSpec s;
+    Box pb;
+    pb.add(j.screen.center[0] - std::fabs(j.screen.dx), j.screen.center[1],
+           j.screen.center[2] - std::fabs(j.screen.dz));
+    pb.add(j.screen.center[0] + std::fabs(j.screen.dx), j.screen.center[1],
+           j.screen.center[2] + std::fabs(j.screen.dz));
+    uint32_t th, tw;
+    tile_shape(c32.P, &th, &tw);
+    const double dq[3] = {
+        j.screen.nx > 1 ? 2.0 * std::fabs(j.screen.dx) / (j.screen.nx - 1) * th : 0.0, 0.0,
+        j.screen.nz > 1 ? 2.0 * std::fabs(j.screen.dz) / (j.screen.nz - 1) * tw : 0.0};
+    s.mode = route_mode(want, rs.box, pb, dq, e);
     s.rays = rs.rays.as<RayRec64>();
     s.nr = rs.nr;
-    s.pix = pixbuf.as<PixF64>() + pix_off[i];
+    s.pix = pixbuf.as<PixF64>() + slot_off[i];
+    s.perm = permbuf.as<uint32_t>() + slot_off[i];
     s.np = j.screen.nx * j.screen.nz;
-    s.energy = j.energy > 0 ? j.energy : (mode == MODE_F32R ? rs.energy : 0.0);
+    s.npad = npad[i];
+    // F32R needs one energy; the FP64 kernel takes the override or the rays'
+    s.energy = s.mode == MODE_F32R ? e : (j.energy > 0 ? j.energy : 0.0);
     s.skip = 1e-30;
     s.static_flags = rs.has_path ? INST_F_PATHUP : 0u;
     s.out = outbuf.as<double>() + 4 * out_off[i];
@@ -100,11 +120,15 @@ int build_job_specs(xrtk_ctx* ctx, uint32_t mode, const XrtkJob* jobs, size_t n,
   return XRTK_OK;
 }
 
-int enqueue_job_grids(xrtk_ctx* ctx, const XrtkJob* jobs, const std::vector<InstSpec>& specs) {
+int enqueue_job_grids(xrtk_ctx* ctx, const LaunchCfg& c32, const XrtkJob* jobs,
+                      const std::vector<InstSpec>& specs) {
+  uint32_t th, tw;
+  tile_shape(c32.P, &th, &tw);
   for (size_t i = 0; i < specs.size(); ++i) {
     const XrtkWaveParams& w = jobs[i].screen;
-    hipError_t e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz,
-                                   const_cast<PixF64*>(specs[i].pix), ctx->stream);
+    hipError_t e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, specs[i].npad,
+                                   const_cast<PixF64*>(specs[i].pix),
+                                   const_cast<uint32_t*>(specs[i].perm), ctx->stream);
     if (e != hipSuccess) return map_err(e);
   }
   return XRTK_OK;
@@ -121,17 +145,20 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
   try {
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
-    const uint32_t mode = opts && opts->mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
-    const LaunchCfg cfg = batch_cfg(mode, opts);
-    if (!pair_cfg_supported(mode, cfg)) return XRTK_EINVAL;
-    const bool use_graph = !(std::getenv("XRTK_NO_GRAPH") && *std::getenv("XRTK_NO_GRAPH") == '1');
+    const uint32_t want = opts && opts->mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
+    const LaunchCfg c32 = batch_cfg(MODE_F32R, opts);
+    const LaunchCfg c64 = batch_cfg(MODE_F64, want == MODE_F64 ? opts : nullptr);
+    if (!pair_cfg_supported(MODE_F32R, c32) || !pair_cfg_supported(MODE_F64, c64))
+      return XRTK_EINVAL;
+    const char* ng = std::getenv("XRTK_NO_GRAPH");
+    const bool use_graph = !(ng && *ng == '1');
     // graph key: everything the captured nodes bake in
     std::vector<uint64_t> key;
     key.push_back(n_jobs);
-    key.push_back(mode);
-    key.push_back(((uint64_t)cfg.P << 32) | ((uint64_t)cfg.W << 16) | ((uint64_t)cfg.ray_src << 8) |
-                  (uint64_t)cfg.persistent);
-    key.push_back(cfg.tile);
+    key.push_back(want);
+    for (const LaunchCfg* c : {&c32, &c64})
+      key.push_back(((uint64_t)c->P << 32) | ((uint64_t)c->W << 16) |
+                    ((uint64_t)c->ray_src << 8) | (uint64_t)c->persistent | ((uint64_t)c->tile << 40));
     for (size_t i = 0; i < n_jobs; ++i) {
       const XrtkJob& j = jobs[i];
       key.push_back(j.rays);
@@ -146,63 +173,62 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
     xrtk_ctx::GraphEntry* ge = nullptr;
     for (auto* g2 : ctx->graphs)
       if (g2->key == key) { ge = g2; break; }
-    std::vector<InstSpec> specs;
-    std::vector<size_t> out_off;
     hipError_t e;
     if (!ge) {
-      ge = new xrtk_ctx::GraphEntry();
-      ge->key = key;
-      int rc = build_job_specs(ctx, mode, jobs, n_jobs, ge->pix, ge->outd, specs, out_off);
-      if (rc != XRTK_OK) { ge->pix.release(); ge->outd.release(); delete ge; return rc; }
-      for (size_t i = 0; i < n_jobs; ++i) ge->outs.emplace_back(specs[i].out, (size_t)specs[i].np);
-      // first run: eager, with the metadata upload; then capture the kernels
-      rc = enqueue_job_grids(ctx, jobs, specs);
-      Plan plan;
-      if (rc == XRTK_OK)
-        rc = run_instances(ctx, mode, cfg, specs, ge->work, ge->meta, true, &plan, 0, 0, true);
-      if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
-      if (rc != XRTK_OK) {
+      auto* ne = new xrtk_ctx::GraphEntry();
+      ne->key = key;
+      auto drop = [&](xrtk_ctx::GraphEntry* x) {
         (void)hipStreamSynchronize(ctx->stream);
-        ge->work.release(); ge->meta.release(); ge->pix.release(); ge->outd.release();
-        delete ge;
-        return rc;
-      }
+        if (x->exec) (void)hipGraphExecDestroy(x->exec);
+        x->work.release(); x->meta.release(); x->pix.release(); x->perm.release();
+        x->outd.release();
+        delete x;
+      };
+      std::vector<InstSpec> specs;
+      int rc = build_job_specs(ctx, want, c32, c64, jobs, n_jobs, ne->pix, ne->perm, ne->outd,
+                               specs);
+      if (rc != XRTK_OK) { drop(ne); return rc; }
+      for (size_t i = 0; i < n_jobs; ++i) ne->outs.emplace_back(specs[i].out, (size_t)specs[i].np);
+      // first run: eager, with the metadata upload; then capture the kernels
+      rc = enqueue_job_grids(ctx, c32, jobs, specs);
+      if (rc == XRTK_OK)
+        rc = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, true, &ne->plan, 0, 0, true);
+      if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
+      if (rc != XRTK_OK) { drop(ne); return rc; }
       if (use_graph) {
-        // Capture the same device work; the metadata is already in place.
         hipGraph_t graph = nullptr;
         e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-          int rc2 = enqueue_job_grids(ctx, jobs, specs);
+          int rc2 = enqueue_job_grids(ctx, c32, jobs, specs);
           if (rc2 == XRTK_OK)
-            rc2 = run_instances(ctx, mode, cfg, specs, ge->work, ge->meta, false, nullptr, 0, 0,
+            rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, false, nullptr, 0, 0,
                                 true, /*upload=*/false);
           e = hipStreamEndCapture(ctx->stream, &graph);
           if (rc2 == XRTK_OK && e == hipSuccess && graph) {
-            e = hipGraphInstantiate(&ge->exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) ge->exec = nullptr;
+            e = hipGraphInstantiate(&ne->exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) ne->exec = nullptr;
           }
           if (graph) (void)hipGraphDestroy(graph);
         }
         (void)hipGetLastError();
       }
-      ctx->graphs.push_back(ge);
+      // keep a bounded cache
+      if (ctx->graphs.size() >= 16) {
+        drop(ctx->graphs.front());
+        ctx->graphs.erase(ctx->graphs.begin());
+      }
+      ctx->graphs.push_back(ne);
+      ge = ne;
       float a = 0, b = 0;
       (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]);
       (void)hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]);
       ctx->last_pair_ms = a;
       ctx->last_solve_ms = b;
-      XrtkPlan& q = ctx->last_plan;
-      q.mode = (int32_t)mode; q.pixels_per_lane = cfg.P; q.waves_per_group = cfg.W;
-      q.tile_rays = cfg.tile; q.split = 0; q.ray_src = cfg.ray_src; q.persistent = cfg.persistent;
-      q.n_items = (int32_t)plan.n_items; q.checked = 0; q.backend = 1;
     } else {
+      if (!ge->exec) return XRTK_ELAUNCH;
       (void)hipEventRecord(ctx->ev[0], ctx->stream);
-      if (ge->exec) {
-        e = hipGraphLaunch(ge->exec, ctx->stream);
-        if (e != hipSuccess) return map_err(e);
-      } else {
-        return XRTK_ELAUNCH;
-      }
+      e = hipGraphLaunch(ge->exec, ctx->stream);
+      if (e != hipSuccess) return map_err(e);
       (void)hipEventRecord(ctx->ev[3], ctx->stream);
       e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return map_err(e);
@@ -210,6 +236,17 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       (void)hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]);
       ctx->last_solve_ms = b;
       ctx->last_pair_ms = b;
+    }
+    {
+      XrtkPlan& q = ctx->last_plan;
+      const Plan& p = ge->plan;
+      const bool all64 = p.n_inst && p.n_f64 == p.n_inst;
+      const LaunchCfg& c = all64 ? p.cfg64 : p.cfg32;
+      q.mode = all64 ? XRTK_MODE_F64 : XRTK_MODE_F32R;
+      q.pixels_per_lane = c.P; q.waves_per_group = c.W; q.tile_rays = c.tile; q.split = 0;
+      q.ray_src = c.ray_src; q.persistent = c.persistent;
+      q.n_items = (int32_t)(p.n_items32 + p.n_items64); q.n_fp64 = (int32_t)p.n_f64;
+      q.backend = 1;
     }
     // results -> host
     for (size_t i = 0; i < n_jobs; ++i) {
@@ -334,18 +371,18 @@ extern "C" int xrtk_autotune(xrtk_ctx* ctx, size_t n_rays, size_t n_pixels, int 
     std::vector<double> out(4 * n_pixels);
     std::vector<LaunchCfg> cands;
     if (mode == MODE_F32R) {
-      for (int src : {1, 2})
+      for (int src : {1})
         for (int P : {2, 4, 8})
           for (int W : {1, 4})
             for (int per : {0, 1}) {
-              LaunchCfg c; c.P = P; c.W = W; c.ray_src = src; c.persistent = per; c.tile = 1024;
+              LaunchCfg c; c.P = P; c.W = W; c.ray_src = src; c.persistent = per; c.tile = 8192;
               if (pair_cfg_supported(mode, c)) cands.push_back(c);
             }
     } else {
       for (int P : {1, 2})
         for (int W : {1, 4})
           for (int per : {0, 1}) {
-            LaunchCfg c; c.P = P; c.W = W; c.ray_src = 1; c.persistent = per; c.tile = 1024;
+            LaunchCfg c; c.P = P; c.W = W; c.ray_src = 1; c.persistent = per; c.tile = 0;
             cands.push_back(c);
           }
     }

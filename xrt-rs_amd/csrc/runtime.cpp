@@ -4,7 +4,7 @@
 // :85-298 and src/gpu/context.rs:20-67) with a HIP design: the context is
 // created once per process (or explicitly), owns one stream and a grow-only
 // workspace, and every solve is centroid -> bounds/plan -> FP32 records ->
-// pair kernel -> FP64 merge on that stream, with no per-call compilation.
+// pair kernels -> FP64 merge on that stream, with no per-call compilation.
 #include "runtime.h"
 
 #include <algorithm>
@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 
 namespace xrtk {
 
@@ -68,12 +69,20 @@ static int ilog2(uint64_t v) {
 static LaunchCfg default_cfg(uint32_t mode, uint64_t nr, uint64_t np) {
   LaunchCfg c;
   if (mode == MODE_F32R) {
-    c.P = 4; c.W = 4; c.ray_src = 1; c.persistent = 0; c.tile = 1024;
+    c.P = 4; c.W = 4;
     if (np < 64ull * 4 * 4 * 64) { c.P = 2; c.W = 4; }
   } else {
-    c.P = 1; c.W = 4; c.ray_src = 1; c.persistent = 0; c.tile = 1024;
+    c.P = 1; c.W = 4;
   }
+  c.ray_src = 1; c.persistent = 1; c.tile = mode == MODE_F32R ? 8192 : 0;
   (void)nr;
+  return c;
+}
+
+LaunchCfg cfg64_for(const LaunchCfg& c32) {
+  LaunchCfg c;
+  c.P = 1; c.W = 4; c.ray_src = 1; c.persistent = c32.persistent; c.tile = 0;
+  c.split = c32.split;
   return c;
 }
 
@@ -86,29 +95,30 @@ LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, con
     if (o->waves_per_group > 0) c.W = o->waves_per_group;
     if (o->tile_rays > 0) c.tile = o->tile_rays;
     if (o->ray_src > 0) c.ray_src = o->ray_src;
-    if (o->persistent >= 0 && o->persistent <= 1 && (o->persistent == 1 || o->split != 0 ||
-                                                      o->pixels_per_lane != 0))
-      c.persistent = o->persistent;
+    if (o->persistent == 0 || o->persistent == 1) c.persistent = o->persistent;
     if (o->split > 0) c.split = o->split;
   }
-  if (c.tile < 64) c.tile = 64;
-  c.tile = (c.tile + 63) / 64 * 64;
+  if (mode == MODE_F32R) {
+    if (c.tile < 64) c.tile = 64;
+    c.tile = (c.tile + 63) / 64 * 64;
+  } else {
+    c.tile = 0;  // the FP64 kernel keeps Kahan sums over any chunk length
+  }
   return c;
 }
 
-// Work items per launch that fill the device: every CU holds ~8 waves of
-// the pair kernel per SIMD; aim for 2 rounds so the tail stays short.
+// Work items per launch that fill the device: ~8 resident waves per SIMD;
+// aim for 2 rounds so the tail stays short.
 static uint64_t target_items(const xrtk_ctx* ctx, const LaunchCfg& cfg) {
   const uint64_t waves = (uint64_t)ctx->n_cu * 4 * 8;
   return std::max<uint64_t>(1, 2 * waves / cfg.W);
 }
 
-uint32_t chunk_len_for(const xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg, uint32_t nr,
-                       uint32_t np, int split_override) {
-  (void)mode;
+uint32_t chunk_len_for(const xrtk_ctx* ctx, const LaunchCfg& cfg, uint32_t nr, uint32_t npad,
+                       int split_override) {
   if (nr == 0) return cfg.tile;
   const uint64_t B = 64ull * cfg.W * cfg.P;
-  const uint64_t pblocks = std::max<uint64_t>(1, (np + B - 1) / B);
+  const uint64_t pblocks = std::max<uint64_t>(1, (npad + B - 1) / B);
   uint64_t split;
   if (split_override > 0) {
     split = (uint64_t)split_override;
@@ -119,8 +129,57 @@ uint32_t chunk_len_for(const xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
   }
   split = std::max<uint64_t>(1, std::min<uint64_t>(split, nr));
   uint64_t cl = (nr + split - 1) / split;
-  cl = (cl + cfg.tile - 1) / cfg.tile * cfg.tile;  // chunk edges on tile edges
+  cl = (cl + 63) / 64 * 64;  // whole LDS ray blocks
+  if (cfg.tile > 0) cl = std::min<uint64_t>(cl, (uint64_t)cfg.tile);
   return (uint32_t)std::min<uint64_t>(cl, 0xffffffffu);
+}
+
+static uint32_t slot_block(const LaunchCfg& c32, const LaunchCfg& c64) {
+  const uint32_t b32 = 64u * c32.P * c32.W, b64 = 64u * c64.P * c64.W;
+  return std::max(b32, b64);  // both powers of two: the larger is a multiple
+}
+
+uint32_t padded_slots(uint32_t np, const LaunchCfg& c32, const LaunchCfg& c64) {
+  const uint32_t b = slot_block(c32, c64);
+  return (np + b - 1) / b * b;
+}
+
+void tile_shape(int P, uint32_t* th, uint32_t* tw) {
+  const uint32_t n = 64u * P;
+  *tw = P >= 8 ? 32u : 16u;
+  *th = n / *tw;
+}
+
+uint32_t grid_slots(uint32_t nx, uint32_t nz, const LaunchCfg& c32, const LaunchCfg& c64) {
+  uint32_t th, tw;
+  tile_shape(c32.P, &th, &tw);
+  const uint64_t ntiles = (uint64_t)((nx + th - 1) / th) * ((nz + tw - 1) / tw);
+  const uint64_t s = ntiles * th * tw;
+  const uint32_t b = slot_block(c32, c64);
+  return (uint32_t)((s + b - 1) / b * b);
+}
+
+double residual_bound_rev(const Box& rays, const Box& pix, const double dq[3], double energy) {
+  if (rays.empty() || pix.empty()) return 0.0;
+  double w2 = 0, gap2 = 0, lin = 0;
+  for (int i = 0; i < 3; ++i) {
+    const double w = rays.hi[i] - rays.lo[i];
+    w2 += w * w;
+    const double g = std::max({0.0, pix.lo[i] - rays.hi[i], rays.lo[i] - pix.hi[i]});
+    gap2 += g * g;
+    lin += w * dq[i];
+  }
+  const double w = std::sqrt(w2), den = 2.0 * std::sqrt(gap2) - w;
+  if (!(std::sqrt(gap2) > w * (1.0 + 1e-3) + 1e-12) || den <= 0.0)
+    return std::numeric_limits<double>::infinity();
+  const double kr = energy / kCHBAR_H * 1e7 / (2.0 * M_PI);
+  return kr * (2.0 * lin + w2 * w / den) / den;
+}
+
+uint32_t route_mode(uint32_t mode, const Box& rays, const Box& pix, const double dq[3],
+                    double energy) {
+  if (mode != MODE_F32R) return mode;
+  return residual_bound_rev(rays, pix, dq, energy) > kMaxResidualRev ? MODE_F64 : MODE_F32R;
 }
 
 static void kr_dd(double k, double* hi, double* lo) {
@@ -132,29 +191,38 @@ static void kr_dd(double k, double* hi, double* lo) {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
+int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
-                  Plan* plan_out, int split_override, uint32_t force_flags,
-                  bool persistent_sorted, bool upload) {
+                  Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
+                  bool upload) {
   hipStream_t st = ctx->stream;
-  if (!pair_cfg_supported(mode, cfg)) return XRTK_EINVAL;
+  if (!pair_cfg_supported(MODE_F32R, cfg32) || !pair_cfg_supported(MODE_F64, cfg64))
+    return XRTK_EINVAL;
   const size_t n = specs.size();
+  const uint32_t sblock = slot_block(cfg32, cfg64);
   // ---- layout of the per-instance buffers in `work`
   struct Lay {
-    size_t rays32, pix32, R64, partial, scr, part;
+    size_t rays32, pix32, tiles, R64, partial, scr, part;
     uint32_t chunk_len, n_chunks;
   };
   std::vector<Lay> lay(n);
   size_t off = 0;
   for (size_t i = 0; i < n; ++i) {
     const InstSpec& s = specs[i];
+    if (s.npad % sblock) return XRTK_EINVAL;
+    const bool f32 = s.mode == MODE_F32R;
+    const LaunchCfg& cfg = f32 ? cfg32 : cfg64;
     Lay& L = lay[i];
-    L.chunk_len = chunk_len_for(ctx, mode, cfg, s.nr, s.np, split_override);
+    L.chunk_len = chunk_len_for(ctx, cfg, s.nr, s.npad, split_override);
     L.n_chunks = s.nr ? (s.nr + L.chunk_len - 1) / L.chunk_len : 0;
-    L.rays32 = off; off = align_up(off + (mode == MODE_F32R ? sizeof(RayRec32) * s.nr : 0), 256);
-    L.pix32 = off; off = align_up(off + (mode == MODE_F32R ? sizeof(PixRec32) * s.np : 0), 256);
-    L.R64 = off; off = align_up(off + sizeof(double) * s.np, 256);
-    L.partial = off; off = align_up(off + sizeof(double) * 4 * (size_t)s.np * L.n_chunks, 256);
+    L.rays32 = off; off = align_up(off + (f32 ? sizeof(RayRec32) * s.nr : 0), 256);
+    L.pix32 = off; off = align_up(off + (f32 ? sizeof(PixRec32) * s.npad : 0), 256);
+    L.tiles = off;
+    off = align_up(off + (f32 ? sizeof(TileRec) * (s.npad / (64u * cfg32.P)) : 0), 256);
+    L.R64 = off; off = align_up(off + sizeof(double) * s.npad, 256);
+    L.partial = off;
+    off = align_up(off + (f32 ? sizeof(float) : sizeof(double)) * 4 * (size_t)s.npad * L.n_chunks,
+                   256);
     L.scr = off; off = align_up(off + sizeof(InstScratch), 256);
     L.part = off; off = align_up(off + sizeof(double) * 4 * kPartBlocks, 256);
   }
@@ -164,42 +232,49 @@ int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
 
   // ---- instance table and work items
   std::vector<KInst> insts(n);
-  std::vector<WorkItem> items;
-  const uint32_t B = 64u * cfg.W * cfg.P;
-  uint32_t max_np = 0;
-  std::vector<uint32_t> merge_list;
+  std::vector<WorkItem> items32, items64;
+  uint32_t n_f64 = 0;
   for (size_t i = 0; i < n; ++i) {
     const InstSpec& s = specs[i];
     const Lay& L = lay[i];
+    const bool f32 = s.mode == MODE_F32R;
     KInst& I = insts[i];
     std::memset(&I, 0, sizeof(I));
     I.rays64 = s.rays;
     I.rays32 = reinterpret_cast<const RayRec32*>(W + L.rays32);
     I.pix64 = s.pix;
     I.pix32 = reinterpret_cast<const PixRec32*>(W + L.pix32);
+    I.tiles = reinterpret_cast<const TileRec*>(W + L.tiles);
+    I.perm = s.perm;
     I.R64 = reinterpret_cast<const double*>(W + L.R64);
-    I.partial = reinterpret_cast<double*>(W + L.partial);
+    I.partial = W + L.partial;
     I.out = s.out;
     I.scr = reinterpret_cast<InstScratch*>(W + L.scr);
-    I.nr = s.nr; I.np = s.np; I.n_chunks = L.n_chunks; I.chunk_len = L.chunk_len;
-    I.mode = mode; I.tile = cfg.tile; I.accumulate = s.accumulate;
+    I.nr = s.nr; I.np = s.np; I.npad = s.npad; I.n_chunks = L.n_chunks;
+    I.chunk_len = L.chunk_len; I.mode = s.mode; I.tile = cfg32.tile;
+    I.accumulate = s.accumulate;
+    I.tile_slots = 64u * cfg32.P;
     I.energy = s.energy;
     I.k = s.energy / kCHBAR_H * 1e7;
     kr_dd(I.k, &I.kr_hi, &I.kr_lo);
     I.skip = s.skip;
-    max_np = std::max(max_np, s.np);
+    if (!f32) ++n_f64;
+    const uint32_t B = f32 ? 64u * cfg32.W * cfg32.P : 64u * cfg64.W * cfg64.P;
+    std::vector<WorkItem>& items = f32 ? items32 : items64;
     for (uint32_t c = 0; c < L.n_chunks; ++c)
-      for (uint32_t p0 = 0; p0 < s.np; p0 += B) items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
+      for (uint32_t p0 = 0; p0 < s.npad; p0 += B) items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
   }
   // Batched launches: longest items first so the queue drains evenly.
-  if (persistent_sorted && n > 1) {
-    std::stable_sort(items.begin(), items.end(), [&](const WorkItem& a, const WorkItem& b) {
+  if (sorted && n > 1) {
+    auto by_len = [&](const WorkItem& a, const WorkItem& b) {
       const KInst& A = insts[a.inst];
       const KInst& Bq = insts[b.inst];
       const uint64_t la = std::min<uint64_t>(A.chunk_len, A.nr - (uint64_t)a.chunk * A.chunk_len);
       const uint64_t lb = std::min<uint64_t>(Bq.chunk_len, Bq.nr - (uint64_t)b.chunk * Bq.chunk_len);
       return la > lb;
-    });
+    };
+    std::stable_sort(items32.begin(), items32.end(), by_len);
+    std::stable_sort(items64.begin(), items64.end(), by_len);
   }
   // merge lists: instances writing their own output merge in one launch;
   // instances that accumulate into an earlier output follow one by one.
@@ -209,28 +284,33 @@ int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
   std::vector<uint32_t> lists(first);
   lists.insert(lists.end(), rest.begin(), rest.end());
   uint32_t first_np = 0;
-  for (uint32_t i : first) first_np = std::max(first_np, specs[i].np);
+  for (uint32_t i : first) first_np = std::max(first_np, specs[i].npad);
 
-  const size_t meta_bytes = align_up(sizeof(KInst) * n, 256) +
-                            align_up(sizeof(WorkItem) * std::max<size_t>(1, items.size()), 256) +
-                            align_up(sizeof(uint32_t) * (lists.size() + 64), 256);
-  e = meta.ensure(meta_bytes);
+  const size_t a_ins = align_up(sizeof(KInst) * std::max<size_t>(n, 1), 256);
+  const size_t a_i32 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items32.size()), 256);
+  const size_t a_i64 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items64.size()), 256);
+  const size_t a_lst = align_up(sizeof(uint32_t) * (lists.size() + 1), 256);
+  e = meta.ensure(a_ins + a_i32 + a_i64 + a_lst + 256);
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* M = meta.as<char>();
   KInst* d_insts = reinterpret_cast<KInst*>(M);
-  WorkItem* d_items = reinterpret_cast<WorkItem*>(M + align_up(sizeof(KInst) * n, 256));
-  uint32_t* d_lists = reinterpret_cast<uint32_t*>(
-      M + align_up(sizeof(KInst) * n, 256) +
-      align_up(sizeof(WorkItem) * std::max<size_t>(1, items.size()), 256));
-  uint32_t* d_queue = d_lists + lists.size() + 32;
-  if (upload) e = hipMemcpyAsync(d_insts, insts.data(), sizeof(KInst) * n, hipMemcpyHostToDevice, st);
-  if (upload && e == hipSuccess && !items.empty())
-    e = hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(),
-                       hipMemcpyHostToDevice, st);
-  if (upload && e == hipSuccess && !lists.empty())
-    e = hipMemcpyAsync(d_lists, lists.data(), sizeof(uint32_t) * lists.size(),
-                       hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return map_err(e);
+  WorkItem* d_items32 = reinterpret_cast<WorkItem*>(M + a_ins);
+  WorkItem* d_items64 = reinterpret_cast<WorkItem*>(M + a_ins + a_i32);
+  uint32_t* d_lists = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64);
+  uint32_t* d_queue = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_lst);
+  if (upload) {
+    if (n) e = hipMemcpyAsync(d_insts, insts.data(), sizeof(KInst) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items32.empty())
+      e = hipMemcpyAsync(d_items32, items32.data(), sizeof(WorkItem) * items32.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items64.empty())
+      e = hipMemcpyAsync(d_items64, items64.data(), sizeof(WorkItem) * items64.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !lists.empty())
+      e = hipMemcpyAsync(d_lists, lists.data(), sizeof(uint32_t) * lists.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return map_err(e);
+  }
 
   if (timed) (void)hipEventRecord(ctx->ev[0], st);
   // ---- prep of every instance
@@ -246,25 +326,37 @@ int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
       }
       continue;
     }
-    e = launch_prep(s.rays, s.nr, s.pix, s.np, reinterpret_cast<InstScratch*>(W + L.scr),
-                    reinterpret_cast<double*>(W + L.part), mode, insts[i].kr_hi, insts[i].kr_lo,
-                    s.skip, /*qfold_rev=*/8.0, force_flags, s.static_flags,
+    e = launch_prep(s.rays, s.nr, s.pix, s.npad, reinterpret_cast<InstScratch*>(W + L.scr),
+                    reinterpret_cast<double*>(W + L.part), s.mode, insts[i].kr_hi,
+                    insts[i].kr_lo, s.skip, force_flags, s.static_flags, cfg32.P,
                     reinterpret_cast<RayRec32*>(W + L.rays32),
-                    reinterpret_cast<PixRec32*>(W + L.pix32), reinterpret_cast<double*>(W + L.R64),
+                    reinterpret_cast<PixRec32*>(W + L.pix32),
+                    reinterpret_cast<TileRec*>(W + L.tiles), reinterpret_cast<double*>(W + L.R64),
                     st);
     if (e != hipSuccess) return map_err(e);
   }
-  // ---- pair kernel
-  const uint32_t n_items = (uint32_t)items.size();
-  uint32_t grid = n_items;
-  if (cfg.persistent) {
-    const uint32_t resident = (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / cfg.W;
-    grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, resident));
-  }
-  const uint32_t remap = (!cfg.persistent && n == 1) ? 1u : 0u;
+  // ---- pair kernels
+  auto grid_of = [&](const LaunchCfg& cfg, uint32_t mode, size_t n_items) {
+    uint32_t g = (uint32_t)n_items;
+    if (cfg.persistent) {
+      const uint32_t resident = (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / cfg.W;
+      g = std::max<uint32_t>(1, std::min<uint32_t>(g, resident));
+    }
+    return g;
+  };
   if (timed) (void)hipEventRecord(ctx->ev[1], st);
-  e = launch_pair(mode, cfg, d_insts, d_items, n_items, d_queue, grid, remap, st);
-  if (e != hipSuccess) return map_err(e);
+  if (!items32.empty()) {
+    const uint32_t remap = (!cfg32.persistent && n == 1) ? 1u : 0u;
+    e = launch_pair(MODE_F32R, cfg32, d_insts, d_items32, (uint32_t)items32.size(), d_queue,
+                    grid_of(cfg32, MODE_F32R, items32.size()), remap, st);
+    if (e != hipSuccess) return map_err(e);
+  }
+  if (!items64.empty()) {
+    const uint32_t remap = (!cfg64.persistent && n == 1) ? 1u : 0u;
+    e = launch_pair(MODE_F64, cfg64, d_insts, d_items64, (uint32_t)items64.size(), d_queue + 16,
+                    grid_of(cfg64, MODE_F64, items64.size()), remap, st);
+    if (e != hipSuccess) return map_err(e);
+  }
   if (timed) (void)hipEventRecord(ctx->ev[2], st);
   // ---- merge
   if (!first.empty()) {
@@ -272,19 +364,18 @@ int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
     if (e != hipSuccess) return map_err(e);
   }
   for (size_t j = 0; j < rest.size(); ++j) {
-    e = launch_merge(d_insts, d_lists + first.size() + j, 1, specs[rest[j]].np, st);
+    e = launch_merge(d_insts, d_lists + first.size() + j, 1, specs[rest[j]].npad, st);
     if (e != hipSuccess) return map_err(e);
   }
   if (timed) (void)hipEventRecord(ctx->ev[3], st);
   if (plan_out) {
-    plan_out->mode = mode;
-    plan_out->cfg = cfg;
-    plan_out->n_items = n_items;
-    plan_out->grid = grid;
-    plan_out->remap = remap;
+    plan_out->cfg32 = cfg32;
+    plan_out->cfg64 = cfg64;
+    plan_out->n_items32 = (uint32_t)items32.size();
+    plan_out->n_items64 = (uint32_t)items64.size();
+    plan_out->n_f64 = n_f64;
+    plan_out->n_inst = (uint32_t)n;
   }
-  (void)max_np;
-  (void)B;
   return XRTK_OK;
 }
 

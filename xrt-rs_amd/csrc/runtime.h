@@ -19,24 +19,31 @@ namespace xrtk {
 // ---- device launchers (device.hip) ----
 hipError_t launch_rays_from_f32aos(const float* src, RayRec64* dst, uint32_t n, hipStream_t st);
 hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, hipStream_t st);
-hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, hipStream_t st);
-hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, hipStream_t st);
+hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, uint32_t npad,
+                                  hipStream_t st);
+hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, uint32_t npad,
+                                hipStream_t st);
 hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,
-                           PixF64* dst, hipStream_t st);
-hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t np,
-                       InstScratch* scr, double* part, uint32_t mode, double kr_hi, double kr_lo,
-                       double skip, double qfold_rev, uint32_t force_flags, uint32_t static_flags,
-                       RayRec32* rays32, PixRec32* pix32, double* R64, hipStream_t st);
+                           uint32_t th, uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
+                           hipStream_t st);
+hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
+                       InstScratch* scr, double* part, uint32_t mode, double kr, double kr_lo,
+                       double skip, uint32_t force_flags, uint32_t static_flags, int P,
+                       RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
+                       hipStream_t st);
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st);
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c);
 hipError_t launch_merge(const KInst* insts, const uint32_t* list, uint32_t n_list,
-                        uint32_t max_np, hipStream_t st);
+                        uint32_t max_npad, hipStream_t st);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t st);
 
 constexpr double kCHBAR_H = 1973.2697177417986;  // src/core/consts.rs:74
 constexpr int kPartBlocks = 240;                 // centroid partial blocks (device.hip)
+// Reduced-phase bound above which an instance runs on the FP64 kernel: the
+// FP32 residual phase would carry more than ~1e-4 rad of rounding per pair.
+constexpr double kMaxResidualRev = 512.0;
 
 // Grow-only device buffer.
 struct DBuf {
@@ -48,6 +55,20 @@ struct DBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// Axis-aligned bounding box.
+struct Box {
+  double lo[3] = {1e300, 1e300, 1e300};
+  double hi[3] = {-1e300, -1e300, -1e300};
+  void add(double x, double y, double z) {
+    const double v[3] = {x, y, z};
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = v[i] < lo[i] ? v[i] : lo[i];
+      hi[i] = v[i] > hi[i] ? v[i] : hi[i];
+    }
+  }
+  bool empty() const { return lo[0] > hi[0]; }
+};
+
 // A resident ray set.
 struct RaySet {
   DBuf rays;          // RayRec64[nr]
@@ -55,14 +76,17 @@ struct RaySet {
   bool has_path = false;
   bool mono = true;   // all energies equal
   double energy = 0;  // the common energy when mono
+  Box box;            // positions
 };
 
 // Description of one instance for a launch (all pointers are device memory).
 struct InstSpec {
+  uint32_t mode = MODE_F32R;
   const RayRec64* rays = nullptr;
   uint32_t nr = 0;
-  const PixF64* pix = nullptr;
-  uint32_t np = 0;
+  const PixF64* pix = nullptr;   // npad slots
+  const uint32_t* perm = nullptr;  // slot -> output index, or null (identity below np)
+  uint32_t np = 0, npad = 0;
   double energy = 0;       // F32R: the instance energy; F64: > 0 overrides
   double skip = 1e-30;     // skip pairs with path below this [mm]
   uint32_t static_flags = 0;
@@ -71,12 +95,10 @@ struct InstSpec {
 };
 
 struct Plan {
-  uint32_t mode = MODE_F32R;
-  LaunchCfg cfg;
-  uint32_t n_items = 0;
-  uint32_t grid = 0;
-  uint32_t remap = 0;
-  int checked = 0;
+  LaunchCfg cfg32, cfg64;
+  uint32_t n_items32 = 0, n_items64 = 0;
+  uint32_t n_f64 = 0;     // instances on the FP64 kernel
+  uint32_t n_inst = 0;
 };
 
 }  // namespace xrtk
@@ -90,7 +112,7 @@ struct xrtk_ctx {
   int n_cu = 256;
   bool shared = false;
   // one-shot workspace
-  xrtk::DBuf stage, rays64, pix64, work, meta, outbuf;
+  xrtk::DBuf stage, rays64, pix64, perm, work, meta, outbuf;
   // resident ray sets
   std::map<uint64_t, xrtk::RaySet> raysets;
   uint64_t next_rs = 1;
@@ -103,8 +125,9 @@ struct xrtk_ctx {
   struct GraphEntry {
     std::vector<uint64_t> key;
     hipGraphExec_t exec = nullptr;
-    xrtk::DBuf work, meta, pix, outd;
+    xrtk::DBuf work, meta, pix, perm, outd;
     std::vector<std::pair<double*, size_t>> outs;  // device out and np of each job
+    xrtk::Plan plan;
   };
   std::vector<GraphEntry*> graphs;
 };
@@ -113,20 +136,36 @@ namespace xrtk {
 
 // Chooses the launch configuration for a solve of nr rays on np pixels.
 LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o);
+// Default FP64 configuration paired with a given F32R configuration.
+LaunchCfg cfg64_for(const LaunchCfg& c32);
 // Ray chunk length of one instance (a multiple of cfg.tile), intrinsic to the
 // instance shape and the launch configuration.
-uint32_t chunk_len_for(const xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg, uint32_t nr,
-                       uint32_t np, int split_override);
+uint32_t chunk_len_for(const xrtk_ctx* ctx, const LaunchCfg& cfg, uint32_t nr, uint32_t npad,
+                       int split_override);
+// Slot count of np pixels: a multiple of both launches' slot blocks.
+uint32_t padded_slots(uint32_t np, const LaunchCfg& c32, const LaunchCfg& c64);
+// Tile shape (rows of ix, columns of iz) of 64 * P slots.
+void tile_shape(int P, uint32_t* th, uint32_t* tw);
+// Slots of a tiled nx x nz grid.
+uint32_t grid_slots(uint32_t nx, uint32_t nz, const LaunchCfg& c32, const LaunchCfg& c64);
+
+// Host-side routing: the bound of the tile-referenced residual phase for rays
+// in `rays` and pixels in `pix`, with in-tile half-extent `dq` per axis.
+// Returns +inf when a ray may lie on a pixel.
+double residual_bound_rev(const Box& rays, const Box& pix, const double dq[3], double energy);
+// Mode of an instance requested as `mode`: F32R unless the bound says FP64.
+uint32_t route_mode(uint32_t mode, const Box& rays, const Box& pix, const double dq[3],
+                    double energy);
 
 // Runs prep, pair and merge for `specs` on ctx->stream (no host sync, no
 // host copies).  work/meta are grown as needed.  When `timed`, HIP events
-// bracket the pair kernel and the whole device pipeline.
-// With upload == false the metadata copies are skipped (they were made by an
-// earlier call with the same specs and buffers), so the call only enqueues
-// kernels and can be captured into a hipGraph.
-int run_instances(xrtk_ctx* ctx, uint32_t mode, const LaunchCfg& cfg,
+// bracket the pair kernels and the whole device pipeline.  With upload ==
+// false the metadata copies are skipped (made by an earlier call with the
+// same specs and buffers), so the call only enqueues kernels and can be
+// captured into a hipGraph.
+int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
-                  Plan* plan_out, int split_override, uint32_t force_flags, bool persistent_sorted,
+                  Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
                   bool upload = true);
 
 int map_err(hipError_t e);

@@ -29,7 +29,7 @@ STATUS_NAMES = {
     ESHUTDOWN: "XRTK_ESHUTDOWN",
 }
 MODE_F32R, MODE_F64 = 0, 1
-RAYSRC_AUTO, RAYSRC_SMEM, RAYSRC_LDS = 0, 1, 2
+RAYSRC_AUTO, RAYSRC_SMEM = 0, 1
 CHBAR = 1973.2697177417986  # src/core/consts.rs:74
 
 
@@ -78,7 +78,7 @@ class XrtkJob(C.Structure):
 class XrtkPlan(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "mode", "pixels_per_lane", "waves_per_group", "tile_rays", "split", "ray_src",
-        "persistent", "n_items", "checked", "backend")]
+        "persistent", "n_items", "n_fp64", "backend")]
 
 
 class XrtkServerCfg(C.Structure):
