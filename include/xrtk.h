@@ -257,7 +257,8 @@ int xrtk_server_create(xrtk_ctx *ctx, const XrtkServerCfg *cfg,
                        xrtk_server **out);
 /* Queues one solve (non-blocking).  out4 must stay valid until the ticket
  * completes.  Returns XRTK_ENOMEM at once when the request alone exceeds the
- * budget. */
+ * budget.  The request holds its ray set until the ticket is collected by
+ * xrtk_server_wait (xrtk_rayset_release returns XRTK_EAGAIN meanwhile). */
 int xrtk_server_submit(xrtk_server *s, const XrtkJob *job,
                        xrtk_ticket *ticket);
 /* XRTK_OK when done, XRTK_EAGAIN while pending. */

@@ -1,0 +1,137 @@
+"""Continuous-batching server: results equal the standalone solve bit for bit,
+alone and in a mix; requests overlap on the device; budget admission."""
+import threading
+
+import numpy as np
+import pytest
+
+import xrtk
+from xrtk import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def grid(scr):
+    return xrtk.WaveParams(scr.center, scr.dx, scr.dz, scr.nx, scr.nz)
+
+
+@pytest.fixture(scope="module")
+def sets(ctx):
+    specs = W.batch_instances(n=8, seed=31, max_rays=120000, max_side=160)
+    rsets = [xrtk.RaySet(ctx, W.instance_rays(s)) for s in specs]
+    yield specs, rsets
+    for r in rsets:
+        r.release()
+
+
+def jobs_of(specs, rsets, energies=None):
+    out = []
+    for i, (s, r) in enumerate(zip(specs, rsets)):
+        e = s["energy"] if energies is None else energies[i]
+        out.append(xrtk.Job(r, e, grid(W.instance_screen(s))))
+    return out
+
+
+def test_server_results_equal_standalone(ctx, sets):
+    specs, rsets = sets
+    jobs = jobs_of(specs, rsets)
+    alone = [xrtk.solve_jobs(ctx, [j])[0] for j in jobs]
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=2.0)
+    try:
+        tickets = [srv.submit(j) for j in jobs]
+        for t, a in zip(tickets, alone):
+            got, ts = srv.wait(t, timeout_ms=60000)
+            assert np.array_equal(got, a)
+            assert ts[0] <= ts[1] <= ts[2] <= ts[3]
+        st = srv.stats()
+        assert st["completed"] == len(jobs) and st["max_concurrent"] > 1
+    finally:
+        srv.close()
+
+
+def test_alone_and_in_mix_identical(ctx, sets):
+    """The same request solved alone and while others arrive and run."""
+    specs, rsets = sets
+    jobs = jobs_of(specs, rsets)
+    target = jobs[2]
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=1.0)
+    try:
+        alone, _ = srv.wait(srv.submit(target), timeout_ms=60000)
+        tickets = []
+        lock = threading.Lock()
+
+        def feeder(js):
+            for j in js:
+                t = srv.submit(j)
+                with lock:
+                    tickets.append(t)
+
+        th = [threading.Thread(target=feeder, args=(jobs[k::2],)) for k in range(2)]
+        for t in th:
+            t.start()
+        mixed_t = srv.submit(target)
+        for t in th:
+            t.join()
+        mixed, _ = srv.wait(mixed_t, timeout_ms=60000)
+        for t in tickets:
+            srv.wait(t, timeout_ms=60000)
+        assert np.array_equal(alone, mixed)
+        assert srv.stats()["max_concurrent"] > 1
+    finally:
+        srv.close()
+
+
+def test_energy_scan_reuses_resident_rays(ctx, sets):
+    specs, rsets = sets
+    s, r = specs[0], rsets[0]
+    energies = [8000.0, 9000.0, 12000.0]
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30)
+    try:
+        ts = [srv.submit(xrtk.Job(r, e, grid(W.instance_screen(s)))) for e in energies]
+        outs = [srv.wait(t, timeout_ms=60000)[0] for t in ts]
+    finally:
+        srv.close()
+    for e, o in zip(energies, outs):
+        ref = xrtk.solve_jobs(ctx, [xrtk.Job(r, e, grid(W.instance_screen(s)))])[0]
+        assert np.array_equal(o, ref)
+
+
+def test_budget_rejects_and_queues(ctx, sets):
+    specs, rsets = sets
+    jobs = jobs_of(specs, rsets)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=1 << 20)  # 1 MiB: nothing fits
+    try:
+        with pytest.raises(xrtk.XrtkError) as ei:
+            srv.submit(jobs[0])
+        assert ei.value.code == xrtk._lib.ENOMEM
+        assert srv.stats()["rejected"] == 1
+    finally:
+        srv.close()
+    # a budget that holds one request at a time still completes them all
+    big = max(jobs, key=lambda j: j.rays.n * j.screen.nx * j.screen.nz)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=3 << 30, max_active=1)
+    try:
+        ts = [srv.submit(j) for j in jobs]
+        for t in ts:
+            srv.wait(t, timeout_ms=120000)
+        st = srv.stats()
+        assert st["completed"] == len(jobs) and st["max_concurrent"] == 1
+    finally:
+        srv.close()
+    assert big is not None
+
+
+def test_release_of_busy_rayset_refused(ctx, sets):
+    specs, rsets = sets
+    rays = W.instance_rays(specs[1])
+    rs = xrtk.RaySet(ctx, rays)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30)
+    try:
+        t = srv.submit(xrtk.Job(rs, specs[1]["energy"], grid(W.instance_screen(specs[1]))))
+        with pytest.raises(xrtk.XrtkError) as ei:
+            rs.release()
+        assert ei.value.code in (xrtk._lib.EAGAIN,)
+        srv.wait(t, timeout_ms=60000)
+    finally:
+        srv.close()
+    rs.release()

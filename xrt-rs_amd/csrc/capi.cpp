@@ -618,6 +618,7 @@ int xrtk_rayset_release(xrtk_ctx* ctx, xrtk_rayset rs) {
   std::lock_guard<std::mutex> lk(ctx->mu);
   auto it = ctx->raysets.find(rs);
   if (it == ctx->raysets.end()) return XRTK_EINVAL;
+  if (it->second.users->load() > 0) return XRTK_EAGAIN;  // a server still holds it
   DeviceGuard g(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   // cached batch graphs may reference this ray set: drop them

@@ -3,9 +3,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -77,6 +79,7 @@ struct RaySet {
   bool mono = true;   // all energies equal
   double energy = 0;  // the common energy when mono
   Box box;            // positions
+  std::shared_ptr<std::atomic<int>> users = std::make_shared<std::atomic<int>>(0);
 };
 
 // Description of one instance for a launch (all pointers are device memory).

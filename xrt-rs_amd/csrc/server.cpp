@@ -1,10 +1,620 @@
-// placeholder, replaced by the serving loop
+// Continuous-batching solver.
+//
+// A scheduler thread owns one HIP stream.  Requests (a resident ray set at
+// one energy on one prepare_wave screen) queue on submit.  Each scheduler
+// iteration
+//   1. admits queued requests while the HBM budget and the active-request
+//      cap allow (their buffers come from a caching pool; their grid and
+//      prep kernels are enqueued at once),
+//   2. fills one "quantum" launch of the persistent pair kernel with work
+//      items taken oldest-request-first from every active request, so a
+//      request that arrives while others run joins at the next quantum and
+//      shares the device with them,
+//   3. enqueues, for each request whose last items went into that launch,
+//      its FP64 merge and an async copy of the result into pinned memory,
+//   4. retires requests whose copy has completed (result handed to the
+//      caller's buffer, timestamps recorded, waiters woken).
+// At most two quanta are in flight, so the host prepares the next launch
+// while the device runs the current one.  A request's plan (chunking, slot
+// layout, kernel shape) depends only on the request, so its result does not
+// depend on which other requests share its launches.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
 #include "runtime.h"
-extern "C" {
-int xrtk_server_create(xrtk_ctx*, const XrtkServerCfg*, xrtk_server** out) { if (out) *out = nullptr; return XRTK_EINVAL; }
-int xrtk_server_submit(xrtk_server*, const XrtkJob*, xrtk_ticket*) { return XRTK_EINVAL; }
-int xrtk_server_poll(xrtk_server*, xrtk_ticket) { return XRTK_EINVAL; }
-int xrtk_server_wait(xrtk_server*, xrtk_ticket, double, double*) { return XRTK_EINVAL; }
-int xrtk_server_stats(xrtk_server*, XrtkServerStats*) { return XRTK_EINVAL; }
-void xrtk_server_destroy(xrtk_server*) {}
+
+using namespace xrtk;
+
+namespace xrtk {
+int build_job_specs(xrtk_ctx* ctx, uint32_t want, const LaunchCfg& c32, const LaunchCfg& c64,
+                    const XrtkJob* jobs, size_t n, DBuf& pixbuf, DBuf& permbuf, DBuf& outbuf,
+                    std::vector<InstSpec>& specs);
 }
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t au(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// Caching pool for device blocks or pinned host blocks.  Sizes round up to
+// one of 8 steps per power of two (<= 12.5% slack); a cached block up to 2x
+// the request is reused.  `allocated` counts every block held, cached or not.
+struct Pool {
+  bool pinned = false;
+  std::multimap<size_t, void*> free_;
+  size_t allocated = 0;
+  static size_t bucket(size_t b) {
+    if (b <= ((size_t)1 << 20)) return (size_t)1 << 20;
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    const size_t step = p / 16;
+    return (b + step - 1) / step * step;
+  }
+  void drop(std::multimap<size_t, void*>::iterator it) {
+    if (pinned) (void)hipHostFree(it->second);
+    else (void)hipFree(it->second);
+    allocated -= it->first;
+    free_.erase(it);
+  }
+  // nullptr when the block cannot be had within `limit` bytes in total
+  void* get(size_t bytes, size_t limit, size_t* got) {
+    const size_t b = bucket(bytes);
+    auto it = free_.lower_bound(b);
+    if (it != free_.end() && it->first <= 2 * b) {
+      void* p = it->second;
+      *got = it->first;
+      free_.erase(it);
+      return p;
+    }
+    while (!free_.empty() && allocated + b > limit) drop(std::prev(free_.end()));
+    if (allocated + b > limit) return nullptr;
+    void* p = nullptr;
+    hipError_t e = pinned ? hipHostMalloc(&p, b, hipHostMallocDefault) : hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    allocated += b;
+    *got = b;
+    return p;
+  }
+  void put(void* p, size_t b) { free_.emplace(b, p); }
+  void release_all() {
+    while (!free_.empty()) drop(free_.begin());
+  }
+};
+
+enum ReqState { QUEUED, ACTIVE, MERGING, DONE, FAILED };
+
+struct Req {
+  uint64_t id = 0;
+  XrtkJob job{};
+  double t[4] = {0, 0, 0, 0};
+  int state = QUEUED;
+  int rc = XRTK_OK;
+  // plan
+  InstSpec spec;
+  LaunchCfg cfg;
+  uint32_t chunk_len = 0, n_chunks = 0, pblocks = 0;
+  uint64_t n_items = 0, next_item = 0;
+  double item_pairs = 0;  // pairs of one full item
+  size_t bytes = 0;
+  // device block and its carve-up
+  void* dblk = nullptr;
+  size_t dblk_size = 0;
+  void* hblk = nullptr;  // pinned result staging
+  size_t hblk_size = 0;
+  KInst kinst{};
+  KInst* d_kinst = nullptr;
+  uint32_t* d_list = nullptr;
+  hipEvent_t done_ev = nullptr;
+  std::shared_ptr<std::atomic<int>> users;  // pending requests on the ray set
+};
+
+struct Launch {
+  hipEvent_t start = nullptr, end = nullptr;
+  bool in_flight = false;
+  size_t n_req = 0;
+  double pairs = 0;
+  void* meta = nullptr;  // device: KInst table + items + queue
+  void* hmeta = nullptr; // pinned host staging of the same bytes
+  size_t meta_size = 0;
+  std::vector<KInst> h_insts;
+  std::vector<WorkItem> h_items32, h_items64;
+};
+
+}  // namespace
+
+struct xrtk_server {
+  xrtk_ctx* ctx = nullptr;
+  XrtkServerCfg cfg{};
+  LaunchCfg c32, c64;
+  hipStream_t stream = nullptr;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  uint64_t next_id = 1;
+  std::deque<Req*> queue;               // submitted, not admitted
+  std::vector<Req*> active;             // admitted, items left or merging
+  std::map<uint64_t, Req*> all;         // every request not yet collected
+  Pool dpool, hpool;
+  size_t hbm_in_use = 0, hbm_peak = 0;
+  XrtkServerStats st{};
+  Launch launches[2];
+  int cur = 0;
+  double rate_pairs_per_s = 1.2e12;     // refined from measured quanta
+
+  // ---- scheduler helpers (called with `mu` held unless stated) ----
+  bool plan_request(Req* r) {
+    std::vector<InstSpec> specs;
+    // build the spec's pixel/perm/out layout sizes only (no allocation)
+    const XrtkJob& j = r->job;
+    const uint32_t np = j.screen.nx * j.screen.nz;
+    auto it = ctx->raysets.find(j.rays);
+    if (it == ctx->raysets.end()) return false;
+    const RaySet& rs = it->second;
+    const double e = j.energy > 0 ? j.energy : rs.energy;
+    if (cfg.mode == XRTK_MODE_F32R && !(e > 0)) return false;
+    Box pb;
+    pb.add(j.screen.center[0] - std::fabs(j.screen.dx), j.screen.center[1],
+           j.screen.center[2] - std::fabs(j.screen.dz));
+    pb.add(j.screen.center[0] + std::fabs(j.screen.dx), j.screen.center[1],
+           j.screen.center[2] + std::fabs(j.screen.dz));
+    uint32_t th, tw;
+    tile_shape(c32.P, &th, &tw);
+    const double dq[3] = {
+        j.screen.nx > 1 ? 2.0 * std::fabs(j.screen.dx) / (j.screen.nx - 1) * th : 0.0, 0.0,
+        j.screen.nz > 1 ? 2.0 * std::fabs(j.screen.dz) / (j.screen.nz - 1) * tw : 0.0};
+    InstSpec& s = r->spec;
+    s.mode = route_mode(cfg.mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R, rs.box, pb, dq, e);
+    s.rays = rs.rays.as<RayRec64>();
+    s.nr = rs.nr;
+    s.np = np;
+    s.npad = np ? grid_slots(j.screen.nx, j.screen.nz, c32, c64) : 0;
+    s.energy = s.mode == MODE_F32R ? e : (j.energy > 0 ? j.energy : 0.0);
+    s.skip = 1e-30;
+    s.static_flags = rs.has_path ? INST_F_PATHUP : 0u;
+    r->cfg = s.mode == MODE_F32R ? c32 : c64;
+    r->chunk_len = chunk_len_for(ctx, r->cfg, s.nr, s.npad, 0);
+    r->n_chunks = s.nr ? (s.nr + r->chunk_len - 1) / r->chunk_len : 0;
+    const uint32_t B = 64u * r->cfg.W * r->cfg.P;
+    r->pblocks = s.npad / B;
+    r->n_items = (uint64_t)r->n_chunks * r->pblocks;
+    r->item_pairs = (double)B * r->chunk_len;
+    // device block: pix64, perm, rays32, pix32, tiles, R64, partial, scr,
+    // part, out, KInst, list
+    const bool f32 = s.mode == MODE_F32R;
+    size_t b = 0;
+    b += au(sizeof(PixF64) * s.npad) + au(sizeof(uint32_t) * s.npad);
+    b += au(f32 ? sizeof(RayRec32) * s.nr : 0) + au(f32 ? sizeof(PixRec32) * s.npad : 0);
+    b += au(f32 ? sizeof(TileRec) * (s.npad / (64u * c32.P)) : 0);
+    b += au(sizeof(double) * s.npad);
+    b += au((f32 ? 16 : 32) * (size_t)s.npad * r->n_chunks);
+    b += au(sizeof(InstScratch)) + au(sizeof(double) * 4 * kPartBlocks);
+    b += au(sizeof(double) * 4 * (size_t)np) + au(sizeof(KInst)) + au(64);
+    r->bytes = b;
+    return true;
+  }
+
+  // Carves the block, fills the KInst, enqueues grid + prep.
+  int start_request(Req* r) {
+    char* p = static_cast<char*>(r->dblk);
+    InstSpec& s = r->spec;
+    const bool f32 = s.mode == MODE_F32R;
+    auto take = [&](size_t bytes) { char* q = p; p += au(bytes); return q; };
+    PixF64* pix = reinterpret_cast<PixF64*>(take(sizeof(PixF64) * s.npad));
+    uint32_t* perm = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * s.npad));
+    RayRec32* rays32 = reinterpret_cast<RayRec32*>(take(f32 ? sizeof(RayRec32) * s.nr : 0));
+    PixRec32* pix32 = reinterpret_cast<PixRec32*>(take(f32 ? sizeof(PixRec32) * s.npad : 0));
+    TileRec* tiles = reinterpret_cast<TileRec*>(
+        take(f32 ? sizeof(TileRec) * (s.npad / (64u * c32.P)) : 0));
+    double* R64 = reinterpret_cast<double*>(take(sizeof(double) * s.npad));
+    void* partial = take((f32 ? 16 : 32) * (size_t)s.npad * r->n_chunks);
+    InstScratch* scr = reinterpret_cast<InstScratch*>(take(sizeof(InstScratch)));
+    double* part = reinterpret_cast<double*>(take(sizeof(double) * 4 * kPartBlocks));
+    double* out = reinterpret_cast<double*>(take(sizeof(double) * 4 * (size_t)s.np));
+    r->d_kinst = reinterpret_cast<KInst*>(take(sizeof(KInst)));
+    r->d_list = reinterpret_cast<uint32_t*>(take(64));
+    s.pix = pix;
+    s.perm = perm;
+    s.out = out;
+    KInst& I = r->kinst;
+    std::memset(&I, 0, sizeof(I));
+    I.rays64 = s.rays; I.rays32 = rays32; I.pix64 = pix; I.pix32 = pix32; I.tiles = tiles;
+    I.perm = perm; I.R64 = R64; I.partial = partial; I.out = out; I.scr = scr;
+    I.nr = s.nr; I.np = s.np; I.npad = s.npad; I.n_chunks = r->n_chunks;
+    I.chunk_len = r->chunk_len; I.mode = s.mode; I.tile = c32.tile; I.accumulate = 0;
+    I.tile_slots = 64u * c32.P;
+    I.energy = s.energy;
+    I.k = s.energy / kCHBAR_H * 1e7;
+    {
+      const long double two_pi = 6.283185307179586476925286766559005768L;
+      const long double kr = (long double)I.k / two_pi;
+      I.kr_hi = (double)kr;
+      I.kr_lo = (double)(kr - (long double)I.kr_hi);
+    }
+    I.skip = s.skip;
+    uint32_t th, tw;
+    tile_shape(c32.P, &th, &tw);
+    const XrtkWaveParams& w = r->job.screen;
+    // KInst and the one-entry merge list go up through the pinned block (the
+    // result lands there later, after these copies in stream order)
+    char* hs = static_cast<char*>(r->hblk);
+    std::memcpy(hs, &I, sizeof(KInst));
+    std::memset(hs + sizeof(KInst), 0, 64);
+    hipError_t e = hipMemcpyAsync(r->d_kinst, hs, sizeof(KInst), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r->d_list, hs + sizeof(KInst), sizeof(uint32_t), hipMemcpyHostToDevice,
+                         stream);
+    if (e == hipSuccess && s.npad)
+      e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, s.npad, pix, perm, stream);
+    if (e == hipSuccess && s.nr)
+      e = launch_prep(s.rays, s.nr, pix, s.npad, scr, part, s.mode, I.kr_hi, I.kr_lo, s.skip, 0,
+                      s.static_flags, c32.P, rays32, pix32, tiles, R64, stream);
+    if (e == hipSuccess && !s.nr && s.np) e = launch_zero_f64(out, (size_t)s.np * 4, stream);
+    return map_err(e);
+  }
+
+  void admit() {
+    while (!queue.empty() && (int)active.size() < cfg.max_active) {
+      Req* r = queue.front();
+      if (hbm_in_use + Pool::bucket(r->bytes) > cfg.hbm_budget_bytes) break;  // wait
+      size_t got = 0;
+      void* d = dpool.get(r->bytes, cfg.hbm_budget_bytes, &got);
+      if (!d) break;  // retry when something retires
+      const size_t hb = std::max<size_t>(sizeof(double) * 4 * r->spec.np, 512);
+      size_t hgot = 0;
+      void* h = hpool.get(hb, (size_t)1 << 62, &hgot);
+      if (!h) {
+        dpool.put(d, got);
+        break;
+      }
+      queue.pop_front();
+      r->dblk = d; r->dblk_size = got; r->hblk = h; r->hblk_size = hgot;
+      hbm_in_use += got;
+      hbm_peak = std::max(hbm_peak, hbm_in_use);
+      r->t[1] = now_s();
+      r->state = ACTIVE;
+      r->rc = start_request(r);
+      if (r->rc != XRTK_OK) {
+        (void)hipStreamSynchronize(stream);
+        r->state = FAILED;
+        finish(r);
+        continue;
+      }
+      st.admitted++;
+      active.push_back(r);
+      if (r->n_items == 0) enqueue_merge(r);
+    }
+  }
+
+  void enqueue_merge(Req* r) {
+    r->state = MERGING;
+    hipError_t e = hipSuccess;
+    if (r->spec.nr) e = launch_merge(r->d_kinst, r->d_list, 1, r->spec.npad, stream);
+    if (e == hipSuccess && r->spec.np)
+      e = hipMemcpyAsync(r->hblk, r->spec.out, sizeof(double) * 4 * r->spec.np,
+                         hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipEventRecord(r->done_ev, stream);
+    if (e != hipSuccess) r->rc = map_err(e);
+  }
+
+  // Fills and enqueues one quantum.  Returns false when there was no work.
+  bool launch_quantum() {
+    Launch& L = launches[cur];  // the caller has retired it
+    const double budget = std::max(1.0, cfg.quantum_ms) * 1e-3 * rate_pairs_per_s;
+    double pairs = 0;
+    L.h_insts.clear();
+    L.h_items32.clear();
+    L.h_items64.clear();
+    std::vector<Req*> finishing;
+    for (Req* r : active) {
+      if (r->state != ACTIVE || r->next_item >= r->n_items) continue;
+      if (pairs >= budget && !L.h_insts.empty()) break;
+      const uint32_t li = (uint32_t)L.h_insts.size();
+      L.h_insts.push_back(r->kinst);
+      const uint32_t B = 64u * r->cfg.W * r->cfg.P;
+      auto& items = r->spec.mode == MODE_F32R ? L.h_items32 : L.h_items64;
+      while (r->next_item < r->n_items && (pairs < budget || items.empty())) {
+        const uint64_t it = r->next_item++;
+        const uint32_t chunk = (uint32_t)(it / r->pblocks);
+        const uint32_t pb = (uint32_t)(it % r->pblocks);
+        items.push_back(WorkItem{li, pb * B, chunk, 0});
+        const double len = std::min<double>(r->chunk_len, (double)r->spec.nr - (double)chunk * r->chunk_len);
+        pairs += (double)B * len;
+      }
+      if (r->t[2] == 0) r->t[2] = now_s();
+      if (r->next_item >= r->n_items) finishing.push_back(r);
+    }
+    if (L.h_insts.empty()) return false;
+    // metadata
+    const size_t a_ins = au(sizeof(KInst) * L.h_insts.size());
+    const size_t a_32 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items32.size()));
+    const size_t a_64 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items64.size()));
+    const size_t need = a_ins + a_32 + a_64 + 256;
+    if (need > L.meta_size) {
+      if (L.meta) (void)hipFree(L.meta);
+      if (L.hmeta) (void)hipHostFree(L.hmeta);
+      L.meta = L.hmeta = nullptr;
+      L.meta_size = std::max(need * 2, (size_t)1 << 20);
+      if (hipMalloc(&L.meta, L.meta_size) != hipSuccess ||
+          hipHostMalloc(&L.hmeta, L.meta_size, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        L.meta_size = 0;
+        // give the items back; the next iteration retries
+        for (Req* r : active) (void)r;
+        return false;
+      }
+    }
+    char* M = static_cast<char*>(L.meta);
+    char* H = static_cast<char*>(L.hmeta);
+    std::memcpy(H, L.h_insts.data(), sizeof(KInst) * L.h_insts.size());
+    std::memcpy(H + a_ins, L.h_items32.data(), sizeof(WorkItem) * L.h_items32.size());
+    std::memcpy(H + a_ins + a_32, L.h_items64.data(), sizeof(WorkItem) * L.h_items64.size());
+    KInst* d_ins = reinterpret_cast<KInst*>(M);
+    WorkItem* d_32 = reinterpret_cast<WorkItem*>(M + a_ins);
+    WorkItem* d_64 = reinterpret_cast<WorkItem*>(M + a_ins + a_32);
+    uint32_t* d_q = reinterpret_cast<uint32_t*>(M + a_ins + a_32 + a_64);
+    hipError_t e = hipMemcpyAsync(M, H, a_ins + a_32 + a_64, hipMemcpyHostToDevice, stream);
+    (void)hipEventRecord(L.start, stream);
+    auto grid_of = [&](const LaunchCfg& c, uint32_t mode, size_t n) {
+      const uint32_t res = (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / c.W;
+      return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n, res));
+    };
+    if (e == hipSuccess && !L.h_items32.empty())
+      e = launch_pair(MODE_F32R, c32, d_ins, d_32, (uint32_t)L.h_items32.size(), d_q,
+                      grid_of(c32, MODE_F32R, L.h_items32.size()), 0, stream);
+    if (e == hipSuccess && !L.h_items64.empty())
+      e = launch_pair(MODE_F64, c64, d_ins, d_64, (uint32_t)L.h_items64.size(), d_q + 16,
+                      grid_of(c64, MODE_F64, L.h_items64.size()), 0, stream);
+    (void)hipEventRecord(L.end, stream);
+    L.in_flight = true;
+    L.n_req = L.h_insts.size();
+    L.pairs = pairs;
+    st.launches++;
+    st.max_concurrent = std::max<uint64_t>(st.max_concurrent, L.n_req);
+    if (e != hipSuccess) {
+      for (Req* r : finishing) r->rc = map_err(e);
+    }
+    for (Req* r : finishing) enqueue_merge(r);
+    cur ^= 1;
+    return true;
+  }
+
+  void retire_launch(Launch& L) {
+    if (!L.in_flight) return;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, L.start, L.end) == hipSuccess && ms > 0.05f) {
+      st.busy_ms += ms;
+      const double r = L.pairs / (ms * 1e-3);
+      rate_pairs_per_s = 0.8 * rate_pairs_per_s + 0.2 * r;
+    } else {
+      (void)hipGetLastError();
+    }
+    st.pairs_done += L.pairs;
+    L.in_flight = false;
+  }
+
+  // Retires merged requests whose copy has landed.
+  void reap() {
+    for (size_t i = 0; i < active.size();) {
+      Req* r = active[i];
+      if (r->state == MERGING) {
+        const hipError_t q = hipEventQuery(r->done_ev);
+        if (q == hipErrorNotReady) { ++i; continue; }
+        if (q != hipSuccess && r->rc == XRTK_OK) r->rc = XRTK_ELAUNCH;
+        (void)hipGetLastError();
+        if (r->rc == XRTK_OK && r->spec.np)
+          std::memcpy(r->job.out4, r->hblk, sizeof(double) * 4 * r->spec.np);
+        r->state = r->rc == XRTK_OK ? DONE : FAILED;
+        active.erase(active.begin() + i);
+        finish(r);
+        continue;
+      }
+      ++i;
+    }
+  }
+
+  void finish(Req* r) {
+    r->t[3] = now_s();
+    if (r->dblk) {
+      dpool.put(r->dblk, r->dblk_size);
+      hbm_in_use -= r->dblk_size;
+      r->dblk = nullptr;
+    }
+    if (r->hblk) {
+      hpool.put(r->hblk, r->hblk_size);
+      r->hblk = nullptr;
+    }
+    st.completed++;
+    cv_done.notify_all();
+  }
+
+  void loop() {
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      admit();
+      Launch& L = launches[cur];
+      if (L.in_flight) {  // keep at most two quanta in flight
+        lk.unlock();
+        (void)hipEventSynchronize(L.end);
+        lk.lock();
+        retire_launch(L);
+      }
+      const bool launched = launch_quantum();
+      reap();
+      if (stop && queue.empty() && active.empty()) break;
+      if (!launched) {
+        bool merging = false;
+        for (Req* r : active) merging |= r->state == MERGING;
+        if (merging || !queue.empty()) {
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(100));
+          lk.lock();
+        } else {
+          cv_work.wait_for(lk, std::chrono::milliseconds(20));
+        }
+      }
+    }
+    for (Launch& L2 : launches)
+      if (L2.in_flight) {
+        lk.unlock();
+        (void)hipEventSynchronize(L2.end);
+        lk.lock();
+        retire_launch(L2);
+      }
+  }
+};
+
+extern "C" {
+
+int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** out) {
+  if (!out) return XRTK_EINVAL;
+  *out = nullptr;
+  if (!ctx) return XRTK_ENODEV;
+  if (!cfg || cfg->hbm_budget_bytes == 0 || cfg->max_active <= 0) return XRTK_EINVAL;
+  try {
+    std::unique_ptr<xrtk_server> s(new xrtk_server());
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (s->cfg.quantum_ms <= 0) s->cfg.quantum_ms = 10.0;
+    s->c32.P = 4; s->c32.W = 4; s->c32.ray_src = 1; s->c32.persistent = 1; s->c32.tile = 8192;
+    s->c64 = cfg64_for(s->c32);
+    s->hpool.pinned = true;
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (hipSetDevice(ctx->device) != hipSuccess) return XRTK_ENODEV;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+      return XRTK_ELAUNCH;
+    for (Launch& L : s->launches) {
+      if (hipEventCreate(&L.start) != hipSuccess || hipEventCreate(&L.end) != hipSuccess)
+        return XRTK_ELAUNCH;
+    }
+    if (dev >= 0) (void)hipSetDevice(dev);
+    xrtk_server* raw = s.get();
+    s->th = std::thread([raw] { raw->loop(); });
+    *out = s.release();
+    return XRTK_OK;
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+int xrtk_server_submit(xrtk_server* s, const XrtkJob* job, xrtk_ticket* ticket) {
+  if (!s || !job || !ticket) return XRTK_EINVAL;
+  try {
+    std::unique_ptr<Req> r(new Req());
+    r->job = *job;
+    r->t[0] = now_s();
+    {
+      std::lock_guard<std::mutex> lk(s->ctx->mu);
+      auto it = s->ctx->raysets.find(job->rays);
+      if (it == s->ctx->raysets.end()) return XRTK_EINVAL;
+      if (!s->plan_request(r.get())) return XRTK_EINVAL;
+      r->users = it->second.users;
+    }
+    if (r->spec.np && !job->out4) return XRTK_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.submitted++;
+    if (s->stop) return XRTK_ESHUTDOWN;
+    if (Pool::bucket(r->bytes) > s->cfg.hbm_budget_bytes) {
+      s->st.rejected++;
+      return XRTK_ENOMEM;
+    }
+    if (hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming) != hipSuccess)
+      return XRTK_ELAUNCH;
+    r->users->fetch_add(1);
+    r->id = s->next_id++;
+    *ticket = r->id;
+    Req* raw = r.release();
+    s->all[raw->id] = raw;
+    s->queue.push_back(raw);
+    s->cv_work.notify_all();
+    return XRTK_OK;
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+int xrtk_server_poll(xrtk_server* s, xrtk_ticket t) {
+  if (!s) return XRTK_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(t);
+  if (it == s->all.end()) return XRTK_EINVAL;
+  const Req* r = it->second;
+  if (r->state == DONE) return XRTK_OK;
+  if (r->state == FAILED) return r->rc ? r->rc : XRTK_ELAUNCH;
+  return XRTK_EAGAIN;
+}
+
+int xrtk_server_wait(xrtk_server* s, xrtk_ticket t, double timeout_ms, double t_out[4]) {
+  if (!s) return XRTK_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(t);
+  if (it == s->all.end()) return XRTK_EINVAL;
+  Req* r = it->second;
+  auto done = [&] { return r->state == DONE || r->state == FAILED; };
+  if (timeout_ms < 0) {
+    s->cv_done.wait(lk, done);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms),
+                                  done)) {
+    return XRTK_ETIMEDOUT;
+  }
+  if (t_out)
+    for (int k = 0; k < 4; ++k) t_out[k] = r->t[k];
+  const int rc = r->state == DONE ? XRTK_OK : (r->rc ? r->rc : XRTK_ELAUNCH);
+  if (r->users) r->users->fetch_sub(1);  // the ray set is held until collected
+  (void)hipEventDestroy(r->done_ev);
+  s->all.erase(it);
+  delete r;
+  return rc;
+}
+
+int xrtk_server_stats(xrtk_server* s, XrtkServerStats* st) {
+  if (!s || !st) return XRTK_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *st = s->st;
+  st->hbm_in_use = s->hbm_in_use;
+  st->hbm_peak = s->hbm_peak;
+  return XRTK_OK;
+}
+
+void xrtk_server_destroy(xrtk_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->th.joinable()) s->th.join();
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->stream);
+  for (auto& kv : s->all) {
+    if (kv.second->users) kv.second->users->fetch_sub(1);
+    if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
+    delete kv.second;
+  }
+  s->dpool.release_all();
+  s->hpool.release_all();
+  for (Launch& L : s->launches) {
+    if (L.meta) (void)hipFree(L.meta);
+    if (L.hmeta) (void)hipHostFree(L.hmeta);
+    if (L.start) (void)hipEventDestroy(L.start);
+    if (L.end) (void)hipEventDestroy(L.end);
+  }
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+}  // extern "C"
