@@ -78,6 +78,49 @@ __device__ __forceinline__ void cis_rev_dd(double a_hi, double a_lo, double x,
   sincospi(2.0 * f, s, c);
 }
 
+// Double-double helpers (Knuth TwoSum, FMA TwoProduct).
+__device__ __forceinline__ void two_sum(double a, double b, double* s, double* e) {
+  *s = a + b;
+  const double bp = *s - a;
+  *e = (a - (*s - bp)) + (b - bp);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double* p, double* e) {
+  *p = a * b;
+  *e = fma(a, b, -*p);
+}
+
+// |q| of q = (x_hi + x_lo, ...) as a double-double.
+__device__ __forceinline__ void norm_dd(const double (&h)[3], const double (&l)[3], double* rh,
+                                        double* rl) {
+  double sh = 0.0, sl = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double p, e;
+    two_prod(h[i], h[i], &p, &e);
+    e += 2.0 * h[i] * l[i];
+    double t, te;
+    two_sum(sh, p, &t, &te);
+    sl += te + e;
+    sh = t;
+  }
+  double s2, e2;
+  two_sum(sh, sl, &s2, &e2);  // renormalise
+  const double r0 = sqrt(s2);
+  double p, e;
+  two_prod(r0, r0, &p, &e);
+  const double corr = r0 > 0.0 ? ((s2 - p) - e + e2) / (2.0 * r0) : 0.0;
+  two_sum(r0, corr, rh, rl);
+}
+
+// exp(i 2 pi (a_hi + a_lo)(x_hi + x_lo)) for a large phase.
+__device__ __forceinline__ void cis_rev_dd2(double a_hi, double a_lo, double x_hi, double x_lo,
+                                            double* c, double* s) {
+  double hi, e;
+  two_prod(a_hi, x_hi, &hi, &e);
+  const double lo = e + a_hi * x_lo + a_lo * x_hi;
+  const double f = (hi - rint(hi)) + lo;
+  sincospi(2.0 * f, s, c);
+}
+
 __device__ __forceinline__ float wave_max(float v) {
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
@@ -88,6 +131,14 @@ __device__ __forceinline__ float wave_min(float v) {
 }
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_maxd(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_mind(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -269,8 +320,11 @@ __global__ __launch_bounds__(256) void k_ray_bounds(
 }
 
 // One wave per tile of tsz = 64 * P slots (P slots per lane, slot = t*tsz +
-// i*64 + lane): the tile reference point (mean q, FP64), the FP32 slot
-// records, R per slot in FP64, and the bounds of |q|, |dq| and |q|.
+// i*64 + lane).  q = p - c is kept as a double-double and |q| likewise (the
+// base phase k|q| reaches ~1e12 rad, where one FP64 rounding of |q| is
+// already ~1e-4 rad); the tile reference point q_t is the centre of the
+// tile's bounding box.  Writes the FP32 slot records, |q| per slot (two
+// doubles) and the tile record, and reduces the bounds of |q|, |dq|, R.
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix, double kr,
                                                   InstScratch* __restrict__ scr,
@@ -279,59 +333,63 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
                                                   double* __restrict__ R64) {
   const uint32_t lane = threadIdx.x;
   const uint32_t base = blockIdx.x * 64u * P;
-  const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
+  const double c[3] = {scr->cx, scr->cy, scr->cz};
   const bool uni = scr->n_differs == 0u;
   const double n0x = scr->n0x, n0y = scr->n0y, n0z = scr->n0z;
-  double qx[P], qy[P], qz[P];
-  double sx = 0, sy = 0, sz = 0;
+  double qh[P][3], ql[P][3];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const PixF64 p = pix[base + i * 64u + lane];
-    qx[i] = p.x - cx; qy[i] = p.y - cy; qz[i] = p.z - cz;
-    sx += qx[i]; sy += qy[i]; sz += qz[i];
+    const double pv[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; ++k) {
+      two_sum(pv[k], -c[k], &qh[i][k], &ql[i][k]);
+      lo[k] = fmin(lo[k], qh[i][k]);
+      hi[k] = fmax(hi[k], qh[i][k]);
+    }
   }
-  const double inv = 1.0 / (64.0 * P);
-  const double tx = wave_sum(sx) * inv, ty = wave_sum(sy) * inv, tz = wave_sum(sz) * inv;
-  const double Rt = sqrt(tx * tx + ty * ty + tz * tz);
-  if (lane == 0) tiles[blockIdx.x] = TileRec{tx, ty, tz, Rt};
-  float aqx = 0.f, aqy = 0.f, aqz = 0.f, adx = 0.f, ady = 0.f, adz = 0.f;
+  double t[3];
+  for (int k = 0; k < 3; ++k) t[k] = 0.5 * (wave_mind(lo[k]) + wave_maxd(hi[k]));
+  const double Rt = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  if (lane == 0) tiles[blockIdx.x] = TileRec{t[0], t[1], t[2], Rt};
+  float aq[3] = {0.f, 0.f, 0.f}, ad[3] = {0.f, 0.f, 0.f};
   float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
-    const double R2 = qx[i] * qx[i] + qy[i] * qy[i] + qz[i] * qz[i];
-    const double R = sqrt(R2);
-    const double dqx = qx[i] - tx, dqy = qy[i] - ty, dqz = qz[i] - tz;
+    double Rh, Rl;
+    norm_dd(qh[i], ql[i], &Rh, &Rl);
+    double dq[3];
+    for (int k = 0; k < 3; ++k) dq[k] = (qh[i][k] - t[k]) + ql[i][k];
     PixRec32 r;
-    r.mx = (float)(-2.0 * kr * dqx);
-    r.my = (float)(-2.0 * kr * dqy);
-    r.mz = (float)(-2.0 * kr * dqz);
-    r.R2 = (float)R2;
-    r.R = (float)R;
-    r.g = (float)((Rt - R) / Rt);
-    r.nq = uni ? (float)(n0x * qx[i] + n0y * qy[i] + n0z * qz[i]) : 0.f;
+    r.mx = (float)(-2.0 * kr * dq[0]);
+    r.my = (float)(-2.0 * kr * dq[1]);
+    r.mz = (float)(-2.0 * kr * dq[2]);
+    r.R2 = (float)(Rh * Rh);
+    r.R = (float)Rh;
+    r.g = (float)((Rt - Rh) / Rt);
+    r.nq = uni ? (float)(n0x * qh[i][0] + n0y * qh[i][1] + n0z * qh[i][2]) : 0.f;
     r.pad = 0.f;
     const uint32_t s = base + i * 64u + lane;
     out32[s] = r;
-    R64[s] = R;
-    aqx = fmaxf(aqx, __double2float_ru(fabs(qx[i])));
-    aqy = fmaxf(aqy, __double2float_ru(fabs(qy[i])));
-    aqz = fmaxf(aqz, __double2float_ru(fabs(qz[i])));
-    adx = fmaxf(adx, __double2float_ru(fabs(dqx)));
-    ady = fmaxf(ady, __double2float_ru(fabs(dqy)));
-    adz = fmaxf(adz, __double2float_ru(fabs(dqz)));
-    rmin = fminf(rmin, __double2float_rd(R));
-    rmax = fmaxf(rmax, __double2float_ru(R));
+    R64[2 * s] = Rh;
+    R64[2 * s + 1] = Rl;
+    for (int k = 0; k < 3; ++k) {
+      aq[k] = fmaxf(aq[k], __double2float_ru(fabs(qh[i][k])));
+      ad[k] = fmaxf(ad[k], __double2float_ru(fabs(dq[k])));
+    }
+    rmin = fminf(rmin, __double2float_rd(Rh));
+    rmax = fmaxf(rmax, __double2float_ru(Rh));
   }
-  aqx = wave_max(aqx); aqy = wave_max(aqy); aqz = wave_max(aqz);
-  adx = wave_max(adx); ady = wave_max(ady); adz = wave_max(adz);
+  for (int k = 0; k < 3; ++k) {
+    aq[k] = wave_max(aq[k]);
+    ad[k] = wave_max(ad[k]);
+  }
   rmin = wave_min(rmin); rmax = wave_max(rmax);
   if (lane == 0) {
-    atomicMax(&scr->qmax_bits[0], __float_as_uint(aqx));
-    atomicMax(&scr->qmax_bits[1], __float_as_uint(aqy));
-    atomicMax(&scr->qmax_bits[2], __float_as_uint(aqz));
-    atomicMax(&scr->dqmax_bits[0], __float_as_uint(adx));
-    atomicMax(&scr->dqmax_bits[1], __float_as_uint(ady));
-    atomicMax(&scr->dqmax_bits[2], __float_as_uint(adz));
+    for (int k = 0; k < 3; ++k) {
+      atomicMax(&scr->qmax_bits[k], __float_as_uint(aq[k]));
+      atomicMax(&scr->dqmax_bits[k], __float_as_uint(ad[k]));
+    }
     atomicMin(&scr->rmin_bits, __float_as_uint(rmin));
     atomicMax(&scr->rmax_bits, __float_as_uint(rmax));
   }
@@ -743,7 +801,8 @@ __global__ __launch_bounds__(64 * W) void k_pair_f64(
 // ---------------------------------------------------------------------------
 // merge: compensated FP64 sum of the chunk partials (FP32 partials of the
 // reduced kernel, FP64 of the plain kernel) in chunk order, then the
-// base phase (FP32R): out = k/(4 pi) exp(i k |p - c|) * sum, written to the
+// base phase (FP32R): out = k/(4 pi) exp(i k |p - c|) * sum (k|p - c| as a
+// double-double product), written to the
 // output index of the slot.  grid = (ceil(max npad / 256), n_list).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
@@ -779,7 +838,7 @@ __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
   for (int k = 0; k < 4; ++k) o[k] = s[k] + e[k];
   if (I.mode == MODE_F32R) {
     double c, sn;
-    cis_rev_dd(I.kr_hi, I.kr_lo, I.R64[p], &c, &sn);
+    cis_rev_dd2(I.kr_hi, I.kr_lo, I.R64[2 * p], I.R64[2 * p + 1], &c, &sn);
     const double kf = I.k / (4.0 * kPI);
     c *= kf; sn *= kf;
     const double a0 = o[0] * c - o[1] * sn, a1 = o[0] * sn + o[1] * c;

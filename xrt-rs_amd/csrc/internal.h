@@ -10,7 +10,7 @@
 //   RayRec32[nr]   per-solve FP32 ray records of the reduced kernel, 48 B
 //   PixRec32[npad] per-solve FP32 slot records, 32 B
 //   TileRec[ntile] per-solve FP64 tile reference points, 32 B
-//   double R64[npad]               FP64 |p - c| for the base phase
+//   double R64[2 * npad]           |p - c| per slot as a double-double
 //   partial[n_chunks][npad]        per-chunk partial sums (float4 from the
 //                                  FP32 kernel, double4 from the FP64 one)
 //   double4 out[np]                final {Es.re, Es.im, Ep.re, Ep.im}
@@ -66,7 +66,7 @@ struct PixRec32 {
 };
 static_assert(sizeof(PixRec32) == 32, "PixRec32 is 32 B");
 
-// FP64 reference point of one tile of 64*P slots (the mean of its q).
+// FP64 reference point of one tile of 64*P slots (its bounding-box centre).
 struct TileRec {
   double qx, qy, qz, R;
 };
@@ -107,7 +107,7 @@ struct KInst {
   const PixRec32* pix32;    // prepared FP32 slots
   const TileRec* tiles;     // tile reference points
   const uint32_t* perm;     // slot -> output index (~0u: padding); null = identity
-  const double* R64;        // FP64 |p - c| per slot
+  const double* R64;        // |p - c| per slot, double-double (hi, lo)
   void* partial;            // [n_chunks][npad] float4 (F32R) or double4 (F64)
   double* out;              // [np][4]
   InstScratch* scr;         // device-computed values
