@@ -71,6 +71,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
 // double-double so the fractional revolution keeps full FP64 accuracy.
 __device__ __forceinline__ void cis_rev_dd(double a_hi, double a_lo, double x,
                                            double* c, double* s) {
+  #pragma clang fp contract(off)
   const double hi = a_hi * x;
   const double lo = fma(a_hi, x, -hi) + a_lo * x;
   const double n = rint(hi);
@@ -80,11 +81,13 @@ __device__ __forceinline__ void cis_rev_dd(double a_hi, double a_lo, double x,
 
 // Double-double helpers (Knuth TwoSum, FMA TwoProduct).
 __device__ __forceinline__ void two_sum(double a, double b, double* s, double* e) {
+  #pragma clang fp contract(off)
   *s = a + b;
   const double bp = *s - a;
   *e = (a - (*s - bp)) + (b - bp);
 }
 __device__ __forceinline__ void two_prod(double a, double b, double* p, double* e) {
+  #pragma clang fp contract(off)
   *p = a * b;
   *e = fma(a, b, -*p);
 }
@@ -92,6 +95,7 @@ __device__ __forceinline__ void two_prod(double a, double b, double* p, double* 
 // |q| of q = (x_hi + x_lo, ...) as a double-double.
 __device__ __forceinline__ void norm_dd(const double (&h)[3], const double (&l)[3], double* rh,
                                         double* rl) {
+  #pragma clang fp contract(off)
   double sh = 0.0, sl = 0.0;
   for (int i = 0; i < 3; ++i) {
     double p, e;
@@ -114,6 +118,7 @@ __device__ __forceinline__ void norm_dd(const double (&h)[3], const double (&l)[
 // exp(i 2 pi (a_hi + a_lo)(x_hi + x_lo)) for a large phase.
 __device__ __forceinline__ void cis_rev_dd2(double a_hi, double a_lo, double x_hi, double x_lo,
                                             double* c, double* s) {
+  #pragma clang fp contract(off)
   double hi, e;
   two_prod(a_hi, x_hi, &hi, &e);
   const double lo = e + a_hi * x_lo + a_lo * x_hi;
@@ -331,6 +336,7 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
                                                   PixRec32* __restrict__ out32,
                                                   TileRec* __restrict__ tiles,
                                                   double* __restrict__ R64) {
+  #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x;
   const uint32_t base = blockIdx.x * 64u * P;
   const double c[3] = {scr->cx, scr->cy, scr->cz};
@@ -469,6 +475,7 @@ __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__
   const double oo = (double)ox * ox + (double)oy * oy + (double)oz * oz;
   double er = r.es_re, ei = r.es_im, pr = r.ep_re, pi = r.ep_im;
   if (flags & INST_F_PATHUP) {
+    #pragma clang fp contract(off)
     const double hi = kr * r.path_up;
     const double lo = fma(kr, r.path_up, -hi) + kr_lo * r.path_up;
     double s, c;
@@ -807,6 +814,7 @@ __global__ __launch_bounds__(64 * W) void k_pair_f64(
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
                                                const uint32_t* __restrict__ list) {
+  #pragma clang fp contract(off)
   const KInst& I = insts[list[blockIdx.y]];
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= I.npad) return;
