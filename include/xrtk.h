@@ -231,6 +231,16 @@ int xrtk_auto_last_backend(void);
 int xrtk_autotune(xrtk_ctx *ctx, size_t n_rays, size_t n_pixels, int mode,
                   XrtkOpts *best, double *best_ms);
 
+/* Times the candidate launch shapes (pixels per lane, waves per group,
+ * persistent or not) on the given batch, each instance cut to its first
+ * 65536 rays, and records the fastest as this context's configuration for
+ * batched solves: xrtk_solve_jobs and servers created afterwards use it
+ * (explicit XrtkOpts fields still override).  best_ms is the device time of
+ * the winner on the cut batch.  No reference counterpart: the reference
+ * builds one fixed 256-thread WGSL pipeline (kirchhoff.rs:151-218). */
+int xrtk_autotune_jobs(xrtk_ctx *ctx, const XrtkJob *jobs, size_t n_jobs, int mode,
+                       XrtkOpts *best, double *best_ms);
+
 /* ---- serving ------------------------------------------------------------ */
 
 typedef struct xrtk_server xrtk_server;

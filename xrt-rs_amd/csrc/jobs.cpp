@@ -37,14 +37,23 @@ uint64_t bits(double v) {
 
 size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 
-// The launch configuration of batched solves: fixed, so that a job's result
-// does not depend on which other jobs share the launch.
-LaunchCfg batch_cfg(uint32_t mode, const XrtkOpts* o) {
+}  // namespace
+
+namespace xrtk {
+
+// The launch configuration of batched solves: fixed per context (the
+// default, or what xrtk_autotune_jobs recorded), so that a job's result does
+// not depend on which other jobs share the launch.
+LaunchCfg batch_cfg(const xrtk_ctx* ctx, uint32_t mode, const XrtkOpts* o) {
   LaunchCfg c;
   if (mode == MODE_F32R) { c.P = 4; c.W = 4; } else { c.P = 1; c.W = 4; }
   c.ray_src = 1;
   c.persistent = 1;
   c.tile = mode == MODE_F32R ? 8192 : 0;
+  if (ctx) {
+    auto it = ctx->tune.find(std::make_tuple((int)mode, kBatchKey, kBatchKey));
+    if (it != ctx->tune.end()) c = it->second;
+  }
   if (o) {
     if (o->pixels_per_lane > 0) c.P = o->pixels_per_lane;
     if (o->waves_per_group > 0) c.W = o->waves_per_group;
@@ -54,10 +63,6 @@ LaunchCfg batch_cfg(uint32_t mode, const XrtkOpts* o) {
   }
   return c;
 }
-
-}  // namespace
-
-namespace xrtk {
 
 // Builds the instance specs of a batch: pixel slots and permutations in
 // `pixbuf` / `permbuf`, outputs in `outbuf` (device).  Each job's F32R/FP64
@@ -146,8 +151,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
     const uint32_t want = opts && opts->mode == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
-    const LaunchCfg c32 = batch_cfg(MODE_F32R, opts);
-    const LaunchCfg c64 = batch_cfg(MODE_F64, want == MODE_F64 ? opts : nullptr);
+    const LaunchCfg c32 = batch_cfg(ctx, MODE_F32R, opts);
+    const LaunchCfg c64 = batch_cfg(ctx, MODE_F64, want == MODE_F64 ? opts : nullptr);
     if (!pair_cfg_supported(MODE_F32R, c32) || !pair_cfg_supported(MODE_F64, c64))
       return XRTK_EINVAL;
     const char* ng = std::getenv("XRTK_NO_GRAPH");
@@ -415,6 +420,76 @@ extern "C" int xrtk_autotune(xrtk_ctx* ctx, size_t n_rays, size_t n_pixels, int 
       best->persistent = bc.persistent; best->tile_rays = bc.tile;
     }
     if (best_ms) *best_ms = bt * (double)n_rays / (double)nr;
+    return XRTK_OK;
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+// Batch autotuner: times each candidate launch shape on the caller's batch
+// and records the fastest as the context's batched-solve configuration
+// (xrtk_solve_jobs, and servers created afterwards).  Each instance is cut to
+// its first kTuneRays rays for the timing: the pixel slots, tiles and launch
+// shape are those of the real batch, and the per-ray cost does not depend on
+// the ray count once chunks are long.
+namespace {
+constexpr uint32_t kTuneRays = 65536;
+struct ScratchBufs {
+  DBuf pix, perm, outd, work, meta;
+  ~ScratchBufs() { pix.release(); perm.release(); outd.release(); work.release(); meta.release(); }
+};
+}  // namespace
+
+extern "C" int xrtk_autotune_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, int mode_in,
+                                  XrtkOpts* best, double* best_ms) {
+  if (!ctx) return XRTK_ENODEV;
+  if (!n_jobs || !jobs) return XRTK_EINVAL;
+  const uint32_t mode = mode_in == XRTK_MODE_F64 ? MODE_F64 : MODE_F32R;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    std::vector<LaunchCfg> cands;
+    for (int P : mode == MODE_F32R ? std::vector<int>{2, 4, 8} : std::vector<int>{1, 2})
+      for (int W : {1, 4})
+        for (int per : {0, 1}) {
+          LaunchCfg c;
+          c.P = P; c.W = W; c.ray_src = 1; c.persistent = per;
+          c.tile = mode == MODE_F32R ? 8192 : 0;
+          if (pair_cfg_supported(mode, c)) cands.push_back(c);
+        }
+    if (cands.empty()) return XRTK_EINVAL;
+    const LaunchCfg other = batch_cfg(ctx, mode == MODE_F32R ? MODE_F64 : MODE_F32R, nullptr);
+    ScratchBufs b;
+    double bt = 1e300;
+    LaunchCfg bc = cands.front();
+    for (const LaunchCfg& c : cands) {
+      const LaunchCfg& c32 = mode == MODE_F32R ? c : other;
+      const LaunchCfg& c64 = mode == MODE_F32R ? other : c;
+      std::vector<InstSpec> specs;
+      int rc = build_job_specs(ctx, mode, c32, c64, jobs, n_jobs, b.pix, b.perm, b.outd, specs);
+      if (rc != XRTK_OK) return rc;
+      for (InstSpec& s : specs) s.nr = std::min<uint32_t>(s.nr, kTuneRays);
+      double t = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        rc = enqueue_job_grids(ctx, c32, jobs, specs);
+        if (rc == XRTK_OK)
+          rc = run_instances(ctx, c32, c64, specs, b.work, b.meta, true, nullptr, 0, 0, true);
+        if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
+        if (rc != XRTK_OK) return rc;
+        float ms = 0.f;
+        if (rep && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess) t = ms;
+      }
+      if (t < bt) { bt = t; bc = c; }
+    }
+    ctx->tune[std::make_tuple((int)mode, kBatchKey, kBatchKey)] = bc;
+    save_cache(ctx);
+    if (best) {
+      xrtk_opts_default(best);
+      best->mode = (int)mode;
+      best->pixels_per_lane = bc.P; best->waves_per_group = bc.W; best->ray_src = bc.ray_src;
+      best->persistent = bc.persistent; best->tile_rays = bc.tile;
+    }
+    if (best_ms) *best_ms = bt;
     return XRTK_OK;
   } catch (...) {
     return XRTK_ENOMEM;

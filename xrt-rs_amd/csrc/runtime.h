@@ -141,6 +141,11 @@ namespace xrtk {
 LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o);
 // Default FP64 configuration paired with a given F32R configuration.
 LaunchCfg cfg64_for(const LaunchCfg& c32);
+// Tune-table key (both log2 fields) of the batched-solve configuration.
+constexpr int kBatchKey = -1;
+// Launch configuration of batched solves (xrtk_solve_jobs and the server):
+// the context's tuned batch entry, or the default; `o` overrides fields.
+LaunchCfg batch_cfg(const xrtk_ctx* ctx, uint32_t mode, const XrtkOpts* o);
 // Ray chunk length of one instance (a multiple of cfg.tile), intrinsic to the
 // instance shape and the launch configuration.
 uint32_t chunk_len_for(const xrtk_ctx* ctx, const LaunchCfg& cfg, uint32_t nr, uint32_t npad,

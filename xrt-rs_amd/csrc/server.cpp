@@ -489,8 +489,11 @@ int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** ou
     s->ctx = ctx;
     s->cfg = *cfg;
     if (s->cfg.quantum_ms <= 0) s->cfg.quantum_ms = 10.0;
-    s->c32.P = 4; s->c32.W = 4; s->c32.ray_src = 1; s->c32.persistent = 1; s->c32.tile = 8192;
-    s->c64 = cfg64_for(s->c32);
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      s->c32 = batch_cfg(ctx, MODE_F32R, nullptr);
+      s->c64 = batch_cfg(ctx, MODE_F64, nullptr);
+    }
     s->hpool.pinned = true;
     int dev = -1;
     (void)hipGetDevice(&dev);

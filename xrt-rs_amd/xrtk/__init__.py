@@ -30,7 +30,8 @@ from ._lib import (CHBAR, EAGAIN, GPU_PIXEL_DTYPE, GPU_RAY_DTYPE, MODE_F32R, MOD
 __all__ = [
     "CHBAR", "GPU_RAY_DTYPE", "GPU_PIXEL_DTYPE", "GpuContext", "kirchhoff_gpu",
     "kirchhoff_auto", "diffraction_integral", "diffraction_integral_rs", "WaveParams",
-    "prepare_wave", "beam_to_diffraction_rays", "RaySet", "Job", "solve_jobs", "Server",
+    "prepare_wave", "beam_to_diffraction_rays", "RaySet", "Job", "solve_jobs", "autotune_jobs",
+    "Server",
     "XrtkError", "MODE_F32R", "MODE_F64", "RAY_FIELDS",
 ]
 
@@ -351,6 +352,19 @@ def solve_jobs(ctx: GpuContext, jobs: Sequence[Job], mode="f32r", **tune):
     o = _lib.opts(_mode(mode), **tune)
     check(lib.xrtk_solve_jobs(ctx.handle, arr, len(jobs), C.byref(o)), "xrtk_solve_jobs")
     return [_to_complex(x) for x in outs]
+
+
+def autotune_jobs(ctx: GpuContext, jobs: Sequence[Job], mode="f32r"):
+    """Times the launch shapes on this batch (each instance cut to 65536
+    rays) and fixes the fastest as the context's batched-solve configuration.
+    Returns (chosen options, device ms of the winner on the cut batch)."""
+    outs = [np.zeros((j.screen.nx * j.screen.nz, 4)) for j in jobs]
+    arr = (_lib.XrtkJob * len(jobs))(*[j.to_c(o) for j, o in zip(jobs, outs)])
+    o = _lib.XrtkOpts()
+    ms = C.c_double()
+    check(lib.xrtk_autotune_jobs(ctx.handle, arr, len(jobs), _mode(mode), C.byref(o),
+                                 C.byref(ms)), "xrtk_autotune_jobs")
+    return {f: getattr(o, f) for f, _ in o._fields_}, ms.value
 
 
 class Server:

@@ -128,6 +128,7 @@ PROTOTYPES = {
     "xrtk_last_plan": (_i, [_p, C.POINTER(XrtkPlan)]),
     "xrtk_auto_last_backend": (_i, []),
     "xrtk_autotune": (_i, [_p, _sz, _sz, _i, C.POINTER(XrtkOpts), _dp]),
+    "xrtk_autotune_jobs": (_i, [_p, C.POINTER(XrtkJob), _sz, _i, C.POINTER(XrtkOpts), _dp]),
     "xrtk_server_create": (_i, [_p, C.POINTER(XrtkServerCfg), C.POINTER(_p)]),
     "xrtk_server_submit": (_i, [_p, C.POINTER(XrtkJob), C.POINTER(C.c_uint64)]),
     "xrtk_server_poll": (_i, [_p, C.c_uint64]),
