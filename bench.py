@@ -151,6 +151,27 @@ def leg_headline(args, ctx, xrtk, W):
                 "roofline_frac_fp64": v64 * 1e9 * F_PAIR / 1e12 / PEAK_FP64}
         log(f"f64: {v64:.2f} Gpair/s, speedup {value / v64:.2f}x, rel-L2 {err:.3e}")
 
+    # the reference's own call shape: kirchhoff_gpu on host GpuRay / GpuPixel
+    # slices (48 MiB + 4 MiB uploaded, 8 MiB of results back per call)
+    oneshot = None
+    if not args.no_oneshot:
+        r32 = xrtk.gpu_rays(nr)
+        for f in xrtk.RAY_FIELDS:
+            r32[f] = rays[f]
+        px, py, pz = xrtk.prepare_wave(wave)
+        p32 = xrtk.gpu_pixels(px, py, pz)
+        xrtk.kirchhoff_gpu(ctx, r32, p32)
+        t_os = []
+        for _ in range(2):
+            t = time.perf_counter()
+            xrtk.kirchhoff_gpu(ctx, r32, p32)
+            t_os.append(time.perf_counter() - t)
+        oneshot = {"value": pairs / min(t_os) / 1e9, "unit": "Gpair-evals/s",
+                   "ms_per_call": 1e3 * min(t_os),
+                   "what": "xrtk_kirchhoff_gpu_f32 from host buffers: upload, prep, pair, "
+                           "merge, result copy (PCIe-inclusive)"}
+        log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call")
+
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "frac_vs_unpacked_fp32": achieved / PEAK_FP32_NOPACK,
             "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None}
@@ -161,7 +182,7 @@ def leg_headline(args, ctx, xrtk, W):
         roof["traffic_frac_hbm"] = roof["traffic_gbps"] / PEAK_HBM
         roof["valu_insts_per_pair"] = t.get("valu_insts_per_pair")
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
-            "plan": plan, "roofline": roof, "fp64": fp64}
+            "plan": plan, "roofline": roof, "fp64": fp64, "oneshot": oneshot}
     return head, rays, wave, out32, rs
 
 
@@ -346,6 +367,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-oneshot", action="store_true")
     ap.add_argument("--rays", type=int, default=1048576)
     ap.add_argument("--side", type=int, default=512)
     ap.add_argument("--batch", type=int, default=64)
@@ -415,6 +437,7 @@ def main():
                    "plan": head["plan"]},
         "roofline": head["roofline"],
         "fp64": head["fp64"],
+        "oneshot_host_buffers": head["oneshot"],
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
