@@ -147,6 +147,29 @@ __device__ __forceinline__ double wave_mind(double v) {
   return v;
 }
 
+// Axis order of the FP32 records (o, n, m) and of the tile reference points:
+// an axis along which every ray has o = r - c == 0 goes last, so that the
+// LIN2 variant of the pair kernel can drop it from o.m (fma(0, m, x) == x,
+// so the result is the same).  Read after k_ray_bounds has reduced |o|.
+template <class T>
+__device__ __forceinline__ T pick3(const T (&v)[3], int a) {
+  return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]);
+}
+// Order-preserving unsigned keys of floats (for atomicMin / atomicMax).
+__device__ __forceinline__ uint32_t fkey(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ void kernel_axes(const InstScratch* scr, int (&ax)[3]) {
+  if (scr->omax_bits[2] == 0u)      { ax[0] = 0; ax[1] = 1; ax[2] = 2; }
+  else if (scr->omax_bits[1] == 0u) { ax[0] = 0; ax[1] = 2; ax[2] = 1; }
+  else if (scr->omax_bits[0] == 0u) { ax[0] = 1; ax[1] = 2; ax[2] = 0; }
+  else                              { ax[0] = 0; ax[1] = 1; ax[2] = 2; }
+}
+
 // ---------------------------------------------------------------------------
 // staging: caller layouts -> resident f64 records
 // ---------------------------------------------------------------------------
@@ -283,6 +306,7 @@ __global__ __launch_bounds__(256) void k_centroid_final(
     scr->pad0 = 0.0;
     for (int k = 0; k < 4; ++k) scr->omax_bits[k] = 0u;
     for (int k = 0; k < 3; ++k) scr->qmax_bits[k] = scr->dqmax_bits[k] = 0u;
+    for (int k = 0; k < 3; ++k) { scr->qlo_key[k] = 0xffffffffu; scr->qhi_key[k] = 0u; }
     scr->rmin_bits = 0x7f800000u;  // +inf
     scr->rmax_bits = 0u;
     scr->flags = 0u;
@@ -354,10 +378,16 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
       hi[k] = fmax(hi[k], qh[i][k]);
     }
   }
-  double t[3];
-  for (int k = 0; k < 3; ++k) t[k] = 0.5 * (wave_mind(lo[k]) + wave_maxd(hi[k]));
+  double t[3], wlo[3], whi[3];
+  for (int k = 0; k < 3; ++k) {
+    wlo[k] = wave_mind(lo[k]);
+    whi[k] = wave_maxd(hi[k]);
+    t[k] = 0.5 * (wlo[k] + whi[k]);
+  }
   const double Rt = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-  if (lane == 0) tiles[blockIdx.x] = TileRec{t[0], t[1], t[2], Rt};
+  int ax[3];
+  kernel_axes(scr, ax);
+  if (lane == 0) tiles[blockIdx.x] = TileRec{pick3(t, ax[0]), pick3(t, ax[1]), pick3(t, ax[2]), Rt};
   float aq[3] = {0.f, 0.f, 0.f}, ad[3] = {0.f, 0.f, 0.f};
   float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
 #pragma unroll
@@ -367,9 +397,9 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     double dq[3];
     for (int k = 0; k < 3; ++k) dq[k] = (qh[i][k] - t[k]) + ql[i][k];
     PixRec32 r;
-    r.mx = (float)(-2.0 * kr * dq[0]);
-    r.my = (float)(-2.0 * kr * dq[1]);
-    r.mz = (float)(-2.0 * kr * dq[2]);
+    r.mx = (float)(-2.0 * kr * pick3(dq, ax[0]));
+    r.my = (float)(-2.0 * kr * pick3(dq, ax[1]));
+    r.mz = (float)(-2.0 * kr * pick3(dq, ax[2]));
     r.R2 = (float)(Rh * Rh);
     r.R = (float)Rh;
     r.g = (float)((Rt - Rh) / Rt);
@@ -395,6 +425,8 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     for (int k = 0; k < 3; ++k) {
       atomicMax(&scr->qmax_bits[k], __float_as_uint(aq[k]));
       atomicMax(&scr->dqmax_bits[k], __float_as_uint(ad[k]));
+      atomicMin(&scr->qlo_key[k], fkey((float)wlo[k]));
+      atomicMax(&scr->qhi_key[k], fkey((float)whi[k]));
     }
     atomicMin(&scr->rmin_bits, __float_as_uint(rmin));
     atomicMax(&scr->rmax_bits, __float_as_uint(rmax));
@@ -429,7 +461,9 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //            v2 = kr (2 sum_i |o_i| |dq_i| + |o|^3 / (2 min|q| - |o|)) / (...)
 //          TILE is used unless v0 is already small; NOFRACT when the phase
 //          the pair kernel sees stays well inside v_sin's +-256 rev domain.
-// force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK.
+//          LIN2 when o is zero along one axis (dropped from o.m).
+// force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
+// 0x400 forbids LIN2.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -453,6 +487,9 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     if (force_flags & 0x100u) tile = (force_flags & INST_F_TILE) != 0u;
     if (tile) f |= INST_F_TILE;
     if ((tile ? v2 : v0) < 128.0) f |= INST_F_NOFRACT;
+    const bool zero_axis = scr->omax_bits[0] == 0u || scr->omax_bits[1] == 0u ||
+                           scr->omax_bits[2] == 0u;
+    if (zero_axis && !(force_flags & 0x400u)) f |= INST_F_LIN2;
   }
   if (force_flags & 0x200u) f |= INST_F_CHECK;
   scr->flags = f;
@@ -460,19 +497,41 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
   scr->v2_rev = v2;
 }
 
-// FP32 ray records of the phase-reduced kernel.
+// FP32 ray records of the phase-reduced kernel (layout and meaning: RayRec32).
+// Rounding o = r - c to FP32 moves the ray by delta = o_f32 - o (~1e-9 mm),
+// a phase error kr delta.d at a pixel in direction d from the ray: first order
+// along the line of sight (an aperture not aligned with an axis), and across
+// it wherever the ray-to-pixel directions fan out (near field).  The
+// tile-referenced variant takes kr delta.(q_t - o)/|q_t - o| per (ray, tile)
+// from delta kept in the record; the centroid-referenced variant, used only
+// when the whole phase is small, takes kr delta.u once per ray, u from the
+// centroid to the centre of the screen's bounding box.
 __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__ rays,
                                                     uint32_t n,
                                                     const InstScratch* __restrict__ scr,
-                                                    double kr, double kr_lo,
+                                                    double kr, double kr_lo, uint32_t have_pix,
                                                     RayRec32* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
   const uint32_t flags = scr->flags;
+  int ax[3];
+  kernel_axes(scr, ax);
+  double u[3] = {0.0, 0.0, 0.0};  // unit direction to the screen
+  if (have_pix) {
+    for (int a = 0; a < 3; ++a)
+      u[a] = 0.5 * ((double)fkey_inv(scr->qlo_key[a]) + (double)fkey_inv(scr->qhi_key[a]));
+    const double m = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+    for (int a = 0; a < 3; ++a) u[a] *= inv;
+  }
   const RayRec64 r = rays[i];
-  const float ox = (float)(r.x - cx), oy = (float)(r.y - cy), oz = (float)(r.z - cz);
-  const double oo = (double)ox * ox + (double)oy * oy + (double)oz * oz;
+  const double oe[3] = {r.x - cx, r.y - cy, r.z - cz};
+  const float o3[3] = {(float)oe[0], (float)oe[1], (float)oe[2]};
+  const double dl[3] = {(double)o3[0] - oe[0], (double)o3[1] - oe[1], (double)o3[2] - oe[2]};
+  const bool tile = (flags & INST_F_TILE) != 0u;
+  const double los = tile ? 0.0 : kr * (dl[0] * u[0] + dl[1] * u[1] + dl[2] * u[2]);
+  const double oo = (double)o3[0] * o3[0] + (double)o3[1] * o3[1] + (double)o3[2] * o3[2];
   double er = r.es_re, ei = r.es_im, pr = r.ep_re, pi = r.ep_im;
   if (flags & INST_F_PATHUP) {
     #pragma clang fp contract(off)
@@ -484,18 +543,38 @@ __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__
     const double pr2 = pr * c - pi * s, pi2 = pr * s + pi * c;
     er = er2; ei = ei2; pr = pr2; pi = pi2;
   }
+  // polarisation form: Es = am e^{i th} cs, Ep = am e^{i th} (qr + i qi)
+  const double es2 = er * er + ei * ei, ep2 = pr * pr + pi * pi;
+  const double am = sqrt(es2 + ep2);
+  double th = 0.0, cs = 0.0, qr = 0.0, qi = 0.0;
+  if (am > 0.0) {
+    th = es2 > 0.0 ? atan2(ei, er) : atan2(pi, pr);
+    double s, c;
+    sincos(th, &s, &c);
+    cs = sqrt(es2) / am;
+    qr = (pr * c + pi * s) / am;
+    qi = (pi * c - pr * s) / am;
+  }
+  double off = th * (0.5 / kPI) + 0.25 + los;  // + 1/4 rev: the factor i
+  off -= rint(off);
+  const double n3[3] = {r.nx, r.ny, r.nz};
   RayRec32 o;
-  o.ox = ox; o.oy = oy; o.oz = oz;
-  o.oo = (float)(kr * oo);
-  // n' = -n / (2 kr): n . d = n' . m + n . (q_t - o) with m = -2 kr dq
-  const double nf = -0.5 / kr;
-  o.nx = (float)(r.nx * nf); o.ny = (float)(r.ny * nf); o.nz = (float)(r.nz * nf);
-  o.nl = (float)r.nl;
-  // i * Es, i * Ep
-  o.ar = (float)(-ei); o.ai = (float)er;
-  o.br = (float)(-pi); o.bi = (float)pr;
-  o.no = (float)(r.nx * (double)ox + r.ny * (double)oy + r.nz * (double)oz);
-  o.pad0 = o.pad1 = o.pad2 = 0.f;
+  o.ox = pick3(o3, ax[0]); o.oy = pick3(o3, ax[1]); o.oz = pick3(o3, ax[2]);
+  o.oo = tile ? (float)pick3(dl, ax[2]) : (float)(kr * oo);
+  o.dx = (float)pick3(dl, ax[0]);
+  o.dy = (float)pick3(dl, ax[1]);
+  // n = -am n_ray / (2 kr): n_ray . d = n . m + n_ray . (q_t - o) with m = -2 kr dq
+  const double nf = -0.5 * am / kr;
+  o.nx = (float)(pick3(n3, ax[0]) * nf);
+  o.ny = (float)(pick3(n3, ax[1]) * nf);
+  o.nz = (float)(pick3(n3, ax[2]) * nf);
+  o.nl = (float)(am * r.nl);
+  o.cs = (float)cs;
+  o.pr = (float)qr;
+  o.pi = (float)qi;
+  o.no = (float)(am * (r.nx * (double)o3[0] + r.ny * (double)o3[1] + r.nz * (double)o3[2]));
+  o.am = (float)am;
+  o.off = (float)off;
   out[i] = o;
 }
 
@@ -504,13 +583,12 @@ __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__
 // ---------------------------------------------------------------------------
 // Per-(ray, tile) values staged in LDS by the TILE variant.
 struct TileRay {
-  float B;       // kr (|o|^2 - 2 o.q_t)
-  float ar, ai;  // i Es rotated by exp(i 2 pi kr (|o|^2 - 2 o.q_t)/(2 R_t))
-  float br, bi;  // i Ep rotated likewise
-  float nqt;     // n . (q_t - o)
-  float pad0, pad1;
+  float B;    // kr (|o|^2 - 2 o.q_t)
+  float off;  // fractional revolution of kr (|o|^2 - 2 o.q_t) / (2 R_t), plus the ray's offset
+  float n;    // UNIFORM_N: -am n.o; else am n.(q_t - o)
+  float pad;
 };
-static_assert(sizeof(TileRay) == 32, "TileRay is 32 B");
+static_assert(sizeof(TileRay) == 16, "TileRay is 16 B");
 
 static constexpr uint32_t kBlockRays = 64;  // rays staged per LDS block (one per lane)
 
@@ -519,24 +597,29 @@ struct SlotRegs {
   float mx[P], my[P], mz[P], R2[P], R[P], g[P], nq[P];
 };
 
-// Accumulates one ray into a[P][4] for the P slots of this lane.  TILE: B and
-// the rotated amplitudes come from the per-(ray, tile) LDS record.
-template <int P, bool TILE, bool UNIN, bool NOFRACT>
-__device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float tar, float tai,
-                                         float tbr, float tbi, float nqt, const SlotRegs<P>& s,
-                                         float ikr, float h, float (&a)[P][4]) {
-  float ar = r.ar, ai = r.ai, br = r.br, bi = r.bi;
+// Accumulates one ray into a[P][4] for the P slots of this lane.
+//   TILE:  B, the phase offset and the normal term come from the
+//          per-(ray, tile) LDS record; otherwise from the ray record.
+//   UNIN:  every ray has the same normal, n.d = am (n.q - n.o).
+//   LIN2:  o is zero along the last kernel axis.
+// Per pair (TILE, UNIN, NOFRACT, LIN2): 20 FP32 add/mul/FMA + rsq, rcp, sin,
+// cos.
+template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2>
+__device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, float nt,
+                                         const SlotRegs<P>& s, float ikr, float h,
+                                         float (&a)[P][4]) {
   float oo = r.oo, mno = -r.no;
-  if constexpr (TILE) {
-    ar = tar; ai = tai; br = tbr; bi = tbi;
-  } else {
+  if constexpr (!TILE) {
     oo = vcopy(oo);
-    if constexpr (!UNIN) mno = vcopy(mno);
+    off = r.off;
+    mno = vcopy(mno);
   }
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     // kr (|o|^2 - 2 o.q) split as base + o.m
-    const float lin = fmaf(r.oz, s.mz[i], fmaf(r.oy, s.my[i], r.ox * s.mx[i]));
+    float lin = r.ox * s.mx[i];
+    lin = fmaf(r.oy, s.my[i], lin);
+    if constexpr (!LIN2) lin = fmaf(r.oz, s.mz[i], lin);
     const float num = lin + (TILE ? B : oo);
     const float p2 = fmaf(num, ikr, s.R2[i]);                         // |p - r|^2
     const float inv = __builtin_amdgcn_rsqf(p2);                       // 1 / path
@@ -545,34 +628,32 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float tar, 
     if constexpr (TILE) {
       const float eps = num * rden;            // kr (path - R)
       const float u = fmaf(-eps, h, s.g[i]);   // (S_t - S) / S_t
-      rev = fmaf(B, u, lin) * rden;
+      rev = fmaf(fmaf(B, u, lin), rden, off);
     } else {
-      rev = num * rden;
+      rev = fmaf(num, rden, off);
     }
-    float nd;
+    float nd;  // am n.d
     if constexpr (UNIN) {
-      nd = s.nq[i] + mno;  // n.q - n.o
+      nd = fmaf(s.nq[i], r.am, TILE ? nt : mno);
     } else {
-      nd = fmaf(r.nz, s.mz[i], fmaf(r.ny, s.my[i], fmaf(r.nx, s.mx[i], TILE ? nqt : mno)));
+      nd = fmaf(r.nz, s.mz[i], fmaf(r.ny, s.my[i], fmaf(r.nx, s.mx[i], TILE ? nt : mno)));
     }
-    const float obl = fmaf(nd, inv, r.nl);     // nl + n.d / path
+    const float obl = fmaf(nd, inv, r.nl);     // am (nl + n.d / path)
     const float f = NOFRACT ? rev : __builtin_amdgcn_fractf(rev);
     const float sn = __builtin_amdgcn_sinf(f);
     const float cs = __builtin_amdgcn_cosf(f);
     const float amp = obl * inv;
     const float wr = amp * cs, wi = amp * sn;
-    a[i][0] = fmaf(ar, wr, a[i][0]);
-    a[i][0] = fmaf(-ai, wi, a[i][0]);
-    a[i][1] = fmaf(ar, wi, a[i][1]);
-    a[i][1] = fmaf(ai, wr, a[i][1]);
-    a[i][2] = fmaf(br, wr, a[i][2]);
-    a[i][2] = fmaf(-bi, wi, a[i][2]);
-    a[i][3] = fmaf(br, wi, a[i][3]);
-    a[i][3] = fmaf(bi, wr, a[i][3]);
+    a[i][0] = fmaf(r.cs, wr, a[i][0]);
+    a[i][1] = fmaf(r.cs, wi, a[i][1]);
+    a[i][2] = fmaf(r.pr, wr, a[i][2]);
+    a[i][2] = fmaf(-r.pi, wi, a[i][2]);
+    a[i][3] = fmaf(r.pr, wi, a[i][3]);
+    a[i][3] = fmaf(r.pi, wr, a[i][3]);
   }
 }
 
-template <int P, int W, bool TILE, bool UNIN, bool NOFRACT>
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   const uint32_t npad = I.npad;
@@ -609,8 +690,8 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     for (uint32_t b0 = r0; b0 < r1; b0 += kBlockRays) {
       const uint32_t nb = min(kBlockRays, r1 - b0);
       // lane-parallel per-(ray, tile) values, one ray per lane: the phase
-      // A = kr (|o|^2 - 2 o.q_t) / (2 |q_t|) is reduced in FP64, its
-      // fractional revolution rotates the amplitudes
+      // A = kr (|o|^2 - 2 o.q_t) / (2 |q_t|) is reduced in FP64 and its
+      // fractional revolution joins the ray's phase offset
       __syncthreads();
       if (lane < nb) {
         const RayRec32 rr = I.rays32[b0 + lane];
@@ -618,18 +699,18 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
         const double Bt = (ox * ox + oy * oy + oz * oz) -
                           2.0 * (ox * T.qx + oy * T.qy + oz * T.qz);
         const double A = kr * Bt / (2.0 * T.R);  // revolutions
-        const float fa = (float)(A - rint(A));
-        const float sn = __builtin_amdgcn_sinf(fa), cs = __builtin_amdgcn_cosf(fa);
+        // phase of the FP32 rounding delta of o towards this tile
+        const double dx = rr.dx, dy = rr.dy, dz = rr.oo;
+        const double corr = kr * (dx * (T.qx - ox) + dy * (T.qy - oy) + dz * (T.qz - oz)) /
+                            sqrt(T.R * T.R + Bt);
         TileRay tr;
         tr.B = (float)(kr * Bt);
-        tr.ar = fmaf(rr.ar, cs, -rr.ai * sn);
-        tr.ai = fmaf(rr.ar, sn, rr.ai * cs);
-        tr.br = fmaf(rr.br, cs, -rr.bi * sn);
-        tr.bi = fmaf(rr.br, sn, rr.bi * cs);
-        // n.(q_t - o), with n = -2 kr n' (the record stores n' = -n/(2 kr))
-        tr.nqt = (float)(-2.0 * kr * ((double)rr.nx * T.qx + (double)rr.ny * T.qy +
-                                      (double)rr.nz * T.qz) - (double)rr.no);
-        tr.pad0 = tr.pad1 = 0.f;
+        tr.off = (float)((A - rint(A)) + ((double)rr.off + corr));
+        // am n.(q_t - o), with am n = -2 kr n' (the record stores n' = -am n/(2 kr))
+        tr.n = UNIN ? -rr.no
+                    : (float)(-2.0 * kr * ((double)rr.nx * T.qx + (double)rr.ny * T.qy +
+                                           (double)rr.nz * T.qz) - (double)rr.no);
+        tr.pad = 0.f;
         my[lane] = tr;
       }
       __syncthreads();
@@ -637,15 +718,14 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
       for (uint32_t jj = 0; jj < nb; ++jj) {
         const RayRec32 r = rays[b0 + jj];
         const TileRay tr = my[jj];
-        pair_ray<P, true, UNIN, NOFRACT>(r, tr.B, tr.ar, tr.ai, tr.br, tr.bi, tr.nqt, s, ikr, h,
-                                         a);
+        pair_ray<P, true, UNIN, NOFRACT, LIN2>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
       }
     }
   } else {
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
-      pair_ray<P, false, UNIN, NOFRACT>(r, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, s, ikr, h, a);
+      pair_ray<P, false, UNIN, NOFRACT, LIN2>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
@@ -653,6 +733,24 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
     part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
+  }
+}
+
+// Runs one work item with the variant the instance's flags select.
+template <int P, int W>
+__device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const WorkItem& w,
+                                              TileRay* lds, uint32_t lane, uint32_t wave) {
+  const uint32_t v = ((f & INST_F_TILE) ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
+                     ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+  switch (v) {
+#define XRTK_V(V_)                                                                   \
+  case V_:                                                                          \
+    f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0>(    \
+        I, w, lds, lane, wave);                                                     \
+    break;
+    XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
+    XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
+#undef XRTK_V
   }
 }
 
@@ -678,24 +776,7 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
     const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
-    const bool tile = f & INST_F_TILE, uni = f & INST_F_UNIFORM_N, nofr = f & INST_F_NOFRACT;
-    if (tile) {
-      if (uni) {
-        if (nofr) f32r_item<P, W, true, true, true>(I, w, lds, lane, wave);
-        else      f32r_item<P, W, true, true, false>(I, w, lds, lane, wave);
-      } else {
-        if (nofr) f32r_item<P, W, true, false, true>(I, w, lds, lane, wave);
-        else      f32r_item<P, W, true, false, false>(I, w, lds, lane, wave);
-      }
-    } else {
-      if (uni) {
-        if (nofr) f32r_item<P, W, false, true, true>(I, w, lds, lane, wave);
-        else      f32r_item<P, W, false, true, false>(I, w, lds, lane, wave);
-      } else {
-        if (nofr) f32r_item<P, W, false, false, true>(I, w, lds, lane, wave);
-        else      f32r_item<P, W, false, false, false>(I, w, lds, lane, wave);
-      }
-    }
+    f32r_dispatch<P, W>(f, I, w, lds, lane, wave);
     if constexpr (!PERSIST) break;
   }
 }
@@ -944,7 +1025,7 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
                      static_flags);
   if (mode == MODE_F32R && nr)
     hipLaunchKernelGGL(k_ray_prep32, dim3(cdiv(nr, 256)), dim3(256), 0, st, rays, nr, scr, kr,
-                       kr_lo, rays32);
+                       kr_lo, npad ? 1u : 0u, rays32);
   return hipGetLastError();
 }
 

@@ -7,7 +7,7 @@
 //   PixF64[npad]   f64 screen points in slot order (tiles of 64*P slots; a
 //                  grid screen is laid out tile by tile, padding slots repeat a
 //                  real pixel), 32 B each, with perm[slot] -> output index
-//   RayRec32[nr]   per-solve FP32 ray records of the reduced kernel, 48 B
+//   RayRec32[nr]   per-solve FP32 ray records of the reduced kernel, 64 B
 //   PixRec32[npad] per-solve FP32 slot records, 32 B
 //   TileRec[ntile] per-solve FP64 tile reference points, 32 B
 //   double R64[2 * npad]           |p - c| per slot as a double-double
@@ -41,22 +41,35 @@ struct PixF64 {
 // FP32 record of one ray for the phase-reduced pair kernel.
 //   o  = r - c, formed in FP64 and rounded once; every later use of the ray
 //        position (phase, path, fold) uses this rounded o, so the geometry is
-//        exactly that of a ray at c + o.
-//   oo = kr |o|^2 (kr = k / 2 pi: phases are in revolutions)
-//   no = n . o
-//   (ar, ai) = i Es', (br, bi) = i Ep', Es'/Ep' carrying the upstream phase.
+//        exactly that of a ray at c + o.  Stored in the kernel axis order
+//        (kernel_axes): an axis along which every o is zero comes last.
+//   oo = kr |o|^2 (kr = k / 2 pi: phases are in revolutions), read by the
+//        centroid-referenced variant; the tile-referenced variant keeps dz
+//        there instead.
+//   (dx, dy, dz) = o_f32 - o in FP64 rounded to FP32: the rounding of the
+//        position, whose phase kr delta.(q_t - o)/|q_t - o| the
+//        tile-referenced variant adds per (ray, tile).
+//   Amplitudes in polarisation form, after the upstream path phase:
+//        Es = am e^{i th} cs,  Ep = am e^{i th} (pr + i pi),
+//        am = sqrt(|Es|^2 + |Ep|^2), cs = |Es| / am >= 0.
+//   am scales the geometric factors: n = -am n_ray / (2 kr) (kernel axis
+//   order), nl = am nl_ray, no = am n_ray . o.
+//   off = th / 2 pi + 1/4 (the quarter turn is the factor i of the
+//        integrand), reduced to [-1/2, 1/2] revolutions; the
+//        centroid-referenced variant also adds kr delta.u here (u: direction
+//        to the screen centre).
 // The factor k/(4 pi) is applied once per pixel in the FP64 merge.
 struct RayRec32 {
   float ox, oy, oz, oo;
   float nx, ny, nz, nl;
-  float ar, ai, br, bi;
-  float no, pad0, pad1, pad2;
+  float cs, pr, pi, no;
+  float am, off, dx, dy;
 };
 static_assert(sizeof(RayRec32) == 64, "RayRec32 is 64 B");
 
 // FP32 record of one pixel slot.  With the slot's tile reference point t
 // (TileRec) and q = p - c, dq = q - q_t:
-//   m  = -2 kr dq   (FP64, rounded once)
+//   m  = -2 kr dq   (FP64, rounded once; kernel axis order)
 //   R2 = |q|^2, R = |q|
 //   g  = (R_t - R) / R_t
 //   nq = n . q   (only when every ray has the same normal n)
@@ -66,7 +79,8 @@ struct PixRec32 {
 };
 static_assert(sizeof(PixRec32) == 32, "PixRec32 is 32 B");
 
-// FP64 reference point of one tile of 64*P slots (its bounding-box centre).
+// FP64 reference point of one tile of 64*P slots (its bounding-box centre),
+// in the kernel axis order.
 struct TileRec {
   double qx, qy, qz, R;
 };
@@ -86,14 +100,17 @@ struct InstScratch {
   uint32_t pad1;
   double n0x, n0y, n0z;    // normal of ray 0
   double v0_rev, v2_rev;   // bounds of the reduced phase of the two variants
+  uint32_t qlo_key[3];     // min q_x, q_y, q_z (order-preserving float keys)
+  uint32_t qhi_key[3];     // max q_x, q_y, q_z
 };
 
 enum : uint32_t {
   INST_F_CHECK = 1u,      // F64 kernel: some pair may fall below the skip threshold
-  INST_F_TILE = 2u,       // F32R: per-(ray, tile) phase folded into the amplitudes
+  INST_F_TILE = 2u,       // F32R: per-(ray, tile) phase folded into the phase offset
   INST_F_PATHUP = 4u,     // rays carry an upstream path phase
   INST_F_UNIFORM_N = 8u,  // every ray has the same normal
   INST_F_NOFRACT = 16u,   // |reduced phase| < 128 rev: v_sin/v_cos take it as is
+  INST_F_LIN2 = 32u,      // F32R: o is zero along one axis (the last kernel axis)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };
