@@ -217,7 +217,18 @@ typedef struct XrtkPlan {
     int32_t split, ray_src, persistent, n_items;
     int32_t n_fp64;     /* instances the bound routed to the FP64 kernel    */
     int32_t backend;    /* 1 = device, 0 = CPU fallback                     */
+    int32_t inst_flags; /* one-shot solves: variant flags the device planner
+                           chose for the first instance (XRTK_VAR_*)        */
+    int32_t reserved;
 } XrtkPlan;
+/* Variant flags reported in XrtkPlan.inst_flags. */
+#define XRTK_VAR_CHECK     1   /* FP64: a pair may fall below the skip threshold */
+#define XRTK_VAR_TILE      2   /* tile-referenced reduced phase                  */
+#define XRTK_VAR_PATHUP    4   /* upstream path phase                            */
+#define XRTK_VAR_UNIFORM_N 8   /* every ray has the same normal                  */
+#define XRTK_VAR_NOFRACT   16  /* reduced phase inside v_sin's direct domain     */
+#define XRTK_VAR_LIN2      32  /* o is zero along one axis                       */
+#define XRTK_VAR_PAX       64  /* paraxial 1/path and 1/(path + R)               */
 int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);
 /* Backend used by the last xrtk_kirchhoff_auto_f32 call of this process:
  * 1 = device, 0 = CPU fallback, -1 = none yet. */

@@ -327,6 +327,40 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
     assert np.array_equal(lin2, full)
 
 
+VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX = 2, 16, 32, 64
+
+
+@pytest.mark.parametrize("distance", [100.0, 1000.0, 30000.0])
+def test_paraxial_expansion_matches_exact_form(ctx, monkeypatch, distance):
+    """The tile variant's paraxial 1/path and 1/(path + R) (one FMA each)
+    against rsq/rcp of the exact path; both against the long-double oracle."""
+    rays, scr, _ = W.stress_cell(20000.0, distance, n_rays=32768, side=96)
+    g = grid(scr)
+    pax = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    flags = ctx.last_plan()["inst_flags"]
+    monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x800")
+    exact = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    assert not ctx.last_plan()["inst_flags"] & VAR_PAX
+    monkeypatch.delenv("XRTK_FORCE_FLAGS")
+    px, py, pz = xrtk.prepare_wave(g)
+    stride = 37
+    ld = O.diffraction_ld(rays, px, py, pz, stride=stride)
+    e_pax, e_ex = rel_l2(pax[::stride], ld), rel_l2(exact[::stride], ld)
+    print(f"L={distance}: flags {flags:#x}, paraxial {e_pax:.2e}, exact form {e_ex:.2e}, "
+          f"between {rel_l2(pax, exact):.2e}")
+    assert flags & VAR_TILE and flags & VAR_PAX and flags & VAR_LIN2
+    assert e_pax < STRESS_BAR and e_ex < STRESS_BAR
+    assert e_pax < 1.5 * e_ex + 1e-6
+
+
+def test_gaussian_case_variant(ctx):
+    """The headline case runs the tile variant with every specialisation."""
+    rays = W.gaussian_source(65536, seed=2)
+    xrtk.diffraction_integral(rays, grid(W.screen(1000.0, 0.02, 128)), ctx=ctx)
+    f = ctx.last_plan()["inst_flags"]
+    assert f & VAR_TILE and f & VAR_NOFRACT and f & VAR_LIN2 and f & VAR_PAX, hex(f)
+
+
 def _transform(rays, pix, M):
     """Applies the orthogonal matrix M to positions, normals and pixels."""
     out = dict(rays)

@@ -143,6 +143,12 @@ void record_timing(xrtk_ctx* ctx) {
   ctx->last_solve_ms = b;
 }
 
+static_assert(XRTK_VAR_CHECK == INST_F_CHECK && XRTK_VAR_TILE == INST_F_TILE &&
+                  XRTK_VAR_PATHUP == INST_F_PATHUP && XRTK_VAR_UNIFORM_N == INST_F_UNIFORM_N &&
+                  XRTK_VAR_NOFRACT == INST_F_NOFRACT && XRTK_VAR_LIN2 == INST_F_LIN2 &&
+                  XRTK_VAR_PAX == INST_F_PAX,
+              "variant flags of the ABI match the device planner's");
+
 void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
   XrtkPlan& q = ctx->last_plan;
   const bool all64 = p.n_inst && p.n_f64 == p.n_inst;
@@ -157,6 +163,8 @@ void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
   q.n_items = (int32_t)(p.n_items32 + p.n_items64);
   q.n_fp64 = (int32_t)p.n_f64;
   q.backend = backend;
+  q.inst_flags = 0;
+  q.reserved = 0;
 }
 
 // Pixel source of a one-shot solve.
@@ -370,6 +378,11 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
   if (e != hipSuccess) return map_err(e);
   record_timing(ctx);
   record_plan(ctx, plan, 1);
+  if (plan.scr0) {
+    uint32_t fl = 0;
+    if (hipMemcpy(&fl, &plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) == hipSuccess)
+      ctx->last_plan.inst_flags = (int32_t)fl;
+  }
   return XRTK_OK;
 }
 

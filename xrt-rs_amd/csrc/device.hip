@@ -404,6 +404,11 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     r.R = (float)Rh;
     r.g = (float)((Rt - Rh) / Rt);
     r.nq = uni ? (float)(n0x * qh[i][0] + n0y * qh[i][1] + n0z * qh[i][2]) : 0.f;
+    const double ri = 1.0 / Rh, ri3 = ri * ri * ri / kr;
+    r.r0 = (float)(0.5 * ri);
+    r.c4 = (float)(0.125 * ri3);
+    r.rinv = (float)ri;
+    r.c2 = (float)(0.5 * ri3);
     r.pad = 0.f;
     const uint32_t s = base + i * 64u + lane;
     out32[s] = r;
@@ -462,8 +467,10 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //          TILE is used unless v0 is already small; NOFRACT when the phase
 //          the pair kernel sees stays well inside v_sin's +-256 rev domain.
 //          LIN2 when o is zero along one axis (dropped from o.m).
+//          PAX (tile variant) when the paraxial expansions of 1/path and
+//          1/(path + R) are exact to FP32.
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
-// 0x400 forbids LIN2.
+// 0x400 forbids LIN2, 0x800 forbids PAX.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -490,6 +497,13 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     const bool zero_axis = scr->omax_bits[0] == 0u || scr->omax_bits[1] == 0u ||
                            scr->omax_bits[2] == 0u;
     if (zero_axis && !(force_flags & 0x400u)) f |= INST_F_LIN2;
+    // rho = |path^2 - R^2| / R^2 <= (|o|^2 + 2 sum_i |o_i| |q_i|) / min R^2;
+    // the expansions' first-order errors are 3 rho^2 / 8 (amplitude) and
+    // rho^2 / 8 (relative, phase)
+    if (tile && rmin > 0.0) {
+      const double rho = (om * om + 2.0 * (ox * qx + oy * qy + oz * qz)) / (rmin * rmin);
+      if (rho <= 1e-4 && !(force_flags & 0x800u)) f |= INST_F_PAX;
+    }
   }
   if (force_flags & 0x200u) f |= INST_F_CHECK;
   scr->flags = f;
@@ -595,6 +609,7 @@ static constexpr uint32_t kBlockRays = 64;  // rays staged per LDS block (one pe
 template <int P>
 struct SlotRegs {
   float mx[P], my[P], mz[P], R2[P], R[P], g[P], nq[P];
+  float r0[P], c4[P], rinv[P], c2[P];
 };
 
 // Accumulates one ray into a[P][4] for the P slots of this lane.
@@ -602,9 +617,11 @@ struct SlotRegs {
 //          per-(ray, tile) LDS record; otherwise from the ray record.
 //   UNIN:  every ray has the same normal, n.d = am (n.q - n.o).
 //   LIN2:  o is zero along the last kernel axis.
-// Per pair (TILE, UNIN, NOFRACT, LIN2): 20 FP32 add/mul/FMA + rsq, rcp, sin,
-// cos.
-template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2>
+//   PAX:   (tile variant) 1/path and 1/(path + R) from the paraxial
+//          expansion: one FMA each instead of rsq and rcp.
+// Per pair (TILE, UNIN, NOFRACT, LIN2, PAX): 21 FP32 add/mul/FMA + sin, cos;
+// without PAX: 20 + rsq, rcp, sin, cos.
+template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX>
 __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, float nt,
                                          const SlotRegs<P>& s, float ikr, float h,
                                          float (&a)[P][4]) {
@@ -621,9 +638,15 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
     lin = fmaf(r.oy, s.my[i], lin);
     if constexpr (!LIN2) lin = fmaf(r.oz, s.mz[i], lin);
     const float num = lin + (TILE ? B : oo);
-    const float p2 = fmaf(num, ikr, s.R2[i]);                         // |p - r|^2
-    const float inv = __builtin_amdgcn_rsqf(p2);                       // 1 / path
-    const float rden = __builtin_amdgcn_rcpf(fmaf(p2, inv, s.R[i]));  // 1 / (path + R)
+    float inv, rden;  // 1 / path, 1 / (path + R)
+    if constexpr (PAX) {
+      inv = fmaf(-num, s.c2[i], s.rinv[i]);
+      rden = fmaf(-num, s.c4[i], s.r0[i]);
+    } else {
+      const float p2 = fmaf(num, ikr, s.R2[i]);  // |p - r|^2
+      inv = __builtin_amdgcn_rsqf(p2);
+      rden = __builtin_amdgcn_rcpf(fmaf(p2, inv, s.R[i]));
+    }
     float rev;
     if constexpr (TILE) {
       const float eps = num * rden;            // kr (path - R)
@@ -653,7 +676,7 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
   }
 }
 
-template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2>
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   const uint32_t npad = I.npad;
@@ -675,6 +698,7 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     s.my[i] = TILE ? pr.my : pr.my + mty;
     s.mz[i] = TILE ? pr.mz : pr.mz + mtz;
     s.R2[i] = pr.R2; s.R[i] = pr.R; s.g[i] = pr.g; s.nq[i] = pr.nq;
+    s.r0[i] = pr.r0; s.c4[i] = pr.c4; s.rinv[i] = pr.rinv; s.c2[i] = pr.c2;
   }
   // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
   // partials are summed in FP64 by k_merge.
@@ -718,14 +742,14 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
       for (uint32_t jj = 0; jj < nb; ++jj) {
         const RayRec32 r = rays[b0 + jj];
         const TileRay tr = my[jj];
-        pair_ray<P, true, UNIN, NOFRACT, LIN2>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+        pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
       }
     }
   } else {
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
-      pair_ray<P, false, UNIN, NOFRACT, LIN2>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
+      pair_ray<P, false, UNIN, NOFRACT, LIN2, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
@@ -736,20 +760,24 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
   }
 }
 
-// Runs one work item with the variant the instance's flags select.
+// Runs one work item with the variant the instance's flags select (PAX only
+// exists in the tile-referenced form).
 template <int P, int W>
 __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const WorkItem& w,
                                               TileRay* lds, uint32_t lane, uint32_t wave) {
-  const uint32_t v = ((f & INST_F_TILE) ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
-                     ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+  const bool tile = (f & INST_F_TILE) != 0u;
+  const uint32_t v = (tile ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
+                     ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u) |
+                     ((tile && (f & INST_F_PAX)) ? 16u : 0u);
   switch (v) {
-#define XRTK_V(V_)                                                                   \
-  case V_:                                                                          \
-    f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0>(    \
-        I, w, lds, lane, wave);                                                     \
+#define XRTK_V(V_)                                                                      \
+  case V_:                                                                             \
+    f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0,         \
+              (V_ & 16) != 0>(I, w, lds, lane, wave);                                  \
     break;
     XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
     XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
+    XRTK_V(24) XRTK_V(25) XRTK_V(26) XRTK_V(27) XRTK_V(28) XRTK_V(29) XRTK_V(30) XRTK_V(31)
 #undef XRTK_V
   }
 }

@@ -8,7 +8,7 @@
 //                  grid screen is laid out tile by tile, padding slots repeat a
 //                  real pixel), 32 B each, with perm[slot] -> output index
 //   RayRec32[nr]   per-solve FP32 ray records of the reduced kernel, 64 B
-//   PixRec32[npad] per-solve FP32 slot records, 32 B
+//   PixRec32[npad] per-solve FP32 slot records, 48 B
 //   TileRec[ntile] per-solve FP64 tile reference points, 32 B
 //   double R64[2 * npad]           |p - c| per slot as a double-double
 //   partial[n_chunks][npad]        per-chunk partial sums (float4 from the
@@ -73,11 +73,16 @@ static_assert(sizeof(RayRec32) == 64, "RayRec32 is 64 B");
 //   R2 = |q|^2, R = |q|
 //   g  = (R_t - R) / R_t
 //   nq = n . q   (only when every ray has the same normal n)
+//   paraxial expansion constants (rho = (path^2 - R^2) / R^2 small):
+//     1 / path       = rinv - num c2,  rinv = 1 / R,     c2 = 1 / (2 kr R^3)
+//     1 / (path + R) = r0 - num c4,    r0 = 1 / (2 R),   c4 = 1 / (8 kr R^3)
+//   with num = kr (path^2 - R^2).
 struct PixRec32 {
   float mx, my, mz, R2;
-  float R, g, nq, pad;
+  float R, g, nq, r0;
+  float c4, rinv, c2, pad;
 };
-static_assert(sizeof(PixRec32) == 32, "PixRec32 is 32 B");
+static_assert(sizeof(PixRec32) == 48, "PixRec32 is 48 B");
 
 // FP64 reference point of one tile of 64*P slots (its bounding-box centre),
 // in the kernel axis order.
@@ -111,6 +116,8 @@ enum : uint32_t {
   INST_F_UNIFORM_N = 8u,  // every ray has the same normal
   INST_F_NOFRACT = 16u,   // |reduced phase| < 128 rev: v_sin/v_cos take it as is
   INST_F_LIN2 = 32u,      // F32R: o is zero along one axis (the last kernel axis)
+  INST_F_PAX = 64u,       // F32R tile variant: 1/path and 1/(path + R) from the
+                          // first-order paraxial expansion (rho <= 1e-4)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };

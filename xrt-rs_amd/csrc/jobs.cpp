@@ -252,6 +252,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       q.ray_src = c.ray_src; q.persistent = c.persistent;
       q.n_items = (int32_t)(p.n_items32 + p.n_items64); q.n_fp64 = (int32_t)p.n_f64;
       q.backend = 1;
+      q.inst_flags = 0;
+      q.reserved = 0;
     }
     // results -> host
     for (size_t i = 0; i < n_jobs; ++i) {

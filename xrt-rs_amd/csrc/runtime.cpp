@@ -375,6 +375,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     plan_out->n_items64 = (uint32_t)items64.size();
     plan_out->n_f64 = n_f64;
     plan_out->n_inst = (uint32_t)n;
+    plan_out->scr0 = n ? reinterpret_cast<const InstScratch*>(W + lay[0].scr) : nullptr;
   }
   return XRTK_OK;
 }

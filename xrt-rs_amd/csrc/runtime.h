@@ -102,6 +102,7 @@ struct Plan {
   uint32_t n_items32 = 0, n_items64 = 0;
   uint32_t n_f64 = 0;     // instances on the FP64 kernel
   uint32_t n_inst = 0;
+  const InstScratch* scr0 = nullptr;  // device scratch of the first instance
 };
 
 }  // namespace xrtk
