@@ -39,8 +39,7 @@ for p in (os.path.join(ROOT, "xrt-rs_amd"), os.path.join(ROOT, "oracle")):
 import numpy as np  # noqa: E402
 
 F_PAIR = 51                 # FP operations per pair (FMA = 2, transcendental = 1)
-PEAK_FP32 = 157.3           # TFLOP/s, MI355X FP32 vector (packed) datasheet
-PEAK_FP32_NOPACK = 78.65    # the same without packed FP32 (what scalar FP32 code can issue)
+PEAK_FP32 = 157.3           # TFLOP/s, MI355X FP32 vector datasheet
 PEAK_FP64 = 78.6            # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0           # GB/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
@@ -173,7 +172,7 @@ def leg_headline(args, ctx, xrtk, W):
         log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call")
 
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32, "frac_vs_unpacked_fp32": achieved / PEAK_FP32_NOPACK,
+            "frac": achieved / PEAK_FP32,
             "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None}
     t = traffic_record(nr, npx, plan)
     if t is not None:
@@ -181,6 +180,7 @@ def leg_headline(args, ctx, xrtk, W):
         roof["traffic_gbps"] = t["hbm_bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9
         roof["traffic_frac_hbm"] = roof["traffic_gbps"] / PEAK_HBM
         roof["valu_insts_per_pair"] = t.get("valu_insts_per_pair")
+        roof["trans_insts_per_pair"] = t.get("trans_insts_per_pair")
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
             "plan": plan, "roofline": roof, "fp64": fp64, "oneshot": oneshot}
     return head, rays, wave, out32, rs
