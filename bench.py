@@ -296,23 +296,34 @@ def energy_scan_stream(rng, n_sets, base_energy, n_req, concurrent=16):
 
 def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=None, load=None,
                 capacity=None):
-    """Poisson arrivals at `rate` req/s, or at `load` x `capacity` pairs/s."""
+    """Poisson arrivals at `rate` req/s over `seconds`; or, given `load`, the
+    requests whose pairs fill load x capacity x seconds, arriving as a
+    Poisson stream whose rate makes the offered pair rate exactly
+    load x capacity."""
     rng = np.random.default_rng(seed)
     pairs_of = [float(s["n_rays"] * s["side"] ** 2) for s in specs]
     reqs = energy_scan_stream(rng, len(sets), [s["energy"] for s in specs], 20000)
     mean_pairs = float(np.mean([pairs_of[i] for i, _ in reqs]))
     if rate is None:
-        rate = load * capacity / mean_pairs
-    arrivals = []
-    t = 0.0
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > seconds:
-            break
-        arrivals.append(t)
-    reqs = reqs[:len(arrivals)]
+        budget_pairs = load * capacity * seconds
+        n, tot = 0, 0.0
+        while n < len(reqs) and tot < budget_pairs:
+            tot += pairs_of[reqs[n][0]]
+            n += 1
+        reqs = reqs[:n]
+        rate = load * capacity * n / tot
+        arrivals = list(np.cumsum(rng.exponential(1.0 / rate, n)))
+        seconds = arrivals[-1]
+    else:
+        arrivals = []
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > seconds:
+                break
+            arrivals.append(t)
+        reqs = reqs[:len(arrivals)]
     screens = [grid_of(xrtk, W.instance_screen(s)) for s in specs]
-    pairs_of = [float(s["n_rays"] * s["side"] ** 2) for s in specs]
     srv = xrtk.Server(ctx, hbm_budget_bytes=budget, max_active=64, quantum_ms=args.quantum_ms)
     tickets = []
     try:
@@ -376,8 +387,8 @@ def main():
     ap.add_argument("--batch-check", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--serve-load", type=float, default=0.7)
-    ap.add_argument("--serve-load-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-load", type=float, default=0.5)
+    ap.add_argument("--serve-load-seconds", type=float, default=15.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
