@@ -174,6 +174,31 @@ int xrtk_beam_to_rays(const double *bx, const double *by, const double *bz,
                       const double *normals3, const size_t *good,
                       size_t n_good, double *const out_cols[12]);
 
+/* ---- Fresnel propagation ------------------------------------------------ */
+
+/* FresnelScreen (src/waves/fresnel.rs:13-27): pixel (ix, iz) lies at
+ * x = x_min + (x_max - x_min) ix / max(nx - 1, 1), likewise z; the output
+ * index is iz * nx + ix (fresnel.rs:78-99). */
+typedef struct XrtkFresnelScreen {
+    double x_min, x_max, z_min, z_max;
+    uint32_t nx, nz;
+} XrtkFresnelScreen;
+
+/* fresnel_propagate (src/waves/fresnel.rs:53-106): from source rays on the
+ * z = 0 plane (x_j, z_j, complex Es_j) to the screen at distance dist,
+ *   U(x', z') = sum_j Es_j exp(i k/(2 dist) ((x' - x_j)^2 + (z' - z_j)^2)),
+ * k = E / CHBAR * 1e7.  out2 receives nx*nz complex values (re, im) in the
+ * index order above; x_pixels (nx) and z_pixels (nz) receive the axes and may
+ * be NULL.  opts->mode XRTK_MODE_F32R (default) multiplies FP32 factor
+ * matrices, XRTK_MODE_F64 FP64 ones; both evaluate every phase in FP64.
+ * nx or nz of 0 is XRTK_EINVAL (the reference's max(n - 1, 1) underflows
+ * there); 0 rays gives zeros. */
+int xrtk_fresnel_propagate(xrtk_ctx *ctx, const double *ray_x, const double *ray_z,
+                           const double *es_re, const double *es_im, size_t n_rays,
+                           double energy, double dist, const XrtkFresnelScreen *screen,
+                           double *out2, double *x_pixels, double *z_pixels,
+                           const XrtkOpts *opts);
+
 /* ---- resident ray sets and batched solves ------------------------------ */
 
 typedef uint64_t xrtk_rayset;

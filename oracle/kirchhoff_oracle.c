@@ -16,6 +16,10 @@
  *                            :96-115,276-297 + src/gpu/shaders/kirchhoff.wgsl
  *                            :65-129, in float32 (the reference GPU path)
  *   oracle_prepare_wave      src/waves/prepare_wave.rs:40-68
+ *   oracle_fresnel_grid      the pixel grid of src/waves/fresnel.rs:66-73
+ *   oracle_fresnel_f64       src/waves/fresnel.rs:53-106 (f64, plain complex
+ *                            sum in ray order, rayon over pixels -> OpenMP;
+ *                            output index iz * nx + ix)
  *   oracle_diffraction_ld    the same sum as oracle_diffraction_f64 with the
  *                            path and phase in x87 extended precision (64-bit
  *                            significand, sinl/cosl): the checker for phases
@@ -262,6 +266,50 @@ void oracle_diffraction_ld(const double *const cols[12], const double *path_up, 
   for (long long q = 0; q < nout; ++q) {
     const size_t p = (size_t)q * pix_stride;
     pixel_ld(&r, path_up, nr, px[p], py[p], pz[p], out4 + 4 * q);
+  }
+}
+
+/* Fresnel screen axis (src/waves/fresnel.rs:66-73):
+ * v[i] = lo + (hi - lo) * i / max(n - 1, 1), evaluated as
+ * lo + (((hi - lo) * i) / d) like the reference expression. */
+void oracle_fresnel_grid(double lo, double hi, size_t n, double *v) {
+  const double d = (double)(n > 1 ? n - 1 : 1);
+  for (size_t i = 0; i < n; ++i) v[i] = lo + (hi - lo) * (double)i / d;
+}
+
+/* fresnel_propagate (src/waves/fresnel.rs:53-106):
+ *   U(x', z') = sum_j Es_j exp(prefactor * ((x' - x_j)^2 + (z' - z_j)^2)),
+ *   prefactor = i k / (2 dist), k = E / CHBAR * 1e7.
+ * prefactor * r2 is (0, (k / (2 dist)) * r2); its exp is (cos, sin) of that
+ * angle (num-complex from_polar(exp(0), im)); the product with Es and the
+ * sum follow num-complex's textbook formulas in ray order.  Pixels are
+ * evaluated for q = 0, stride, 2 stride, ... of the iz * nx + ix order. */
+void oracle_fresnel_f64(const double *rx, const double *rz, const double *es_re,
+                        const double *es_im, size_t nr, double energy, double dist,
+                        const double *xp, size_t nx, const double *zp, size_t nz,
+                        size_t pix_stride, int threads, double *out2) {
+  const double k = energy / CHBAR * 1e7;
+  const double pim = k / (2.0 * dist);
+  if (pix_stride < 1) pix_stride = 1;
+  const size_t np = nx * nz;
+  const long long nout = (long long)((np + pix_stride - 1) / pix_stride);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+  for (long long q = 0; q < nout; ++q) {
+    const size_t p = (size_t)q * pix_stride;
+    const size_t iz = p / nx, ix = p % nx;
+    const double x = xp[ix], z = zp[iz];
+    double sr = 0.0, si = 0.0;
+    for (size_t j = 0; j < nr; ++j) {
+      const double dx = x - rx[j];
+      const double dz = z - rz[j];
+      const double r2 = dx * dx + dz * dz;
+      const double th = pim * r2;
+      const double c = cos(th), s = sin(th);
+      sr += es_re[j] * c - es_im[j] * s;
+      si += es_re[j] * s + es_im[j] * c;
+    }
+    out2[2 * q] = sr;
+    out2[2 * q + 1] = si;
   }
 }
 

@@ -34,6 +34,11 @@ def _load():
     lib.oracle_fallback_f32.restype = None
     lib.oracle_reduced_f32.argtypes = [fp, C.c_size_t, fp, C.c_size_t, dp]
     lib.oracle_reduced_f32.restype = None
+    lib.oracle_fresnel_grid.argtypes = [C.c_double, C.c_double, C.c_size_t, dp]
+    lib.oracle_fresnel_grid.restype = None
+    lib.oracle_fresnel_f64.argtypes = [dp, dp, dp, dp, C.c_size_t, C.c_double, C.c_double, dp,
+                                       C.c_size_t, dp, C.c_size_t, C.c_size_t, C.c_int, dp]
+    lib.oracle_fresnel_f64.restype = None
     lib.oracle_prepare_wave.argtypes = [dp, C.c_double, C.c_double, C.c_size_t, C.c_size_t,
                                         dp, dp, dp]
     lib.oracle_prepare_wave.restype = None
@@ -113,3 +118,26 @@ def prepare_wave(center, dx, dz, nx, nz):
     x, y, z = np.zeros(n), np.zeros(n), np.zeros(n)
     lib.oracle_prepare_wave(_d(c), float(dx), float(dz), nx, nz, _d(x), _d(y), _d(z))
     return x, y, z
+
+
+def fresnel_grid(lo, hi, n):
+    v = np.zeros(int(n))
+    lib.oracle_fresnel_grid(float(lo), float(hi), int(n), _d(v))
+    return v
+
+
+def fresnel_f64(ray_x, ray_z, ray_es, energy, dist, x_min, x_max, nx, z_min, z_max, nz,
+                stride=1, threads=None):
+    """fresnel_propagate (src/waves/fresnel.rs:53-106): (n_out,) complex128 for
+    pixels 0, stride, 2 stride, ... of the iz * nx + ix order, plus the axes."""
+    rx = np.ascontiguousarray(ray_x, dtype=np.float64)
+    rz = np.ascontiguousarray(ray_z, dtype=np.float64)
+    es = np.ascontiguousarray(ray_es, dtype=np.complex128)
+    er, ei = np.ascontiguousarray(es.real), np.ascontiguousarray(es.imag)
+    xp, zp = fresnel_grid(x_min, x_max, nx), fresnel_grid(z_min, z_max, nz)
+    n_out = (nx * nz + stride - 1) // stride
+    out = np.zeros(2 * n_out)
+    lib.oracle_fresnel_f64(_d(rx), _d(rz), _d(er), _d(ei), rx.shape[0], float(energy),
+                           float(dist), _d(xp), int(nx), _d(zp), int(nz), int(stride),
+                           int(threads or lib.oracle_max_threads()), _d(out))
+    return out.view(np.complex128), xp, zp

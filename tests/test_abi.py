@@ -120,6 +120,22 @@ def test_device_entry_points_report_enodev_without_context():
     assert lib.xrtk_solve_jobs(None, None, 0, None) == _lib.ENODEV
     srv = C.c_void_p()
     assert lib.xrtk_server_create(None, None, C.byref(srv)) in (_lib.ENODEV, _lib.EINVAL)
+    sc = _lib.XrtkFresnelScreen(-0.1, 0.1, -0.1, 0.1, 4, 4)
+    out = np.zeros(32)
+    assert lib.xrtk_fresnel_propagate(None, None, None, None, None, 0, 1e4, 1e3, C.byref(sc),
+                                      _lib.dptr(out), None, None, None) == _lib.ENODEV
+
+
+def test_fresnel_argument_errors():
+    """ValueError on mismatched ray columns and an empty screen axis, before
+    any device is needed (the reference's max(n - 1, 1) underflows at n = 0)."""
+    import xrtk
+    scr = xrtk.FresnelScreen(-0.1, 0.1, 4, -0.1, 0.1, 4)
+    with pytest.raises(ValueError):
+        xrtk.fresnel_propagate([0.0, 1.0], [0.0], [1 + 0j, 1 + 0j], 1e4, 1e3, scr)
+    with pytest.raises(ValueError):
+        xrtk.fresnel_propagate([0.0], [0.0], [1 + 0j], 1e4, 1e3,
+                               xrtk.FresnelScreen(-0.1, 0.1, 0, -0.1, 0.1, 4))
 
 
 def test_workload_generators_deterministic():

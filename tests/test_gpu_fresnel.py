@@ -1,0 +1,112 @@
+"""Fresnel propagation on the device (src/waves/fresnel.rs:53-106): both
+factor precisions against the oracle, the reference's unit tests, closed
+forms, screen axes and ray chunking."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import xrtk
+from xrtk import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL_F32 = 1e-5   # FP32 factors and products, FP64 phases and accumulation
+REL_F64 = 1e-11  # FP64 factors and products
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def k_of(e):
+    return e / W.CHBAR * 1e7
+
+
+def run_both(ctx, rx, rz, es, e, d, scr):
+    return {m: xrtk.fresnel_propagate(rx, rz, es, e, d, scr, ctx=ctx, mode=m)
+            for m in ("f32r", "f64")}
+
+
+def test_reference_unit_tests(ctx):
+    """fresnel_single_point_source and fresnel_two_slit (fresnel.rs:130-165)."""
+    r = xrtk.fresnel_propagate([0.0], [0.0], [1 + 0j], 10000.0, 1000.0,
+                               xrtk.FresnelScreen(-0.1, 0.1, 21, -0.1, 0.1, 21), ctx=ctx)
+    assert r.amplitude.shape == (21 * 21,)
+    assert abs(r.amplitude[10 * 21 + 10]) >= abs(r.amplitude[0]) * (1 - 1e-6)
+    r = xrtk.fresnel_propagate([-0.01, 0.01], [0.0, 0.0], [1 + 0j, 1 + 0j], 10000.0, 1000.0,
+                               xrtk.FresnelScreen(-0.1, 0.1, 101, 0.0, 0.0, 1), ctx=ctx)
+    inten = r.intensity()
+    assert inten[50] > 0.0 and inten[60] > 0.0
+
+
+def test_closed_forms(ctx):
+    e, d, a = 10000.0, 1000.0, 0.01
+    scr = xrtk.FresnelScreen(-0.1, 0.1, 101, 0.0, 0.0, 1)
+    for m, r in run_both(ctx, [-a, a], [0.0, 0.0], [1 + 0j, 1 + 0j], e, d, scr).items():
+        exp = 2 + 2 * np.cos(2 * k_of(e) * a * r.x_pixels / d)
+        tol = 1e-9 if m == "f64" else 2e-6
+        assert np.max(np.abs(r.intensity() - exp)) < tol, m
+    x0, z0, es = 0.013, -0.007, 0.6 - 0.8j
+    scr = xrtk.FresnelScreen(-0.05, 0.05, 9, -0.04, 0.06, 7)
+    for m, r in run_both(ctx, [x0], [z0], [es], 12000.0, 750.0, scr).items():
+        X, Z = np.meshgrid(r.x_pixels, r.z_pixels)
+        exp = es * np.exp(1j * (k_of(12000.0) / (2 * 750.0)) * ((X - x0) ** 2 + (Z - z0) ** 2))
+        tol = 1e-12 if m == "f64" else 2e-7
+        assert np.max(np.abs(r.amplitude - exp.ravel())) < tol, m
+
+
+def test_axes_match_reference_formula(ctx):
+    scr = xrtk.FresnelScreen(-0.1, 0.3, 5, 0.25, 9.0, 1)
+    r = xrtk.fresnel_propagate([0.0], [0.0], [1 + 0j], 1e4, 1e3, scr, ctx=ctx)
+    assert np.array_equal(r.x_pixels, O.fresnel_grid(-0.1, 0.3, 5))
+    assert np.array_equal(r.z_pixels, O.fresnel_grid(0.25, 9.0, 1))
+
+
+def test_zero_rays(ctx):
+    r = xrtk.fresnel_propagate([], [], [], 1e4, 1e3, xrtk.FresnelScreen(-1, 1, 3, -1, 1, 2),
+                               ctx=ctx)
+    assert r.amplitude.shape == (6,) and not np.any(r.amplitude)
+
+
+def _aperture(n, half, e, seed):
+    rays = W.point_source_aperture(n, half, e, seed)
+    return rays["x"], rays["z"], rays["es_re"] + 1j * rays["es_im"]
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (1, 37), (29, 1)])
+def test_vs_oracle_small(ctx, shape):
+    nx, nz = shape
+    rx, rz, es = _aperture(4096, 0.1, 10000.0, 5)
+    scr = xrtk.FresnelScreen(-0.06, 0.05, nx, -0.04, 0.07, nz)
+    ref, _, _ = O.fresnel_f64(rx, rz, es, 10000.0, 1000.0, scr.x_min, scr.x_max, nx,
+                              scr.z_min, scr.z_max, nz)
+    got = run_both(ctx, rx, rz, es, 10000.0, 1000.0, scr)
+    assert rel_l2(got["f64"].amplitude, ref) < REL_F64
+    assert rel_l2(got["f32r"].amplitude, ref) < REL_F32
+
+
+def test_config0_aperture_to_256sq(ctx):
+    """65,536 point-source rays on +-0.1 mm at 10 keV, 1 m to a 256^2 screen of
+    +-0.05 mm; oracle on a strided pixel sample."""
+    rx, rz, es = _aperture(65536, 0.1, 10000.0, 1)
+    scr = xrtk.FresnelScreen(-0.05, 0.05, 256, -0.05, 0.05, 256)
+    stride = 97
+    ref, _, _ = O.fresnel_f64(rx, rz, es, 10000.0, 1000.0, -0.05, 0.05, 256, -0.05, 0.05, 256,
+                              stride=stride)
+    got = run_both(ctx, rx, rz, es, 10000.0, 1000.0, scr)
+    e64, e32 = rel_l2(got["f64"].amplitude[::stride], ref), rel_l2(got["f32r"].amplitude[::stride], ref)
+    print(f"fresnel 65536 x 256^2: f64 {e64:.2e}, f32 factors {e32:.2e}, "
+          f"f32 vs f64 over all pixels {rel_l2(got['f32r'].amplitude, got['f64'].amplitude):.2e}")
+    assert e64 < REL_F64 and e32 < REL_F32
+
+
+def test_ray_chunks(ctx):
+    """A wide screen makes the ray chunk short (several GEMMs accumulate)."""
+    rx, rz, es = _aperture(20000, 0.05, 8000.0, 11)
+    scr = xrtk.FresnelScreen(-0.2, 0.2, 8192, -0.01, 0.01, 3)
+    stride = 211
+    ref, _, _ = O.fresnel_f64(rx, rz, es, 8000.0, 500.0, -0.2, 0.2, 8192, -0.01, 0.01, 3,
+                              stride=stride)
+    got = run_both(ctx, rx, rz, es, 8000.0, 500.0, scr)
+    assert rel_l2(got["f64"].amplitude[::stride], ref) < REL_F64
+    assert rel_l2(got["f32r"].amplitude[::stride], ref) < REL_F32

@@ -134,6 +134,9 @@ struct xrtk_ctx {
     xrtk::Plan plan;
   };
   std::vector<GraphEntry*> graphs;
+  // Fresnel propagation: rocBLAS handle (created on first use) and workspace
+  void* blas = nullptr;
+  xrtk::DBuf fresnel;
 };
 
 namespace xrtk {
@@ -179,5 +182,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
 
 int map_err(hipError_t e);
 double now_s();
+// Releases the Fresnel workspace and rocBLAS handle of a context.
+void fresnel_release(xrtk_ctx* ctx);
 
 }  // namespace xrtk

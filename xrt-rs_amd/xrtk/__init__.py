@@ -31,7 +31,7 @@ __all__ = [
     "CHBAR", "GPU_RAY_DTYPE", "GPU_PIXEL_DTYPE", "GpuContext", "kirchhoff_gpu",
     "kirchhoff_auto", "diffraction_integral", "diffraction_integral_rs", "WaveParams",
     "prepare_wave", "beam_to_diffraction_rays", "RaySet", "Job", "solve_jobs", "autotune_jobs",
-    "Server",
+    "Server", "FresnelScreen", "FresnelResult", "fresnel_propagate",
     "XrtkError", "MODE_F32R", "MODE_F64", "RAY_FIELDS",
 ]
 
@@ -418,6 +418,61 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class FresnelScreen:
+    """FresnelScreen (src/waves/fresnel.rs:13-27)."""
+    x_min: float
+    x_max: float
+    nx: int
+    z_min: float
+    z_max: float
+    nz: int
+
+    def to_c(self) -> _lib.XrtkFresnelScreen:
+        return _lib.XrtkFresnelScreen(float(self.x_min), float(self.x_max), float(self.z_min),
+                                      float(self.z_max), int(self.nx), int(self.nz))
+
+
+@dataclass
+class FresnelResult:
+    """FresnelResult (src/waves/fresnel.rs:29-48): amplitude[iz * nx + ix]."""
+    amplitude: np.ndarray
+    x_pixels: np.ndarray
+    z_pixels: np.ndarray
+    nx: int
+    nz: int
+
+    def intensity(self) -> np.ndarray:
+        return np.abs(self.amplitude) ** 2
+
+
+def fresnel_propagate(ray_x, ray_z, ray_es, energy: float, dist: float, screen: FresnelScreen,
+                      ctx: Optional[GpuContext] = None, mode="f32r") -> FresnelResult:
+    """fresnel_propagate (src/waves/fresnel.rs:53-106) on the device: source
+    rays at (x, z) on the plane z = 0 with complex amplitudes ray_es,
+    propagated dist mm at energy eV onto the screen grid."""
+    rx = np.ascontiguousarray(np.asarray(ray_x, dtype=np.float64).reshape(-1))
+    rz = np.ascontiguousarray(np.asarray(ray_z, dtype=np.float64).reshape(-1))
+    es = np.asarray(ray_es, dtype=np.complex128).reshape(-1)
+    if not (rx.shape == rz.shape == es.shape):
+        raise ValueError("ray_x, ray_z and ray_es differ in length")
+    if int(screen.nx) < 1 or int(screen.nz) < 1:
+        raise ValueError("screen needs nx >= 1 and nz >= 1")
+    ctx = ctx or GpuContext.shared()
+    if ctx is None:
+        raise XrtkError(_lib.ENODEV, "fresnel_propagate")
+    er, ei = np.ascontiguousarray(es.real), np.ascontiguousarray(es.imag)
+    out = np.zeros(2 * int(screen.nx) * int(screen.nz))
+    xp, zp = np.zeros(int(screen.nx)), np.zeros(int(screen.nz))
+    sc = screen.to_c()
+    o = _lib.opts(_mode(mode))
+    check(lib.xrtk_fresnel_propagate(ctx.handle, dptr(rx), dptr(rz), dptr(er), dptr(ei),
+                                     rx.shape[0], float(energy), float(dist), C.byref(sc),
+                                     dptr(out), dptr(xp), dptr(zp), C.byref(o)),
+          "xrtk_fresnel_propagate")
+    return FresnelResult(out.view(np.complex128), xp, zp, int(screen.nx), int(screen.nz))
 
 
 def now() -> float:

@@ -81,6 +81,11 @@ class XrtkPlan(C.Structure):
         "persistent", "n_items", "n_fp64", "backend", "inst_flags", "reserved")]
 
 
+class XrtkFresnelScreen(C.Structure):
+    _fields_ = [("x_min", C.c_double), ("x_max", C.c_double), ("z_min", C.c_double),
+                ("z_max", C.c_double), ("nx", C.c_uint32), ("nz", C.c_uint32)]
+
+
 class XrtkServerCfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int32),
                 ("quantum_ms", C.c_double), ("mode", C.c_int32), ("reserved", C.c_int32)]
@@ -129,6 +134,9 @@ PROTOTYPES = {
     "xrtk_auto_last_backend": (_i, []),
     "xrtk_autotune": (_i, [_p, _sz, _sz, _i, C.POINTER(XrtkOpts), _dp]),
     "xrtk_autotune_jobs": (_i, [_p, C.POINTER(XrtkJob), _sz, _i, C.POINTER(XrtkOpts), _dp]),
+    "xrtk_fresnel_propagate": (_i, [_p, _dp, _dp, _dp, _dp, _sz, C.c_double, C.c_double,
+                                    C.POINTER(XrtkFresnelScreen), _dp, _dp, _dp,
+                                    C.POINTER(XrtkOpts)]),
     "xrtk_server_create": (_i, [_p, C.POINTER(XrtkServerCfg), C.POINTER(_p)]),
     "xrtk_server_submit": (_i, [_p, C.POINTER(XrtkJob), C.POINTER(C.c_uint64)]),
     "xrtk_server_poll": (_i, [_p, C.c_uint64]),
