@@ -19,7 +19,10 @@ Further legs, reported in the same JSON line:
             a fraction of the measured batched capacity;
   cpu       the oracle (the reference's f64 Kahan loop restated in C, OpenMP
             over pixels) timed on a strided pixel sample of the headline
-            workload, all threads and one thread.
+            workload, all threads and one thread;
+  fresnel   fresnel_propagate of the same rays onto a 1024^2 screen (complex
+            GEMM of FP64-phase factors), FP32 and FP64 factors, with the
+            oracle's loop on a pixel sample.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -41,6 +44,8 @@ import numpy as np  # noqa: E402
 F_PAIR = 51                 # FP operations per pair (FMA = 2, transcendental = 1)
 PEAK_FP32 = 157.3           # TFLOP/s, MI355X FP32 vector datasheet
 PEAK_FP64 = 78.6            # TFLOP/s, MI355X FP64 vector
+PEAK_MFMA_FP32 = 157.3      # TFLOP/s, MI355X dense FP32 matrix
+PEAK_MFMA_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0           # GB/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
 
@@ -274,6 +279,58 @@ def leg_batched(args, ctx, xrtk, W):
     return res, sets, jobs, specs
 
 
+def leg_fresnel(args, ctx, xrtk, W):
+    """fresnel_propagate: the Gaussian-source rays (x, z, Es) of the headline
+    case, 10 keV, 1000 mm onto a side^2 screen of +-0.1 mm."""
+    rays = W.gaussian_source(args.rays, seed=2)
+    rx, rz = rays["x"], rays["z"]
+    es = rays["es_re"] + 1j * rays["es_im"]
+    side = args.fresnel_side
+    scr = xrtk.FresnelScreen(-0.1, 0.1, side, -0.1, 0.1, side)
+    pairs = float(rx.shape[0]) * side * side
+    res = {"workload": f"fresnel_propagate, {rx.shape[0]} Gaussian-source rays, 10 keV, "
+                       f"1000 mm, {side}^2 screen of +-0.1 mm", "pairs": pairs}
+    out = {}
+    for mode, steps in (("f32r", args.fresnel_steps), ("f64", max(1, args.fresnel_steps // 2))):
+        xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode)
+        wall, dev = [], []
+        for _ in range(steps):
+            t = time.perf_counter()
+            r = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode)
+            wall.append(time.perf_counter() - t)
+            dev.append(ctx.last_timing()[1])
+        out[mode] = r.amplitude
+        d_ms = float(np.median(dev))
+        peak = PEAK_MFMA_FP32 if mode == "f32r" else PEAK_MFMA_FP64
+        tflops = pairs * 8 / (d_ms * 1e-3) / 1e12
+        res[mode] = {"value": pairs / float(np.median(wall)) / 1e9, "unit": "Gpair-evals/s",
+                     "ms_per_call": 1e3 * float(np.median(wall)), "device_ms": d_ms,
+                     "device_gpairs_per_s": pairs / (d_ms * 1e-3) / 1e9,
+                     "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak,
+                                  "unit": "TFLOP/s", "frac": tflops / peak,
+                                  "flop_per_pair": 8}}
+        log(f"fresnel {mode}: {1e3 * np.median(wall):.1f} ms/call (device {d_ms:.1f} ms), "
+            f"{res[mode]['value']:.0f} Gpair/s, {tflops:.1f} TFLOP/s")
+    res["rel_l2_f32r_vs_f64"] = W.rel_l2(out["f32r"], out["f64"])
+    if not args.no_cpu:
+        import pyoracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        stride = max(1, side * side // max(threads, int(args.fresnel_cpu_pixels)))
+        t = time.perf_counter()
+        ref, _, _ = pyoracle.fresnel_f64(rx, rz, es, 10000.0, 1000.0, -0.1, 0.1, side, -0.1, 0.1,
+                                         side, stride=stride, threads=threads)
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": ref.shape[0] * rx.shape[0] / dt / 1e9,
+                               "unit": "Gpair-evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{ref.shape[0]} pixels (every {stride}th) x "
+                                         f"{rx.shape[0]} rays, {dt:.1f} s"}
+        res["rel_l2_f32r_vs_oracle_on_sample"] = W.rel_l2(out["f32r"][::stride], ref)
+        res["rel_l2_f64_vs_oracle_on_sample"] = W.rel_l2(out["f64"][::stride], ref)
+        log(f"fresnel cpu: {res['cpu_baseline']['value']:.4f} Gpair/s; f32r vs oracle "
+            f"{res['rel_l2_f32r_vs_oracle_on_sample']:.2e}")
+    return res
+
+
 def energy_scan_stream(rng, n_sets, base_energy, n_req, concurrent=16):
     """(set index, energy) requests from `concurrent` interleaved energy scans:
     each scan covers U{1..50} energies (5 eV steps around the set's energy)
@@ -392,6 +449,10 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-fresnel", action="store_true")
+    ap.add_argument("--fresnel-side", type=int, default=1024)
+    ap.add_argument("--fresnel-steps", type=int, default=4)
+    ap.add_argument("--fresnel-cpu-pixels", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the Kirchhoff engine has no multi-GPU path "
@@ -415,6 +476,8 @@ def main():
     cpu = None if args.no_cpu else leg_cpu(args, xrtk, W, rays, wave, out32, head["nr"],
                                            head["npx"])
     del rays
+
+    fresnel = None if args.no_fresnel else leg_fresnel(args, ctx, xrtk, W)
 
     batched = serving = None
     if not args.no_batch:
@@ -452,6 +515,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "fresnel": fresnel,
     }
     print(json.dumps(line), flush=True)
 

@@ -11,11 +11,15 @@
 //
 // a complex matrix product over the rays.  The two factor matrices
 // ((nx + nz) n_rays entries, phases reduced and evaluated in FP64) are built
-// by k_fresnel_factor; the product, nx nz n_rays complex multiply-adds, runs
-// on the matrix cores through rocBLAS (cgemm with FP32 factors, or zgemm with
-// FP64 factors), chunk by chunk over the rays; k_accum_c64 adds each FP32
-// chunk product into the FP64 result.  Output index iz * nx + ix: the
-// column-major nx x nz result.
+// by k_fresnel_factor*; the product, nx nz n_rays complex multiply-adds, runs
+// on the matrix cores through rocBLAS, chunk by chunk over the rays:
+//   FP64  zgemm on the complex factors;
+//   FP32  one real sgemm on the block forms
+//           [Xr -Xi; Xi Xr] (2 nx x 2 nc) times [Zr Zi]^T (2 nc x nz)
+//         = [Ur; Ui] (2 nx x nz), the same 8 flops per pair, which gfx950's
+//         FP32 matrix path runs at about twice the rate of cgemm;
+//         k_accum_blk adds each chunk product into the FP64 result.
+// Output index iz * nx + ix: the column-major nx x nz result.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -62,16 +66,59 @@ __global__ __launch_bounds__(256) void k_fresnel_factor(const double* __restrict
   }
 }
 
-// acc[i] += c[i] (FP32 chunk product into the FP64 result).
-__global__ __launch_bounds__(256) void k_accum_c64(const float2* __restrict__ c, size_t n,
-                                                   double2* __restrict__ acc) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const float2 v = c[i];
-    double2 a = acc[i];
-    a.x += (double)v.x;
-    a.y += (double)v.y;
-    acc[i] = a;
+// Block factors for the real product, FP32 (column-major):
+//   kind 0 (X):  A is 2n x 2nc, A[i][j] = A[n+i][nc+j] = Re, A[n+i][j] = Im,
+//                A[i][nc+j] = -Im;
+//   kind 1 (Z'): B is n x 2nc,  B[i][j] = Re, B[i][nc+j] = Im;
+// with Re + i Im = (amp_j or 1) exp(i 2 pi krev (p_i - r_j)^2).
+__global__ __launch_bounds__(256) void k_fresnel_factor_blk(const double* __restrict__ p,
+                                                            uint32_t n,
+                                                            const double* __restrict__ r,
+                                                            const double2* __restrict__ amp,
+                                                            uint32_t nc, double krev, int kind,
+                                                            float* __restrict__ out) {
+  const size_t total = (size_t)n * nc;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = e % n, j = e / n;
+    double s, c;
+    {
+#pragma clang fp contract(off)
+      const double d = p[i] - r[j];
+      const double ph = krev * (d * d);  // revolutions
+      sincospi(2.0 * (ph - rint(ph)), &s, &c);
+    }
+    if (amp) {
+      const double2 a = amp[j];
+      const double re = a.x * c - a.y * s, im = a.x * s + a.y * c;
+      c = re;
+      s = im;
+    }
+    if (kind == 0) {
+      const size_t ld = 2 * (size_t)n;
+      out[i + j * ld] = (float)c;
+      out[i + (nc + j) * ld] = (float)-s;
+      out[n + i + j * ld] = (float)s;
+      out[n + i + (nc + j) * ld] = (float)c;
+    } else {
+      out[i + j * (size_t)n] = (float)c;
+      out[i + (nc + j) * (size_t)n] = (float)s;
+    }
+  }
+}
+
+// acc[ix + iz nx] += (C[ix + iz 2nx], C[nx + ix + iz 2nx]): the FP32 chunk
+// product [Ur; Ui] into the FP64 result.
+__global__ __launch_bounds__(256) void k_accum_blk(const float* __restrict__ c, uint32_t nx,
+                                                   size_t np, double2* __restrict__ acc) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < np;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t ix = e % nx, iz = e / nx;
+    const float* col = c + iz * 2 * (size_t)nx;
+    double2 a = acc[e];
+    a.x += (double)col[ix];
+    a.y += (double)col[nx + ix];
+    acc[e] = a;
   }
 }
 
@@ -90,15 +137,15 @@ struct Blas {
   using create_t = int (*)(void**);
   using destroy_t = int (*)(void*);
   using stream_t = int (*)(void*, hipStream_t);
-  using cgemm_t = int (*)(void*, int, int, int, int, int, const float2*, const float2*, int,
-                          const float2*, int, const float2*, float2*, int);
+  using sgemm_t = int (*)(void*, int, int, int, int, int, const float*, const float*, int,
+                          const float*, int, const float*, float*, int);
   using zgemm_t = int (*)(void*, int, int, int, int, int, const double2*, const double2*, int,
                           const double2*, int, const double2*, double2*, int);
   void* lib = nullptr;
   create_t create = nullptr;
   destroy_t destroy = nullptr;
   stream_t set_stream = nullptr;
-  cgemm_t cgemm = nullptr;
+  sgemm_t sgemm = nullptr;
   zgemm_t zgemm = nullptr;
   bool ok = false;
 };
@@ -116,9 +163,9 @@ Blas& blas() {
     b.create = reinterpret_cast<Blas::create_t>(dlsym(b.lib, "rocblas_create_handle"));
     b.destroy = reinterpret_cast<Blas::destroy_t>(dlsym(b.lib, "rocblas_destroy_handle"));
     b.set_stream = reinterpret_cast<Blas::stream_t>(dlsym(b.lib, "rocblas_set_stream"));
-    b.cgemm = reinterpret_cast<Blas::cgemm_t>(dlsym(b.lib, "rocblas_cgemm"));
+    b.sgemm = reinterpret_cast<Blas::sgemm_t>(dlsym(b.lib, "rocblas_sgemm"));
     b.zgemm = reinterpret_cast<Blas::zgemm_t>(dlsym(b.lib, "rocblas_zgemm"));
-    b.ok = b.create && b.destroy && b.set_stream && b.cgemm && b.zgemm;
+    b.ok = b.create && b.destroy && b.set_stream && b.sgemm && b.zgemm;
   });
   return b;
 }
@@ -170,7 +217,10 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
   if (!screen || !out2 || screen->nx == 0 || screen->nz == 0) return XRTK_EINVAL;
   if (n_rays && (!ray_x || !ray_z || !es_re || !es_im)) return XRTK_EINVAL;
   const uint32_t nx = screen->nx, nz = screen->nz;
-  if ((uint64_t)nx * nz > (1ull << 31) || n_rays > 0x7fffffffull) return XRTK_EINVAL;
+  // rocBLAS takes int dimensions; the FP32 product has 2 nx rows and 2 nc columns
+  if ((uint64_t)nx * nz > (1ull << 31) || nx > (1u << 29) || nz > (1u << 30) ||
+      n_rays > 0x7fffffffull)
+    return XRTK_EINVAL;
   const bool f64 = opts && opts->mode == XRTK_MODE_F64;
   try {
     std::vector<double> xp(nx), zp(nz);
@@ -193,8 +243,9 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     }
     if (b.set_stream(ctx->blas, st) != 0) return XRTK_ELAUNCH;
 
-    // ray chunk: each factor block at most ~256 MiB
-    const size_t esz = f64 ? sizeof(double2) : sizeof(float2);
+    // ray chunk: each factor block at most ~256 MiB (an FP32 X entry takes
+    // four floats in the block form)
+    const size_t esz = f64 ? sizeof(double2) : 4 * sizeof(float);
     const size_t cap = (256ull << 20) / (esz * std::max(nx, nz));
     size_t nc = std::min<size_t>(n_rays, std::max<size_t>(256, cap / 256 * 256));
     // device buffers: pixel axes, rays (x, z, Es), factors, chunk product, result
@@ -204,7 +255,7 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     const size_t o_fx = (o_es + 16 * n_rays + 255) / 256 * 256;
     const size_t o_fz = (o_fx + esz * nx * nc + 255) / 256 * 256;
     const size_t o_c = (o_fz + esz * nz * nc + 255) / 256 * 256;
-    const size_t o_u = (o_c + (f64 ? 0 : sizeof(float2) * np) + 255) / 256 * 256;
+    const size_t o_u = (o_c + (f64 ? 0 : 2 * sizeof(float) * np) + 255) / 256 * 256;
     const size_t total = o_u + sizeof(double2) * np;
     if (W.ensure(total) != hipSuccess) return XRTK_ENOMEM;
     char* base = W.as<char>();
@@ -240,18 +291,18 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
                     FZ, (int)nz, j0 ? &one : &zero, U, (int)nx) != 0)
           return XRTK_ELAUNCH;
       } else {
-        float2* FX = reinterpret_cast<float2*>(base + o_fx);
-        float2* FZ = reinterpret_cast<float2*>(base + o_fz);
-        float2* Cc = reinterpret_cast<float2*>(base + o_c);
-        hipLaunchKernelGGL(k_fresnel_factor<float2>, dim3(grid_for((size_t)nx * c)), dim3(256),
-                           0, st, dxp, nx, drx + j0, nullptr, c, krev, FX);
-        hipLaunchKernelGGL(k_fresnel_factor<float2>, dim3(grid_for((size_t)nz * c)), dim3(256),
-                           0, st, dzp, nz, drz + j0, des + j0, c, krev, FZ);
-        const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f);
-        if (b.cgemm(ctx->blas, kOpNone, kOpTrans, (int)nx, (int)nz, (int)c, &one, FX, (int)nx,
-                    FZ, (int)nz, &zero, Cc, (int)nx) != 0)
+        float* FX = reinterpret_cast<float*>(base + o_fx);
+        float* FZ = reinterpret_cast<float*>(base + o_fz);
+        float* Cc = reinterpret_cast<float*>(base + o_c);
+        hipLaunchKernelGGL(k_fresnel_factor_blk, dim3(grid_for((size_t)nx * c)), dim3(256), 0,
+                           st, dxp, nx, drx + j0, nullptr, c, krev, 0, FX);
+        hipLaunchKernelGGL(k_fresnel_factor_blk, dim3(grid_for((size_t)nz * c)), dim3(256), 0,
+                           st, dzp, nz, drz + j0, des + j0, c, krev, 1, FZ);
+        const float one = 1.f, zero = 0.f;
+        if (b.sgemm(ctx->blas, kOpNone, kOpTrans, 2 * (int)nx, (int)nz, 2 * (int)c, &one, FX,
+                    2 * (int)nx, FZ, (int)nz, &zero, Cc, 2 * (int)nx) != 0)
           return XRTK_ELAUNCH;
-        hipLaunchKernelGGL(k_accum_c64, dim3(grid_for(np)), dim3(256), 0, st, Cc, np, U);
+        hipLaunchKernelGGL(k_accum_blk, dim3(grid_for(np)), dim3(256), 0, st, Cc, nx, np, U);
       }
     }
     (void)hipEventRecord(ctx->ev[3], st);
