@@ -199,6 +199,42 @@ int xrtk_fresnel_propagate(xrtk_ctx *ctx, const double *ray_x, const double *ray
                            double *out2, double *x_pixels, double *z_pixels,
                            const XrtkOpts *opts);
 
+/* ---- undulator intensity map ------------------------------------------- */
+
+/* The parameters Undulator::build_i_map reads (src/sources/undulator.rs
+ * :36-60 and SynchrotronParams, src/sources/bending_magnet.rs:38-53):
+ * gamma = E[GeV] 1e3 / 0.510998928, gamma2 = gamma * gamma. */
+typedef struct XrtkUndulator {
+    double gamma, gamma2;
+    double beam_current;   /* [A]                                          */
+    double kx, ky;         /* deflection parameters of B_x and B_y          */
+    double period_mm;
+    double phase_deg;      /* K_x - K_y phase                              */
+    uint32_t n_periods;
+    uint32_t n_steps;      /* integration steps per period (64 in build_i_map) */
+} XrtkUndulator;
+
+/* Undulator::build_i_map (src/sources/undulator.rs:120-244) on the device,
+ * FP64: intensity[n], amp_s and amp_p as n complex values (re, im) each.
+ * build_i_map_auto (:251-266) sends maps of 1024 points or more here. */
+int xrtk_undulator_i_map(xrtk_ctx *ctx, const XrtkUndulator *und, const double *energies,
+                         const double *thetas, const double *psis, size_t n,
+                         double *intensity, double *amp_s2, double *amp_p2);
+
+/* Byte-identical to GpuObsPoint (src/gpu/undulator.rs:14-22), 16 bytes. */
+typedef struct XrtkObsPoint {
+    float energy, theta, psi, _pad;
+} XrtkObsPoint;
+
+/* undulator_gpu (src/gpu/undulator.rs:66-91,93-275): out5 receives, per
+ * point, ax_re, ax_im, az_re, az_im, intensity (the fields of
+ * GpuUndulatorResult, undulator.rs:24-36), computed in FP64 from the f32
+ * inputs. */
+int xrtk_undulator_gpu_f32(xrtk_ctx *ctx, const XrtkObsPoint *obs, size_t n, double kx,
+                           double ky, double period_mm, size_t n_periods, double gamma,
+                           double beam_current, uint32_t n_steps, double phase_deg,
+                           double *out5);
+
 /* ---- resident ray sets and batched solves ------------------------------ */
 
 typedef uint64_t xrtk_rayset;

@@ -39,6 +39,17 @@ def _load():
     lib.oracle_fresnel_f64.argtypes = [dp, dp, dp, dp, C.c_size_t, C.c_double, C.c_double, dp,
                                        C.c_size_t, dp, C.c_size_t, C.c_size_t, C.c_int, dp]
     lib.oracle_fresnel_f64.restype = None
+    lib.oracle_undulator_gamma.argtypes = [C.c_double]
+    lib.oracle_undulator_gamma.restype = C.c_double
+    lib.oracle_undulator_e1.argtypes = [C.c_double] * 4
+    lib.oracle_undulator_e1.restype = C.c_double
+    lib.oracle_undulator_i_map.argtypes = [C.c_double] * 6 + [C.c_size_t, C.c_double, C.c_int,
+                                                               dp, dp, dp, C.c_size_t, C.c_int,
+                                                               dp, dp, dp]
+    lib.oracle_undulator_i_map.restype = None
+    lib.oracle_undulator_wgsl_f32.argtypes = [C.c_double] * 5 + [C.c_size_t, C.c_double, C.c_int,
+                                                                  fp, C.c_size_t, dp]
+    lib.oracle_undulator_wgsl_f32.restype = None
     lib.oracle_prepare_wave.argtypes = [dp, C.c_double, C.c_double, C.c_size_t, C.c_size_t,
                                         dp, dp, dp]
     lib.oracle_prepare_wave.restype = None
@@ -141,3 +152,52 @@ def fresnel_f64(ray_x, ray_z, ray_es, energy, dist, x_min, x_max, nx, z_min, z_m
                            float(dist), _d(xp), int(nx), _d(zp), int(nz), int(stride),
                            int(threads or lib.oracle_max_threads()), _d(out))
     return out.view(np.complex128), xp, zp
+
+
+def undulator_gamma(electron_energy_gev):
+    return lib.oracle_undulator_gamma(float(electron_energy_gev))
+
+
+def undulator_e1(gamma, kx, ky, period_mm):
+    return lib.oracle_undulator_e1(float(gamma), float(kx), float(ky), float(period_mm))
+
+
+def undulator_i_map(und: dict, energies, thetas, psis, n_steps=64, threads=None):
+    """build_i_map (src/sources/undulator.rs:120-244).  und: gamma, gamma2,
+    beam_current, kx, ky, period_mm, n_periods, phase_deg.  Returns
+    (intensity, amp_s, amp_p) as float64 / complex128 arrays."""
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    t = np.ascontiguousarray(thetas, dtype=np.float64)
+    p = np.ascontiguousarray(psis, dtype=np.float64)
+    n = e.shape[0]
+    inten, a_s, a_p = np.zeros(n), np.zeros(2 * n), np.zeros(2 * n)
+    lib.oracle_undulator_i_map(und["gamma"], und["gamma2"], und["beam_current"], und["kx"],
+                               und["ky"], und["period_mm"], int(und["n_periods"]),
+                               und.get("phase_deg", 0.0), int(n_steps), _d(e), _d(t), _d(p), n,
+                               int(threads or lib.oracle_max_threads()), _d(inten), _d(a_s),
+                               _d(a_p))
+    return inten, a_s.view(np.complex128), a_p.view(np.complex128)
+
+
+def undulator_wgsl_f32(und: dict, energies, thetas, psis, n_steps=64):
+    """The reference GPU path (undulator.wgsl) in float32: (intensity,
+    amp_s, amp_p)."""
+    e = np.asarray(energies, dtype=np.float64)
+    obs = np.zeros((e.shape[0], 4), dtype=np.float32)
+    obs[:, 0], obs[:, 1], obs[:, 2] = e, thetas, psis
+    out = np.zeros(5 * e.shape[0])
+    lib.oracle_undulator_wgsl_f32(und["gamma"], und["beam_current"], und["kx"], und["ky"],
+                                  und["period_mm"], int(und["n_periods"]),
+                                  und.get("phase_deg", 0.0), int(n_steps),
+                                  obs.ctypes.data_as(C.POINTER(C.c_float)), e.shape[0], _d(out))
+    out = out.reshape(-1, 5)
+    return out[:, 4].copy(), out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def undulator(electron_energy_gev, beam_current, kx, ky, period_mm, n_periods, phase_deg=0.0):
+    """The parameters of Undulator::new (src/sources/undulator.rs:64-92) as
+    the map functions take them."""
+    g = undulator_gamma(electron_energy_gev)
+    return {"gamma": g, "gamma2": g * g, "beam_current": float(beam_current), "kx": float(kx),
+            "ky": float(ky), "period_mm": float(period_mm), "n_periods": int(n_periods),
+            "phase_deg": float(phase_deg)}

@@ -438,6 +438,7 @@ void xrtk_ctx_destroy(xrtk_ctx* ctx) {
   ctx->stage.release(); ctx->rays64.release(); ctx->pix64.release(); ctx->perm.release();
   ctx->work.release(); ctx->meta.release(); ctx->outbuf.release();
   fresnel_release(ctx);
+  ctx->undulator.release();
   for (auto& kv : ctx->raysets) kv.second.rays.release();
   for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

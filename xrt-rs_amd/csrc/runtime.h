@@ -137,6 +137,8 @@ struct xrtk_ctx {
   // Fresnel propagation: rocBLAS handle (created on first use) and workspace
   void* blas = nullptr;
   xrtk::DBuf fresnel;
+  // undulator maps
+  xrtk::DBuf undulator;
 };
 
 namespace xrtk {

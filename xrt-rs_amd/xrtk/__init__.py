@@ -24,14 +24,15 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (CHBAR, EAGAIN, GPU_PIXEL_DTYPE, GPU_RAY_DTYPE, MODE_F32R, MODE_F64, OK,
-                   XrtkError, check, dptr, lib)
+from ._lib import (CHBAR, EAGAIN, GPU_OBS_DTYPE, GPU_PIXEL_DTYPE, GPU_RAY_DTYPE, MODE_F32R,
+                   MODE_F64, OK, XrtkError, check, dptr, lib)
 
 __all__ = [
     "CHBAR", "GPU_RAY_DTYPE", "GPU_PIXEL_DTYPE", "GpuContext", "kirchhoff_gpu",
     "kirchhoff_auto", "diffraction_integral", "diffraction_integral_rs", "WaveParams",
     "prepare_wave", "beam_to_diffraction_rays", "RaySet", "Job", "solve_jobs", "autotune_jobs",
-    "Server", "FresnelScreen", "FresnelResult", "fresnel_propagate",
+    "Server", "FresnelScreen", "FresnelResult", "fresnel_propagate", "Undulator",
+    "UndulatorGpuResult", "undulator_gpu", "GPU_OBS_DTYPE",
     "XrtkError", "MODE_F32R", "MODE_F64", "RAY_FIELDS",
 ]
 
@@ -473,6 +474,80 @@ def fresnel_propagate(ray_x, ray_z, ray_es, energy: float, dist: float, screen: 
                                      dptr(out), dptr(xp), dptr(zp), C.byref(o)),
           "xrtk_fresnel_propagate")
     return FresnelResult(out.view(np.complex128), xp, zp, int(screen.nx), int(screen.nz))
+
+
+M0C2 = 0.510998928  # src/core/consts.rs:33 [MeV]
+
+
+@dataclass
+class Undulator:
+    """The parameters of Undulator::new (src/sources/undulator.rs:64-92) that
+    the intensity map reads; gamma from SynchrotronParams::new
+    (bending_magnet.rs:38-53)."""
+    electron_energy_gev: float
+    beam_current: float
+    kx: float
+    ky: float
+    period_mm: float
+    n_periods: int
+    phase_deg: float = 0.0
+
+    @property
+    def gamma(self) -> float:
+        return self.electron_energy_gev * 1e3 / M0C2
+
+    def k_squared(self) -> float:
+        return self.kx * self.kx + self.ky * self.ky
+
+    def to_c(self, n_steps: int = 64) -> _lib.XrtkUndulator:
+        g = self.gamma
+        return _lib.XrtkUndulator(g, g * g, float(self.beam_current), float(self.kx),
+                                  float(self.ky), float(self.period_mm), float(self.phase_deg),
+                                  int(self.n_periods), int(n_steps))
+
+    def build_i_map(self, energies, thetas, psis, ctx: Optional[GpuContext] = None,
+                    n_steps: int = 64):
+        """build_i_map (src/sources/undulator.rs:120-244) on the device, FP64:
+        (intensity, amp_s, amp_p) as float64 / complex128 arrays."""
+        e = np.ascontiguousarray(np.asarray(energies, dtype=np.float64).reshape(-1))
+        t = np.ascontiguousarray(np.asarray(thetas, dtype=np.float64).reshape(-1))
+        p = np.ascontiguousarray(np.asarray(psis, dtype=np.float64).reshape(-1))
+        if not (e.shape == t.shape == p.shape):
+            raise ValueError("energies, thetas and psis differ in length")
+        ctx = ctx or GpuContext.shared()
+        if ctx is None:
+            raise XrtkError(_lib.ENODEV, "build_i_map")
+        n = e.shape[0]
+        inten, a_s, a_p = np.zeros(n), np.zeros(2 * n), np.zeros(2 * n)
+        u = self.to_c(n_steps)
+        check(lib.xrtk_undulator_i_map(ctx.handle, C.byref(u), dptr(e), dptr(t), dptr(p), n,
+                                       dptr(inten), dptr(a_s), dptr(a_p)), "xrtk_undulator_i_map")
+        return inten, a_s.view(np.complex128), a_p.view(np.complex128)
+
+
+@dataclass
+class UndulatorGpuResult:
+    """UndulatorGpuResult (src/gpu/undulator.rs:57-64)."""
+    amp_s: np.ndarray
+    amp_p: np.ndarray
+    intensity: np.ndarray
+
+
+def undulator_gpu(ctx: GpuContext, obs_points, kx, ky, period_mm, n_periods, gamma,
+                  beam_current, n_integration_steps=64, phase_deg=0.0) -> UndulatorGpuResult:
+    """undulator_gpu (src/gpu/undulator.rs:66-91): obs_points as a
+    GPU_OBS_DTYPE array (GpuObsPoint layout)."""
+    o = np.ascontiguousarray(obs_points)
+    if o.dtype != GPU_OBS_DTYPE:
+        raise TypeError("obs_points must be a GPU_OBS_DTYPE array (GpuObsPoint layout)")
+    n = o.shape[0]
+    out = np.zeros(5 * n)
+    check(lib.xrtk_undulator_gpu_f32(ctx.handle, o.ctypes.data, n, float(kx), float(ky),
+                                     float(period_mm), int(n_periods), float(gamma),
+                                     float(beam_current), int(n_integration_steps),
+                                     float(phase_deg), dptr(out)), "xrtk_undulator_gpu_f32")
+    r = out.reshape(-1, 5)
+    return UndulatorGpuResult(r[:, 0] + 1j * r[:, 1], r[:, 2] + 1j * r[:, 3], r[:, 4].copy())
 
 
 def now() -> float:

@@ -86,6 +86,12 @@ class XrtkFresnelScreen(C.Structure):
                 ("z_max", C.c_double), ("nx", C.c_uint32), ("nz", C.c_uint32)]
 
 
+class XrtkUndulator(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("gamma2", C.c_double), ("beam_current", C.c_double),
+                ("kx", C.c_double), ("ky", C.c_double), ("period_mm", C.c_double),
+                ("phase_deg", C.c_double), ("n_periods", C.c_uint32), ("n_steps", C.c_uint32)]
+
+
 class XrtkServerCfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int32),
                 ("quantum_ms", C.c_double), ("mode", C.c_int32), ("reserved", C.c_int32)]
@@ -104,6 +110,8 @@ GPU_RAY_DTYPE = np.dtype([(n, np.float32) for n in (
 GPU_PIXEL_DTYPE = np.dtype([(n, np.float32) for n in ("x", "y", "z", "_pad")])
 assert GPU_RAY_DTYPE.itemsize == C.sizeof(XrtkGpuRay) == 48
 assert GPU_PIXEL_DTYPE.itemsize == C.sizeof(XrtkGpuPixel) == 16
+GPU_OBS_DTYPE = np.dtype([(n, np.float32) for n in ("energy", "theta", "psi", "_pad")])
+assert GPU_OBS_DTYPE.itemsize == 16 and C.sizeof(XrtkUndulator) == 64
 
 # ---- prototypes --------------------------------------------------------------
 _p = C.c_void_p
@@ -134,6 +142,10 @@ PROTOTYPES = {
     "xrtk_auto_last_backend": (_i, []),
     "xrtk_autotune": (_i, [_p, _sz, _sz, _i, C.POINTER(XrtkOpts), _dp]),
     "xrtk_autotune_jobs": (_i, [_p, C.POINTER(XrtkJob), _sz, _i, C.POINTER(XrtkOpts), _dp]),
+    "xrtk_undulator_i_map": (_i, [_p, C.POINTER(XrtkUndulator), _dp, _dp, _dp, _sz, _dp, _dp,
+                                  _dp]),
+    "xrtk_undulator_gpu_f32": (_i, [_p, _p, _sz, C.c_double, C.c_double, C.c_double, _sz,
+                                    C.c_double, C.c_double, C.c_uint32, C.c_double, _dp]),
     "xrtk_fresnel_propagate": (_i, [_p, _dp, _dp, _dp, _dp, _sz, C.c_double, C.c_double,
                                     C.POINTER(XrtkFresnelScreen), _dp, _dp, _dp,
                                     C.POINTER(XrtkOpts)]),
