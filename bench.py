@@ -22,7 +22,9 @@ Further legs, reported in the same JSON line:
             workload, all threads and one thread;
   fresnel   fresnel_propagate of the same rays onto a 1024^2 screen (complex
             GEMM of FP64-phase factors), FP32 and FP64 factors, with the
-            oracle's loop on a pixel sample.
+            oracle's loop on a pixel sample;
+  undulator Undulator::build_i_map of a planar undulator on a 1024 x 1024
+            energy-angle grid (FP64), with the oracle on a point sample.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -331,6 +333,63 @@ def leg_fresnel(args, ctx, xrtk, W):
     return res
 
 
+# FP64 operations per point of the undulator map: per step the retarded
+# phase (7), the two transverse terms (2), four accumulations (12) and sin,
+# cos (1 each); plus ~30 for the N-period sum and the output scaling.
+UND_FLOP_STEP = 23
+UND_FLOP_TAIL = 30
+
+
+def leg_undulator(args, ctx, xrtk, W):
+    """Undulator::build_i_map of a planar undulator (3 GeV, 0.3 A, K_y = 2,
+    30 mm x 50 periods) on an energies x theta grid (psi = 0), 64 steps."""
+    import pyoracle
+    u = xrtk.Undulator(3.0, 0.3, 0.0, 2.0, 30.0, 50)
+    e1 = pyoracle.undulator_e1(u.gamma, u.kx, u.ky, u.period_mm)
+    ne, nt = args.und_energies, args.und_thetas
+    E, T = np.meshgrid(e1 * np.linspace(0.5, 9.5, ne), np.linspace(-1.5, 1.5, nt) / u.gamma)
+    e, th = E.ravel(), T.ravel()
+    ps = np.zeros_like(e)
+    n = e.shape[0]
+    u.build_i_map(e, th, ps, ctx=ctx)
+    wall, dev = [], []
+    for _ in range(args.und_steps):
+        t = time.perf_counter()
+        inten, amp_s, amp_p = u.build_i_map(e, th, ps, ctx=ctx)
+        wall.append(time.perf_counter() - t)
+        dev.append(ctx.last_timing()[1])
+    d_ms = float(np.median(dev))
+    flop = n * (64 * UND_FLOP_STEP + UND_FLOP_TAIL)
+    tf = flop / (d_ms * 1e-3) / 1e12
+    res = {"workload": f"undulator build_i_map, planar K_y = 2, 3 GeV, 50 x 30 mm, {ne} energies "
+                       f"x {nt} angles ({n} points), 64 steps",
+           "points": n, "value": n / float(np.median(wall)) / 1e6, "unit": "Mpoints/s",
+           "ms_per_map": 1e3 * float(np.median(wall)), "device_ms": d_ms,
+           "device_mpoints_per_s": n / (d_ms * 1e-3) / 1e6,
+           "note": "value includes the PCIe copies of 3 input and 5 output doubles per point",
+           "roofline": {"bound": "valu", "achieved": tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": tf / PEAK_FP64,
+                        "flop_per_point": 64 * UND_FLOP_STEP + UND_FLOP_TAIL}}
+    log(f"undulator: {res['ms_per_map']:.1f} ms/map (device {d_ms:.2f} ms), "
+        f"{res['value']:.0f} Mpoints/s, {tf:.1f} TFLOP/s FP64")
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        stride = max(1, n // args.und_cpu_points)
+        t = time.perf_counter()
+        oi, _, _ = pyoracle.undulator_i_map(pyoracle.undulator(3.0, 0.3, 0.0, 2.0, 30.0, 50),
+                                            e[::stride], th[::stride], ps[::stride],
+                                            threads=threads)
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": oi.shape[0] / dt / 1e6, "unit": "Mpoints/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{oi.shape[0]} points (every {stride}th), {dt:.2f} s"}
+        res["max_abs_err_vs_oracle_over_peak"] = float(
+            np.max(np.abs(inten[::stride] - oi)) / oi.max())
+        log(f"undulator cpu: {res['cpu_baseline']['value']:.3f} Mpoints/s; max |dI|/peak "
+            f"{res['max_abs_err_vs_oracle_over_peak']:.2e}")
+    return res
+
+
 def energy_scan_stream(rng, n_sets, base_energy, n_req, concurrent=16):
     """(set index, energy) requests from `concurrent` interleaved energy scans:
     each scan covers U{1..50} energies (5 eV steps around the set's energy)
@@ -450,6 +509,11 @@ def main():
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-fresnel", action="store_true")
+    ap.add_argument("--no-undulator", action="store_true")
+    ap.add_argument("--und-energies", type=int, default=1024)
+    ap.add_argument("--und-thetas", type=int, default=1024)
+    ap.add_argument("--und-steps", type=int, default=5)
+    ap.add_argument("--und-cpu-points", type=int, default=65536)
     ap.add_argument("--fresnel-side", type=int, default=1024)
     ap.add_argument("--fresnel-steps", type=int, default=4)
     ap.add_argument("--fresnel-cpu-pixels", type=int, default=64)
@@ -478,6 +542,7 @@ def main():
     del rays
 
     fresnel = None if args.no_fresnel else leg_fresnel(args, ctx, xrtk, W)
+    undulator = None if args.no_undulator else leg_undulator(args, ctx, xrtk, W)
 
     batched = serving = None
     if not args.no_batch:
@@ -516,6 +581,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "fresnel": fresnel,
+        "undulator": undulator,
     }
     print(json.dumps(line), flush=True)
 
