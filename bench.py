@@ -411,7 +411,7 @@ def energy_scan_stream(rng, n_sets, base_energy, n_req, concurrent=16):
 
 
 def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=None, load=None,
-                capacity=None):
+                capacity=None, policy="shortest"):
     """Poisson arrivals at `rate` req/s over `seconds`; or, given `load`, the
     requests whose pairs fill load x capacity x seconds, arriving as a
     Poisson stream whose rate makes the offered pair rate exactly
@@ -440,7 +440,8 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
             arrivals.append(t)
         reqs = reqs[:len(arrivals)]
     screens = [grid_of(xrtk, W.instance_screen(s)) for s in specs]
-    srv = xrtk.Server(ctx, hbm_budget_bytes=budget, max_active=64, quantum_ms=args.quantum_ms)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=budget, max_active=64, quantum_ms=args.quantum_ms,
+                      policy=policy)
     tickets = []
     try:
         t0 = xrtk.now()
@@ -464,6 +465,11 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
     total = sum(p for _, _, p in tickets)
     span = max(done) - t0
     lat_ms = np.array(lat) * 1e3
+    # slowdown: latency over the request's own time alone on the device at
+    # the measured batched rate (sizes span four decades, so the latency
+    # percentiles mostly follow the size mix)
+    alone_ms = np.array([p for _, _, p in tickets]) / capacity * 1e3
+    slow = lat_ms / alone_ms
     res = {"requests": len(tickets), "arrival_rate_per_s": rate, "window_s": seconds,
            "stream_mean_pairs_per_request": mean_pairs,
            "submit_span_s": t_sub, "drain_span_s": span,
@@ -471,13 +477,23 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
            "sustained_gpairs_per_s": total / span / 1e9,
            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
            "mean_ms": float(lat_ms.mean()), "max_ms": float(lat_ms.max()),
+           "alone_ms_p50": float(np.percentile(alone_ms, 50)),
+           "alone_ms_max": float(alone_ms.max()),
+           "slowdown_p50": float(np.percentile(slow, 50)),
+           "slowdown_p99": float(np.percentile(slow, 99)),
+           "largest_request_latency_ms": float(lat_ms[int(np.argmax(alone_ms))]),
+           "slowest_requests_alone_and_latency_ms": [
+               [round(float(alone_ms[k]), 1), round(float(lat_ms[k]), 1)]
+               for k in np.argsort(lat_ms)[-4:][::-1]],
            "admission_wait_p99_ms": float(np.percentile(np.array(queue) * 1e3, 99)),
            "max_concurrent": st.get("max_concurrent"), "launches": st.get("launches"),
            "pair_kernel_busy_frac": st.get("busy_ms", 0.0) * 1e-3 / span,
            "hbm_peak_gb": st.get("hbm_peak", 0) / 1e9,
-           "hbm_budget_gb": budget / 1e9, "quantum_ms": args.quantum_ms}
-    log(f"serving at {rate:.1f} req/s for {seconds:.1f}s: {len(tickets)} requests, "
-        f"p50 {res['p50_ms']:.0f} ms, p99 {res['p99_ms']:.0f} ms, sustained "
+           "hbm_budget_gb": budget / 1e9, "quantum_ms": args.quantum_ms, "policy": policy}
+    log(f"serving ({policy} first) at {rate:.1f} req/s for {seconds:.1f}s: {len(tickets)} requests, "
+        f"p50 {res['p50_ms']:.0f} ms, p99 {res['p99_ms']:.0f} ms (largest request: "
+        f"{res['largest_request_latency_ms']:.0f} ms, {res['alone_ms_max']:.0f} ms alone), "
+        f"slowdown p50 {res['slowdown_p50']:.1f} p99 {res['slowdown_p99']:.1f}, sustained "
         f"{res['sustained_gpairs_per_s']:.0f} Gpair/s (offered {res['offered_gpairs_per_s']:.0f})")
     return res
 
@@ -506,6 +522,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.5)
     ap.add_argument("--serve-load-seconds", type=float, default=15.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-serve-compare", action="store_true")
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-fresnel", action="store_true")
@@ -551,11 +568,19 @@ def main():
             budget = int(args.serve_budget_gb * 1e9)
             cap = batched["value"] * 1e9
             serving = {"stated": leg_serving(args, ctx, xrtk, W, sets, specs, args.serve_seconds,
-                                             7, budget, rate=args.serve_rate)}
+                                             7, budget, rate=args.serve_rate, capacity=cap)}
             serving["sustainable"] = leg_serving(args, ctx, xrtk, W, sets, specs,
                                                  args.serve_load_seconds, 8, budget,
                                                  load=args.serve_load, capacity=cap)
             serving["sustainable"]["load_fraction_of_batched_capacity"] = args.serve_load
+            if not args.no_serve_compare:
+                # the same request stream with requests taken in arrival order
+                s = leg_serving(args, ctx, xrtk, W, sets, specs, args.serve_load_seconds, 8,
+                                budget, load=args.serve_load, capacity=cap, policy="arrival")
+                serving["sustainable_arrival_order"] = {
+                    k: s[k] for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "slowdown_p50",
+                                      "slowdown_p99", "largest_request_latency_ms",
+                                      "sustained_gpairs_per_s", "policy")}
             over = serving["stated"]["offered_gpairs_per_s"] * 1e9 / cap
             serving["note"] = (f"the stated {args.serve_rate:.0f} req/s of config-3-sized requests "
                                f"offers {over:.1f}x the pair rate one GPU completes, so its "

@@ -323,8 +323,14 @@ typedef struct XrtkServerCfg {
     int32_t max_active;        /* concurrent requests on the device         */
     double quantum_ms;         /* target device time of one batched launch  */
     int32_t mode;              /* XRTK_MODE_*                               */
-    int32_t reserved;
+    int32_t policy;            /* XRTK_SCHED_*                              */
 } XrtkServerCfg;
+
+/* Which requests fill a launch.  Every active request with work left gets
+ * at least one work item in every launch under either policy; the results
+ * do not depend on it. */
+#define XRTK_SCHED_SHORTEST 0  /* fewest remaining pairs first (default)    */
+#define XRTK_SCHED_ARRIVAL  1  /* oldest request first                      */
 
 typedef struct XrtkServerStats {
     uint64_t submitted, admitted, completed, rejected;

@@ -1,5 +1,6 @@
 """Continuous-batching server: results equal the standalone solve bit for bit,
 alone and in a mix; requests overlap on the device; budget admission."""
+import ctypes
 import threading
 
 import numpy as np
@@ -49,12 +50,14 @@ def test_server_results_equal_standalone(ctx, sets):
         srv.close()
 
 
-def test_alone_and_in_mix_identical(ctx, sets):
+@pytest.mark.parametrize("policy", ["shortest", "arrival"])
+def test_alone_and_in_mix_identical(ctx, sets, policy):
     """The same request solved alone and while others arrive and run."""
     specs, rsets = sets
     jobs = jobs_of(specs, rsets)
     target = jobs[2]
-    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=1.0)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=1.0,
+                      policy=policy)
     try:
         alone, _ = srv.wait(srv.submit(target), timeout_ms=60000)
         tickets = []
@@ -135,3 +138,37 @@ def test_release_of_busy_rayset_refused(ctx, sets):
     finally:
         srv.close()
     rs.release()
+
+
+def test_shortest_first_lets_small_requests_pass(ctx, sets):
+    """Shortest-remaining-first: small requests that arrive while a large
+    one runs finish before it; every result still equals the standalone
+    solve."""
+    specs, rsets = sets
+    big_spec = {"n_rays": 1 << 20, "side": 512, "energy": 10000.0, "distance": 1000.0,
+                "seed": 77, "half": 0.2 * 0.1 * (1000.0 + W.SRC_DIST) / W.SRC_DIST}
+    big_rs = xrtk.RaySet(ctx, W.instance_rays(big_spec))
+    try:
+        big = xrtk.Job(big_rs, big_spec["energy"], grid(W.instance_screen(big_spec)))
+        small = sorted(jobs_of(specs, rsets), key=lambda j: j.rays.n * j.screen.nx * j.screen.nz)[:4]
+        ref = [xrtk.solve_jobs(ctx, [j])[0] for j in small]
+        srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, quantum_ms=2.0, policy="shortest")
+        try:
+            tb = srv.submit(big)
+            ts = [srv.submit(j) for j in small]
+            outs = [srv.wait(t, timeout_ms=60000) for t in ts]
+            _, tbig = srv.wait(tb, timeout_ms=60000)
+        finally:
+            srv.close()
+        for (got, t), r in zip(outs, ref):
+            assert np.array_equal(got, r)
+            assert t[3] < tbig[3]
+    finally:
+        big_rs.release()
+
+
+def test_invalid_policy_rejected(ctx):
+    cfg = xrtk._lib.XrtkServerCfg(1 << 30, 4, 2.0, 0, 7)
+    h = ctypes.c_void_p()
+    rc = xrtk._lib.lib.xrtk_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == xrtk._lib.EINVAL and not h.value

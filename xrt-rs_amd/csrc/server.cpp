@@ -7,13 +7,20 @@
 //      cap allow (their buffers come from a caching pool; their grid and
 //      prep kernels are enqueued at once),
 //   2. fills one "quantum" launch of the persistent pair kernel with work
-//      items taken oldest-request-first from every active request, so a
-//      request that arrives while others run joins at the next quantum and
-//      shares the device with them,
+//      items of the active requests, shortest remaining work first (the
+//      request with the fewest pairs left fills the budget first; or oldest
+//      first, XRTK_SCHED_ARRIVAL), and then one item of every active request
+//      not yet in the launch, so a request that arrives while others run
+//      joins at the next quantum, small requests are not stuck behind large
+//      ones and every request advances in every quantum.  The quantum is a
+//      whole number of rounds of full-size items over the resident
+//      workgroups, and its items run longest first, so the workgroups end
+//      the launch together,
 //   3. enqueues, for each request whose last items went into that launch,
 //      its FP64 merge and an async copy of the result into pinned memory,
 //   4. retires requests whose copy has completed (result handed to the
-//      caller's buffer, timestamps recorded, waiters woken).
+//      caller's buffer, timestamps recorded, waiters woken); it also does so,
+//      and admits, while it polls for the older launch to end.
 // At most two quanta are in flight, so the host prepares the next launch
 // while the device runs the current one.  A request's plan (chunking, slot
 // layout, kernel shape) depends only on the request, so its result does not
@@ -21,6 +28,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -309,34 +317,83 @@ struct xrtk_server {
     if (e != hipSuccess) r->rc = map_err(e);
   }
 
+  // Resident workgroups of one persistent pair-kernel launch.
+  uint32_t grid_res(const LaunchCfg& c, uint32_t mode) const {
+    return std::max<uint32_t>(1, (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / c.W);
+  }
+
   // Fills and enqueues one quantum.  Returns false when there was no work.
   bool launch_quantum() {
     Launch& L = launches[cur];  // the caller has retired it
-    const double budget = std::max(1.0, cfg.quantum_ms) * 1e-3 * rate_pairs_per_s;
+    double budget = std::max(1.0, cfg.quantum_ms) * 1e-3 * rate_pairs_per_s;
+    if (c32.tile > 0) {
+      // whole rounds of full-size items over the persistent workgroups, so
+      // that they end the launch together
+      const double round_pairs = (double)grid_res(c32, MODE_F32R) * 64.0 * c32.W * c32.P * c32.tile;
+      budget = std::max(1.0, std::round(budget / round_pairs)) * round_pairs;
+    }
     double pairs = 0;
     L.h_insts.clear();
     L.h_items32.clear();
     L.h_items64.clear();
     std::vector<Req*> finishing;
-    for (Req* r : active) {
-      if (r->state != ACTIVE || r->next_item >= r->n_items) continue;
-      if (pairs >= budget && !L.h_insts.empty()) break;
+    // requests with work left, fewest remaining pairs first
+    std::vector<Req*> order;
+    for (Req* r : active)
+      if (r->state == ACTIVE && r->next_item < r->n_items) order.push_back(r);
+    auto left = [](const Req* r) { return (double)(r->n_items - r->next_item) * r->item_pairs; };
+    if (cfg.policy == XRTK_SCHED_SHORTEST)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](const Req* a, const Req* b) { return left(a) < left(b); });
+    // where each request stood, to give its items back if the launch fails
+    struct Undo { Req* r; uint64_t next_item; double t2; };
+    std::vector<Undo> undo;
+    // items with their pair counts, ordered longest first below
+    std::vector<std::pair<double, WorkItem>> it32, it64;
+    // take up to `max_items` items of r (0: until `limit` pairs are in)
+    auto take = [&](Req* r, uint64_t max_items, double limit) {
+      undo.push_back(Undo{r, r->next_item, r->t[2]});
       const uint32_t li = (uint32_t)L.h_insts.size();
       L.h_insts.push_back(r->kinst);
       const uint32_t B = 64u * r->cfg.W * r->cfg.P;
-      auto& items = r->spec.mode == MODE_F32R ? L.h_items32 : L.h_items64;
-      while (r->next_item < r->n_items && (pairs < budget || items.empty())) {
+      auto& items = r->spec.mode == MODE_F32R ? it32 : it64;
+      uint64_t taken = 0;
+      while (r->next_item < r->n_items &&
+             (max_items ? taken < max_items : (pairs < limit || taken == 0))) {
         const uint64_t it = r->next_item++;
         const uint32_t chunk = (uint32_t)(it / r->pblocks);
         const uint32_t pb = (uint32_t)(it % r->pblocks);
-        items.push_back(WorkItem{li, pb * B, chunk, 0});
-        const double len = std::min<double>(r->chunk_len, (double)r->spec.nr - (double)chunk * r->chunk_len);
+        const double len =
+            std::min<double>(r->chunk_len, (double)r->spec.nr - (double)chunk * r->chunk_len);
+        items.emplace_back((double)B * len, WorkItem{li, pb * B, chunk, 0});
         pairs += (double)B * len;
+        ++taken;
       }
       if (r->t[2] == 0) r->t[2] = now_s();
       if (r->next_item >= r->n_items) finishing.push_back(r);
-    }
+    };
+    // reserve[k]: one item of each of order[k..], the progress share
+    std::vector<double> reserve(order.size() + 1, 0.0);
+    for (size_t k = order.size(); k-- > 0;)
+      reserve[k] = reserve[k + 1] + std::min(left(order[k]), order[k]->item_pairs);
+    size_t filled = 0;
+    for (; filled < order.size() && (pairs + reserve[filled] < budget || filled == 0); ++filled)
+      take(order[filled], 0, budget - reserve[filled + 1]);
+    // progress for the rest: one item each
+    for (size_t k = filled; k < order.size(); ++k) take(order[k], 1, 0.0);
     if (L.h_insts.empty()) return false;
+    // The workgroups of the persistent kernel pull items in list order:
+    // longest first leaves the short items to even out the end of the
+    // launch.  Each item writes its own partial, so the order does not
+    // change any result.
+    auto lpt = [](std::vector<std::pair<double, WorkItem>>& v, std::vector<WorkItem>& out) {
+      std::stable_sort(v.begin(), v.end(),
+                       [](const auto& a, const auto& b) { return a.first > b.first; });
+      out.reserve(v.size());
+      for (const auto& x : v) out.push_back(x.second);
+    };
+    lpt(it32, L.h_items32);
+    lpt(it64, L.h_items64);
     // metadata
     const size_t a_ins = au(sizeof(KInst) * L.h_insts.size());
     const size_t a_32 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items32.size()));
@@ -352,7 +409,10 @@ struct xrtk_server {
         (void)hipGetLastError();
         L.meta_size = 0;
         // give the items back; the next iteration retries
-        for (Req* r : active) (void)r;
+        for (const Undo& u : undo) {
+          u.r->next_item = u.next_item;
+          u.r->t[2] = u.t2;
+        }
         return false;
       }
     }
@@ -368,8 +428,7 @@ struct xrtk_server {
     hipError_t e = hipMemcpyAsync(M, H, a_ins + a_32 + a_64, hipMemcpyHostToDevice, stream);
     (void)hipEventRecord(L.start, stream);
     auto grid_of = [&](const LaunchCfg& c, uint32_t mode, size_t n) {
-      const uint32_t res = (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / c.W;
-      return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n, res));
+      return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n, grid_res(c, mode)));
     };
     if (e == hipSuccess && !L.h_items32.empty())
       e = launch_pair(MODE_F32R, c32, d_ins, d_32, (uint32_t)L.h_items32.size(), d_q,
@@ -447,9 +506,16 @@ struct xrtk_server {
       admit();
       Launch& L = launches[cur];
       if (L.in_flight) {  // keep at most two quanta in flight
-        lk.unlock();
-        (void)hipEventSynchronize(L.end);
-        lk.lock();
+        // wait for the older launch, handing back merged requests and
+        // admitting new ones meanwhile (a merge is done well before the
+        // launch after it)
+        while (hipEventQuery(L.end) == hipErrorNotReady) {
+          reap();
+          admit();
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(50));
+          lk.lock();
+        }
         retire_launch(L);
       }
       const bool launched = launch_quantum();
@@ -484,6 +550,7 @@ int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** ou
   *out = nullptr;
   if (!ctx) return XRTK_ENODEV;
   if (!cfg || cfg->hbm_budget_bytes == 0 || cfg->max_active <= 0) return XRTK_EINVAL;
+  if (cfg->policy != XRTK_SCHED_SHORTEST && cfg->policy != XRTK_SCHED_ARRIVAL) return XRTK_EINVAL;
   try {
     std::unique_ptr<xrtk_server> s(new xrtk_server());
     s->ctx = ctx;

@@ -372,9 +372,10 @@ class Server:
     """Continuous-batching solver (xrtk_server_*)."""
 
     def __init__(self, ctx: GpuContext, hbm_budget_bytes: int, max_active: int = 64,
-                 quantum_ms: float = 10.0, mode="f32r"):
+                 quantum_ms: float = 10.0, mode="f32r", policy="shortest"):
+        pol = {"shortest": 0, "arrival": 1}[policy]
         cfg = _lib.XrtkServerCfg(int(hbm_budget_bytes), int(max_active), float(quantum_ms),
-                                 _mode(mode), 0)
+                                 _mode(mode), pol)
         h = C.c_void_p()
         check(lib.xrtk_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "xrtk_server_create")
         self._h = h

@@ -94,7 +94,7 @@ class XrtkUndulator(C.Structure):
 
 class XrtkServerCfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int32),
-                ("quantum_ms", C.c_double), ("mode", C.c_int32), ("reserved", C.c_int32)]
+                ("quantum_ms", C.c_double), ("mode", C.c_int32), ("policy", C.c_int32)]
 
 
 class XrtkServerStats(C.Structure):
