@@ -99,6 +99,8 @@ def traffic_record(nr, npx, plan):
     if any(w.get("plan", {}).get(k) != plan.get(k)
            for k in ("pixels_per_lane", "waves_per_group", "tile_rays", "persistent")):
         return None
+    if t.get("variant_flags") != plan.get("inst_flags"):
+        return None
     return t
 
 

@@ -290,6 +290,7 @@ typedef struct XrtkPlan {
 #define XRTK_VAR_NOFRACT   16  /* reduced phase inside v_sin's direct domain     */
 #define XRTK_VAR_LIN2      32  /* o is zero along one axis                       */
 #define XRTK_VAR_PAX       64  /* paraxial 1/path and 1/(path + R)               */
+#define XRTK_VAR_SEPAMP    128 /* amplitude = ray factor x pixel factor          */
 int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);
 /* Backend used by the last xrtk_kirchhoff_auto_f32 call of this process:
  * 1 = device, 0 = CPU fallback, -1 = none yet. */

@@ -327,7 +327,7 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
     assert np.array_equal(lin2, full)
 
 
-VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX = 2, 16, 32, 64
+VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP = 2, 16, 32, 64, 128
 
 
 @pytest.mark.parametrize("distance", [100.0, 1000.0, 30000.0])
@@ -359,6 +359,40 @@ def test_gaussian_case_variant(ctx):
     xrtk.diffraction_integral(rays, grid(W.screen(1000.0, 0.02, 128)), ctx=ctx)
     f = ctx.last_plan()["inst_flags"]
     assert f & VAR_TILE and f & VAR_NOFRACT and f & VAR_LIN2 and f & VAR_PAX, hex(f)
+    assert f & VAR_SEPAMP, hex(f)
+
+
+@pytest.mark.parametrize("case", ["gaussian", "L100", "L1000", "L10000", "L30000"])
+def test_separable_amplitude_matches_exact_form(ctx, monkeypatch, case):
+    """The amplitude as a ray factor times a slot factor (no per-pair
+    amplitude) against the per-pair form, both against the long-double
+    oracle.  The planner allows it when its bound on the relative amplitude
+    error is <= 1e-6: not at L = 100 mm, where the rays' paths to one pixel
+    differ by ~1e-6 of their length."""
+    if case == "gaussian":
+        rays, g = W.gaussian_source(65536, seed=2), grid(W.screen(1000.0, 0.02, 96))
+    else:
+        rays, scr, _ = W.stress_cell(20000.0, float(case[1:]), n_rays=32768, side=96)
+        g = grid(scr)
+    sep = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    flags = ctx.last_plan()["inst_flags"]
+    monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x1000")
+    per_pair = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    assert not ctx.last_plan()["inst_flags"] & VAR_SEPAMP
+    monkeypatch.delenv("XRTK_FORCE_FLAGS")
+    between = rel_l2(sep, per_pair)
+    print(f"{case}: flags {flags:#x}, separable vs per-pair {between:.2e}")
+    if case == "L100":
+        assert not flags & VAR_SEPAMP and between == 0.0
+        return
+    assert flags & VAR_SEPAMP and flags & VAR_PAX
+    px, py, pz = xrtk.prepare_wave(g)
+    stride = 37
+    ld = O.diffraction_ld(rays, px, py, pz, stride=stride)
+    e_sep, e_pp = rel_l2(sep[::stride], ld), rel_l2(per_pair[::stride], ld)
+    print(f"   vs ld: separable {e_sep:.2e}, per-pair {e_pp:.2e}")
+    assert between < 2e-6
+    assert e_sep < STRESS_BAR and e_sep < e_pp + 2e-6
 
 
 def _transform(rays, pix, M):

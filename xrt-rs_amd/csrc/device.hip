@@ -307,6 +307,10 @@ __global__ __launch_bounds__(256) void k_centroid_final(
     for (int k = 0; k < 4; ++k) scr->omax_bits[k] = 0u;
     for (int k = 0; k < 3; ++k) scr->qmax_bits[k] = scr->dqmax_bits[k] = 0u;
     for (int k = 0; k < 3; ++k) { scr->qlo_key[k] = 0xffffffffu; scr->qhi_key[k] = 0u; }
+    scr->nl_lo_key = scr->c_lo_key = 0xffffffffu;
+    scr->nl_hi_key = scr->c_hi_key = 0u;
+    scr->nu_max_bits = 0u;
+    scr->pad2 = 0u;
     scr->rmin_bits = 0x7f800000u;  // +inf
     scr->rmax_bits = 0u;
     scr->flags = 0u;
@@ -320,23 +324,30 @@ __global__ __launch_bounds__(256) void k_centroid_final(
 }
 
 // Per-axis max |o| (o rounded to FP32 as the pair kernel sees it), max |o|,
-// whether any Ep != 0 and whether any normal differs from ray 0's.
+// whether any Ep != 0 and whether any normal differs from ray 0's; the range
+// of nl and max |n0.o| (rounded outwards) for the separable amplitude.
 __global__ __launch_bounds__(256) void k_ray_bounds(
     const RayRec64* __restrict__ rays, uint32_t n, InstScratch* __restrict__ scr) {
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
   const double n0x = scr->n0x, n0y = scr->n0y, n0z = scr->n0z;
   float mx = 0.f, my = 0.f, mz = 0.f, mo = 0.f;
+  float nlo = __uint_as_float(0x7f800000u), nhi = -__uint_as_float(0x7f800000u), nu = 0.f;
   uint32_t ep = 0u, nd = 0u;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const RayRec64 r = rays[i];
-    const float ox = (float)(r.x - cx), oy = (float)(r.y - cy), oz = (float)(r.z - cz);
+    const double oe[3] = {r.x - cx, r.y - cy, r.z - cz};
+    const float ox = (float)oe[0], oy = (float)oe[1], oz = (float)oe[2];
     mx = fmaxf(mx, fabsf(ox)); my = fmaxf(my, fabsf(oy)); mz = fmaxf(mz, fabsf(oz));
     const double o2 = (double)ox * ox + (double)oy * oy + (double)oz * oz;
     mo = fmaxf(mo, __double2float_ru(sqrt(o2)));
     ep |= (r.ep_re != 0.0 || r.ep_im != 0.0) ? 1u : 0u;
     nd |= (r.nx != n0x || r.ny != n0y || r.nz != n0z) ? 1u : 0u;
+    nlo = fminf(nlo, __double2float_rd(r.nl));
+    nhi = fmaxf(nhi, __double2float_ru(r.nl));
+    nu = fmaxf(nu, __double2float_ru(fabs(n0x * oe[0] + n0y * oe[1] + n0z * oe[2])));
   }
   mx = wave_max(mx); my = wave_max(my); mz = wave_max(mz); mo = wave_max(mo);
+  nlo = wave_min(nlo); nhi = wave_max(nhi); nu = wave_max(nu);
   const bool epw = __any(ep), ndw = __any(nd);
   if ((threadIdx.x & 63) == 0) {
     atomicMax(&scr->omax_bits[0], __float_as_uint(mx));
@@ -345,7 +356,16 @@ __global__ __launch_bounds__(256) void k_ray_bounds(
     atomicMax(&scr->omax_bits[3], __float_as_uint(mo));
     if (epw) atomicOr(&scr->ep_nonzero, 1u);
     if (ndw) atomicOr(&scr->n_differs, 1u);
+    atomicMin(&scr->nl_lo_key, fkey(nlo));
+    atomicMax(&scr->nl_hi_key, fkey(nhi));
+    atomicMax(&scr->nu_max_bits, __float_as_uint(nu));
   }
+}
+
+// Reference value of the rays' nl (midpoint of its range; 0 without rays).
+__device__ __forceinline__ double nl_ref(const InstScratch* scr) {
+  if (scr->nl_lo_key > scr->nl_hi_key) return 0.0;
+  return 0.5 * ((double)fkey_inv(scr->nl_lo_key) + (double)fkey_inv(scr->nl_hi_key));
 }
 
 // One wave per tile of tsz = 64 * P slots (P slots per lane, slot = t*tsz +
@@ -390,6 +410,8 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
   if (lane == 0) tiles[blockIdx.x] = TileRec{pick3(t, ax[0]), pick3(t, ax[1]), pick3(t, ax[2]), Rt};
   float aq[3] = {0.f, 0.f, 0.f}, ad[3] = {0.f, 0.f, 0.f};
   float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
+  float clo = __uint_as_float(0x7f800000u), chi = -__uint_as_float(0x7f800000u);
+  const double nlb = nl_ref(scr);
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     double Rh, Rl;
@@ -409,7 +431,14 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     r.c4 = (float)(0.125 * ri3);
     r.rinv = (float)ri;
     r.c2 = (float)(0.5 * ri3);
+    // separable amplitude: slot factor (nl_ref + C) / |q|, C = n0.q / |q|
     r.pad = 0.f;
+    if (uni && Rh > 0.0) {
+      const double C = (n0x * qh[i][0] + n0y * qh[i][1] + n0z * qh[i][2]) * ri;
+      r.pad = (float)((nlb + C) * ri);
+      clo = fminf(clo, __double2float_rd(C));
+      chi = fmaxf(chi, __double2float_ru(C));
+    }
     const uint32_t s = base + i * 64u + lane;
     out32[s] = r;
     R64[2 * s] = Rh;
@@ -426,7 +455,10 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     ad[k] = wave_max(ad[k]);
   }
   rmin = wave_min(rmin); rmax = wave_max(rmax);
+  clo = wave_min(clo); chi = wave_max(chi);
   if (lane == 0) {
+    atomicMin(&scr->c_lo_key, fkey(clo));
+    atomicMax(&scr->c_hi_key, fkey(chi));
     for (int k = 0; k < 3; ++k) {
       atomicMax(&scr->qmax_bits[k], __float_as_uint(aq[k]));
       atomicMax(&scr->dqmax_bits[k], __float_as_uint(ad[k]));
@@ -469,8 +501,17 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //          LIN2 when o is zero along one axis (dropped from o.m).
 //          PAX (tile variant) when the paraxial expansions of 1/path and
 //          1/(path + R) are exact to FP32.
+//          SEPAMP (PAX, uniform normal) when the amplitude
+//            am (nl + n.d / path) / path
+//          is a ray factor am (1 + (nl - nl_ref - n.o / R_ref) / (nl_ref + C_ref))
+//          times a slot factor (nl_ref + C) / |q| to a relative 1e-6, C =
+//          n.q / |q|.  With path = |q| (1 + e), |e| <= rho, the difference
+//          of the two, times |q| / am, is
+//            nl (1/(1+e) - 1) + (C - n.o/|q|) (1/(1+e)^2 - 1)
+//            - n.o (1/|q| - 1/R_ref) - (nl - nl_ref - n.o/R_ref) (C - C_ref) / (nl_ref + C_ref),
+//          bounded term by term from the ranges of nl, C, |q| and max |n.o|.
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
-// 0x400 forbids LIN2, 0x800 forbids PAX.
+// 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -503,6 +544,21 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     if (tile && rmin > 0.0) {
       const double rho = (om * om + 2.0 * (ox * qx + oy * qy + oz * qz)) / (rmin * rmin);
       if (rho <= 1e-4 && !(force_flags & 0x800u)) f |= INST_F_PAX;
+      const bool keys = scr->nl_lo_key <= scr->nl_hi_key && scr->c_lo_key <= scr->c_hi_key;
+      if ((f & INST_F_PAX) && (f & INST_F_UNIFORM_N) && keys && !(force_flags & 0x1000u)) {
+        const double nlo = fkey_inv(scr->nl_lo_key), nhi = fkey_inv(scr->nl_hi_key);
+        const double clo = fkey_inv(scr->c_lo_key), chi = fkey_inv(scr->c_hi_key);
+        const double nu = __uint_as_float(scr->nu_max_bits);
+        const double rmax = __uint_as_float(scr->rmax_bits);
+        const double nlb = 0.5 * (nlo + nhi), cb = 0.5 * (clo + chi), rb = 0.5 * (rmin + rmax);
+        const double e1 = rho / (1.0 - rho), e2 = rho * (2.0 + rho) / ((1.0 - rho) * (1.0 - rho));
+        const double nla = fmax(fabs(nlo), fabs(nhi)), ca = fmax(fabs(clo), fabs(chi));
+        const double num = nla * e1 + (ca + nu / rmin) * e2 +
+                           nu * fmax(1.0 / rmin - 1.0 / rb, 1.0 / rb - 1.0 / rmax) +
+                           (0.5 * (nhi - nlo) + nu / rb) * 0.5 * (chi - clo) / (nlb + cb);
+        const double den = (nlo + clo - nu / rmin) - nla * e1 - (ca + nu / rmin) * e2;
+        if (nlb + cb > 0.0 && den > 0.0 && num <= 1e-6 * den) f |= INST_F_SEPAMP;
+      }
     }
   }
   if (force_flags & 0x200u) f |= INST_F_CHECK;
@@ -583,6 +639,18 @@ __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__
   o.ny = (float)(pick3(n3, ax[1]) * nf);
   o.nz = (float)(pick3(n3, ax[2]) * nf);
   o.nl = (float)(am * r.nl);
+  if (flags & INST_F_SEPAMP) {
+    // the ray factor of the separable amplitude joins the polarisation terms
+    const double nlb = nl_ref(scr);
+    const double cb = 0.5 * ((double)fkey_inv(scr->c_lo_key) + (double)fkey_inv(scr->c_hi_key));
+    const double rb = 0.5 * ((double)__uint_as_float(scr->rmin_bits) +
+                             (double)__uint_as_float(scr->rmax_bits));
+    const double nu = scr->n0x * oe[0] + scr->n0y * oe[1] + scr->n0z * oe[2];
+    const double A = am * (1.0 + (r.nl - nlb - nu / rb) / (nlb + cb));
+    cs *= A;
+    qr *= A;
+    qi *= A;
+  }
   o.cs = (float)cs;
   o.pr = (float)qr;
   o.pi = (float)qi;
@@ -609,7 +677,7 @@ static constexpr uint32_t kBlockRays = 64;  // rays staged per LDS block (one pe
 template <int P>
 struct SlotRegs {
   float mx[P], my[P], mz[P], R2[P], R[P], g[P], nq[P];
-  float r0[P], c4[P], rinv[P], c2[P];
+  float r0[P], c4[P], rinv[P], c2[P], sep[P];
 };
 
 // Accumulates one ray into a[P][4] for the P slots of this lane.
@@ -619,9 +687,12 @@ struct SlotRegs {
 //   LIN2:  o is zero along the last kernel axis.
 //   PAX:   (tile variant) 1/path and 1/(path + R) from the paraxial
 //          expansion: one FMA each instead of rsq and rcp.
+//   SEP:   (PAX variant) the amplitude is the ray factor already in the
+//          ray's polarisation terms times the slot factor applied to the
+//          sums afterwards: no per-pair amplitude.
 // Per pair (TILE, UNIN, NOFRACT, LIN2, PAX): 21 FP32 add/mul/FMA + sin, cos;
-// without PAX: 20 + rsq, rcp, sin, cos.
-template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX>
+// with SEP: 14 + sin, cos; without PAX: 20 + rsq, rcp, sin, cos.
+template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP>
 __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, float nt,
                                          const SlotRegs<P>& s, float ikr, float h,
                                          float (&a)[P][4]) {
@@ -661,12 +732,16 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
     } else {
       nd = fmaf(r.nz, s.mz[i], fmaf(r.ny, s.my[i], fmaf(r.nx, s.mx[i], TILE ? nt : mno)));
     }
-    const float obl = fmaf(nd, inv, r.nl);     // am (nl + n.d / path)
     const float f = NOFRACT ? rev : __builtin_amdgcn_fractf(rev);
     const float sn = __builtin_amdgcn_sinf(f);
     const float cs = __builtin_amdgcn_cosf(f);
-    const float amp = obl * inv;
-    const float wr = amp * cs, wi = amp * sn;
+    float wr = cs, wi = sn;
+    if constexpr (!SEP) {
+      const float obl = fmaf(nd, inv, r.nl);   // am (nl + n.d / path)
+      const float amp = obl * inv;
+      wr = amp * cs;
+      wi = amp * sn;
+    }
     a[i][0] = fmaf(r.cs, wr, a[i][0]);
     a[i][1] = fmaf(r.cs, wi, a[i][1]);
     a[i][2] = fmaf(r.pr, wr, a[i][2]);
@@ -676,7 +751,7 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
   }
 }
 
-template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX>
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   const uint32_t npad = I.npad;
@@ -699,6 +774,7 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     s.mz[i] = TILE ? pr.mz : pr.mz + mtz;
     s.R2[i] = pr.R2; s.R[i] = pr.R; s.g[i] = pr.g; s.nq[i] = pr.nq;
     s.r0[i] = pr.r0; s.c4[i] = pr.c4; s.rinv[i] = pr.rinv; s.c2[i] = pr.c2;
+    s.sep[i] = pr.pad;
   }
   // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
   // partials are summed in FP64 by k_merge.
@@ -742,42 +818,47 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
       for (uint32_t jj = 0; jj < nb; ++jj) {
         const RayRec32 r = rays[b0 + jj];
         const TileRay tr = my[jj];
-        pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+        pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
       }
     }
   } else {
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
-      pair_ray<P, false, UNIN, NOFRACT, LIN2, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
+      pair_ray<P, false, UNIN, NOFRACT, LIN2, false, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
-    part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
+    const float m = SEP ? s.sep[i] : 1.f;  // the slot factor of the amplitude
+    part[p] = make_float4(a[i][0] * m, a[i][1] * m, a[i][2] * m, a[i][3] * m);
   }
 }
 
 // Runs one work item with the variant the instance's flags select (PAX only
-// exists in the tile-referenced form).
+// exists in the tile-referenced form, SEPAMP only with PAX and a uniform
+// normal).
 template <int P, int W>
 __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const WorkItem& w,
                                               TileRay* lds, uint32_t lane, uint32_t wave) {
   const bool tile = (f & INST_F_TILE) != 0u;
+  const bool pax = tile && (f & INST_F_PAX);
+  const bool sep = pax && (f & INST_F_UNIFORM_N) && (f & INST_F_SEPAMP);
   const uint32_t v = (tile ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
                      ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u) |
-                     ((tile && (f & INST_F_PAX)) ? 16u : 0u);
+                     (pax ? 16u : 0u) | (sep ? 32u : 0u);
   switch (v) {
 #define XRTK_V(V_)                                                                      \
   case V_:                                                                             \
     f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0,         \
-              (V_ & 16) != 0>(I, w, lds, lane, wave);                                  \
+              (V_ & 16) != 0, (V_ & 32) != 0>(I, w, lds, lane, wave);                  \
     break;
     XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
     XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
     XRTK_V(24) XRTK_V(25) XRTK_V(26) XRTK_V(27) XRTK_V(28) XRTK_V(29) XRTK_V(30) XRTK_V(31)
+    XRTK_V(60) XRTK_V(61) XRTK_V(62) XRTK_V(63)
 #undef XRTK_V
   }
 }

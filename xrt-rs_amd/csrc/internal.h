@@ -107,6 +107,12 @@ struct InstScratch {
   double v0_rev, v2_rev;   // bounds of the reduced phase of the two variants
   uint32_t qlo_key[3];     // min q_x, q_y, q_z (order-preserving float keys)
   uint32_t qhi_key[3];     // max q_x, q_y, q_z
+  // separable amplitude (uniform normal n0): ranges of the rays' nl, of
+  // |n0.o| and of the slots' C = n0.q / |q|
+  uint32_t nl_lo_key, nl_hi_key;
+  uint32_t nu_max_bits;
+  uint32_t c_lo_key, c_hi_key;
+  uint32_t pad2;
 };
 
 enum : uint32_t {
@@ -118,6 +124,8 @@ enum : uint32_t {
   INST_F_LIN2 = 32u,      // F32R: o is zero along one axis (the last kernel axis)
   INST_F_PAX = 64u,       // F32R tile variant: 1/path and 1/(path + R) from the
                           // first-order paraxial expansion (rho <= 1e-4)
+  INST_F_SEPAMP = 128u,   // F32R PAX variant: the amplitude as a ray factor times a
+                          // slot factor (bound <= 1e-6)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };

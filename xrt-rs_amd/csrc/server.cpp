@@ -142,10 +142,18 @@ struct Launch {
 
 }  // namespace
 
+// Nominal pair rates of one resident wave of the pair kernels on gfx950
+// (32 waves per CU for F32R, 16 for FP64); they size the server's work items.
+static constexpr double kWavePairsF32 = 2.9e8, kWavePairsF64 = 3.2e7;
+
 struct xrtk_server {
   xrtk_ctx* ctx = nullptr;
   XrtkServerCfg cfg{};
   LaunchCfg c32, c64;
+  // Longest ray chunk of a work item (0: the configuration's own), so that
+  // one item takes at most half a quantum on one wave.  It depends only on
+  // the configuration and quantum_ms, never on the other requests.
+  uint32_t cap32 = 0, cap64 = 0;
   hipStream_t stream = nullptr;
   std::thread th;
   std::mutex mu;
@@ -194,6 +202,8 @@ struct xrtk_server {
     s.static_flags = rs.has_path ? INST_F_PATHUP : 0u;
     r->cfg = s.mode == MODE_F32R ? c32 : c64;
     r->chunk_len = chunk_len_for(ctx, r->cfg, s.nr, s.npad, 0);
+    const uint32_t cap = s.mode == MODE_F32R ? cap32 : cap64;
+    if (cap && r->chunk_len > cap) r->chunk_len = cap;
     r->n_chunks = s.nr ? (s.nr + r->chunk_len - 1) / r->chunk_len : 0;
     const uint32_t B = 64u * r->cfg.W * r->cfg.P;
     r->pblocks = s.npad / B;
@@ -329,7 +339,8 @@ struct xrtk_server {
     if (c32.tile > 0) {
       // whole rounds of full-size items over the persistent workgroups, so
       // that they end the launch together
-      const double round_pairs = (double)grid_res(c32, MODE_F32R) * 64.0 * c32.W * c32.P * c32.tile;
+      const double rays = cap32 ? std::min<double>(cap32, c32.tile) : c32.tile;
+      const double round_pairs = (double)grid_res(c32, MODE_F32R) * 64.0 * c32.W * c32.P * rays;
       budget = std::max(1.0, std::round(budget / round_pairs)) * round_pairs;
     }
     double pairs = 0;
@@ -561,6 +572,13 @@ int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** ou
       s->c32 = batch_cfg(ctx, MODE_F32R, nullptr);
       s->c64 = batch_cfg(ctx, MODE_F64, nullptr);
     }
+    auto cap = [&](const LaunchCfg& c, double wave_rate) -> uint32_t {
+      const double rays = 0.5e-3 * s->cfg.quantum_ms * wave_rate / (64.0 * c.P);
+      const uint32_t r = std::max<uint32_t>(512u, (uint32_t)std::min(rays, 1e9) / 64u * 64u);
+      return c.tile > 0 && r >= (uint32_t)c.tile ? 0u : r;
+    };
+    s->cap32 = cap(s->c32, kWavePairsF32);
+    s->cap64 = cap(s->c64, kWavePairsF64);
     s->hpool.pinned = true;
     int dev = -1;
     (void)hipGetDevice(&dev);
