@@ -327,7 +327,7 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
     assert np.array_equal(lin2, full)
 
 
-VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP = 2, 16, 32, 64, 128
+VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP, VAR_LINE = 2, 16, 32, 64, 128, 256
 
 
 @pytest.mark.parametrize("distance", [100.0, 1000.0, 30000.0])
@@ -393,6 +393,70 @@ def test_separable_amplitude_matches_exact_form(ctx, monkeypatch, case):
     print(f"   vs ld: separable {e_sep:.2e}, per-pair {e_pp:.2e}")
     assert between < 2e-6
     assert e_sep < STRESS_BAR and e_sep < e_pp + 2e-6
+
+
+def _line_case(case):
+    if case == "gaussian":
+        return W.gaussian_source(65536, seed=2), grid(W.screen(1000.0, 0.02, 96))
+    rays, scr, _ = W.stress_cell(20000.0, float(case[1:]), n_rays=32768, side=96)
+    return rays, grid(scr)
+
+
+@pytest.mark.parametrize("P", [4, 8])
+@pytest.mark.parametrize("case", ["gaussian", "L1000", "L10000", "L30000"])
+def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
+    """A lane's slots on a line of the grid: the phase at slot 0 and the step
+    to the next slot, then one rotation per slot, against every slot's own
+    phase (both with the separable amplitude) and the long-double oracle."""
+    rays, g = _line_case(case)
+    line = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
+    flags = ctx.last_plan()["inst_flags"]
+    monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x2000")
+    per_slot = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
+    f2 = ctx.last_plan()["inst_flags"]
+    monkeypatch.delenv("XRTK_FORCE_FLAGS")
+    assert flags & VAR_LINE and flags & VAR_SEPAMP, hex(flags)
+    assert not f2 & VAR_LINE and f2 & VAR_SEPAMP, hex(f2)
+    px, py, pz = xrtk.prepare_wave(g)
+    stride = 37
+    ld = O.diffraction_ld(rays, px, py, pz, stride=stride)
+    e_line, e_ps = rel_l2(line[::stride], ld), rel_l2(per_slot[::stride], ld)
+    between = rel_l2(line, per_slot)
+    print(f"{case} P={P}: flags {flags:#x}, line vs per-slot {between:.2e}, "
+          f"vs ld: line {e_line:.2e}, per-slot {e_ps:.2e}")
+    assert between < 3e-6
+    assert e_line < STRESS_BAR and e_line < e_ps + 3e-6
+
+
+def test_slot_line_needs_a_line(ctx):
+    """Pixel lists keep their order: a lane's slots are 64 list entries apart.
+    On a 256-wide grid in prepare_wave order that is a line of the grid,
+    allowed with a 0.01 mm screen (64 pixels = 2.5 um); with a 0.04 mm screen
+    the 10 um step leaves the phase 3e-7 rev off the line within four slots
+    (the bound refuses it).  On a 96-wide grid the step wraps rows, and a
+    shuffled list has no line at all (both refused).  At L = 100 mm the
+    separable amplitude, and with it the line, is refused; at P = 2 there is
+    no line variant."""
+    rays = W.gaussian_source(65536, seed=2)
+    for side, half, line in ((256, 0.005, True), (256, 0.02, False), (96, 0.02, False)):
+        g = grid(W.screen(1000.0, half, side))
+        px, py, pz = xrtk.prepare_wave(g)
+        got = xrtk.diffraction_integral(rays, (px, py, pz), ctx=ctx, pixels_per_lane=4)
+        assert bool(ctx.last_plan()["inst_flags"] & VAR_LINE) == line, (side, half)
+        ref = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
+        assert rel_l2(got, ref) < 1e-5
+    perm = np.random.default_rng(5).permutation(px.shape[0])
+    shuffled = xrtk.diffraction_integral(rays, (px[perm], py[perm], pz[perm]), ctx=ctx,
+                                         pixels_per_lane=4)
+    assert not ctx.last_plan()["inst_flags"] & VAR_LINE
+    assert rel_l2(shuffled, ref[perm]) < 1e-5
+    rays, g = _line_case("L100")
+    xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=8)
+    assert not ctx.last_plan()["inst_flags"] & (VAR_LINE | VAR_SEPAMP)
+    rays, g = _line_case("gaussian")
+    xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=2)
+    f = ctx.last_plan()["inst_flags"]
+    assert f & VAR_SEPAMP and not f & VAR_LINE
 
 
 def _transform(rays, pix, M):

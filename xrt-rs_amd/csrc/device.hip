@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_centroid_final(
     scr->nl_lo_key = scr->c_lo_key = 0xffffffffu;
     scr->nl_hi_key = scr->c_hi_key = 0u;
     scr->nu_max_bits = 0u;
-    scr->pad2 = 0u;
+    scr->hmax_bits = scr->sigma_bits = scr->ldev_bits = 0u;
     scr->rmin_bits = 0x7f800000u;  // +inf
     scr->rmax_bits = 0u;
     scr->flags = 0u;
@@ -373,13 +373,17 @@ __device__ __forceinline__ double nl_ref(const InstScratch* scr) {
 // base phase k|q| reaches ~1e12 rad, where one FP64 rounding of |q| is
 // already ~1e-4 rad); the tile reference point q_t is the centre of the
 // tile's bounding box.  Writes the FP32 slot records, |q| per slot (two
-// doubles) and the tile record, and reduces the bounds of |q|, |dq|, R.
+// doubles) and the tile record, and reduces the bounds of |q|, |dq|, R and
+// of the lanes' slot lines (real slots only: perm[s] != ~0u, or s < np
+// without perm; a lane's real slots are a prefix of its P slots).
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix, double kr,
                                                   InstScratch* __restrict__ scr,
                                                   PixRec32* __restrict__ out32,
                                                   TileRec* __restrict__ tiles,
-                                                  double* __restrict__ R64) {
+                                                  double* __restrict__ R64,
+                                                  const uint32_t* __restrict__ perm,
+                                                  uint32_t np) {
   #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x;
   const uint32_t base = blockIdx.x * 64u * P;
@@ -456,7 +460,34 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
   }
   rmin = wave_min(rmin); rmax = wave_max(rmax);
   clo = wave_min(clo); chi = wave_max(chi);
+  // the lane's slot line q_i = q_0 + i h
+  float hm = 0.f, sg = 0.f, dv = 0.f;
+  auto real = [&](int i) {
+    const uint32_t s = base + i * 64u + lane;
+    return perm ? perm[s] != ~0u : s < np;
+  };
+  if (P >= 2 && real(0) && real(1)) {
+    double h[3];
+    for (int k = 0; k < 3; ++k) h[k] = (qh[1][k] - qh[0][k]) + (ql[1][k] - ql[0][k]);
+    const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    const double r0 = sqrt(qh[0][0] * qh[0][0] + qh[0][1] * qh[0][1] + qh[0][2] * qh[0][2]);
+    hm = __double2float_ru(hn);
+    if (hn > 0.0 && r0 > 0.0)
+      sg = __double2float_ru(fabs(qh[0][0] * h[0] + qh[0][1] * h[1] + qh[0][2] * h[2]) / (r0 * hn));
+#pragma unroll
+    for (int i = 2; i < P; ++i) {
+      if (!real(i)) break;
+      for (int k = 0; k < 3; ++k) {
+        const double d = ((qh[i][k] - qh[0][k]) + (ql[i][k] - ql[0][k])) - (double)i * h[k];
+        dv = fmaxf(dv, __double2float_ru(fabs(d)));
+      }
+    }
+  }
+  hm = wave_max(hm); sg = wave_max(sg); dv = wave_max(dv);
   if (lane == 0) {
+    atomicMax(&scr->hmax_bits, __float_as_uint(hm));
+    atomicMax(&scr->sigma_bits, __float_as_uint(sg));
+    atomicMax(&scr->ldev_bits, __float_as_uint(dv));
     atomicMin(&scr->c_lo_key, fkey(clo));
     atomicMax(&scr->c_hi_key, fkey(chi));
     for (int k = 0; k < 3; ++k) {
@@ -510,10 +541,22 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //            nl (1/(1+e) - 1) + (C - n.o/|q|) (1/(1+e)^2 - 1)
 //            - n.o (1/|q| - 1/R_ref) - (nl - nl_ref - n.o/R_ref) (C - C_ref) / (nl_ref + C_ref),
 //          bounded term by term from the ranges of nl, C, |q| and max |n.o|.
+//          LINE (SEPAMP, P >= 4) when every lane's real slots lie on a line
+//          q_i = q_0 + i h and the ray's reduced phase along it,
+//            psi(t) = n(t) (r0(t) - n(t) c4(t)),  n(t) = kr (|o|^2 - 2 o.q(t)),
+//            r0 = 1/(2|q|), c4 = 1/(8 kr |q|^3)  (the PAX form),
+//          departs from the line through slots 0 and 1 by at most
+//            (P-1)(P-2)/2 max |psi''| + kr |q_i - q_0 - i h| 2 |o| / min|q|
+//          <= 1e-7 rev, with |n'| <= 2 kr |o| |h|, |q|' <= sigma |h|, |q|'' <= |h|^2 / |q|:
+//            |psi''| <= 2 kr |o| |h|^2 s / R^2 + kr rho |h|^2 (1 + 2 s^2) / (2 R)
+//                       + kr |o|^2 |h|^2 / R^3 + 3 kr rho |o| |h|^2 s / R^2
+//                       + 15/8 kr rho^2 |h|^2 / R,
+//          R = min |q|, s = sigma + (P-1) |h| / R bounds |q^.h^| along the line.
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
-// 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP.
+// 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP, 0x2000
+// forbids LINE.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
-                       uint32_t force_flags, uint32_t static_flags) {
+                       uint32_t force_flags, uint32_t static_flags, uint32_t P) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
   const double oz = __uint_as_float(scr->omax_bits[2]), om = __uint_as_float(scr->omax_bits[3]);
   const double qx = __uint_as_float(scr->qmax_bits[0]), qy = __uint_as_float(scr->qmax_bits[1]);
@@ -558,6 +601,17 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
                            (0.5 * (nhi - nlo) + nu / rb) * 0.5 * (chi - clo) / (nlb + cb);
         const double den = (nlo + clo - nu / rmin) - nla * e1 - (ca + nu / rmin) * e2;
         if (nlb + cb > 0.0 && den > 0.0 && num <= 1e-6 * den) f |= INST_F_SEPAMP;
+      }
+      if ((f & INST_F_SEPAMP) && P >= 4u && !(force_flags & 0x2000u)) {
+        const double hm = __uint_as_float(scr->hmax_bits);
+        const double dv = __uint_as_float(scr->ldev_bits);
+        const double R = rmin, h2 = hm * hm;
+        const double s = fmin(1.0, (double)__uint_as_float(scr->sigma_bits) + (P - 1.0) * hm / R);
+        const double d2 = 2.0 * kr * om * h2 * s / (R * R) + 0.5 * kr * rho * h2 * (1.0 + 2.0 * s * s) / R +
+                          kr * om * om * h2 / (R * R * R) + 3.0 * kr * rho * om * h2 * s / (R * R) +
+                          1.875 * kr * rho * rho * h2 / R;
+        const double eta = 0.5 * (P - 1.0) * (P - 2.0) * d2 + kr * dv * 1.7320508075688772 * 2.0 * om / R;
+        if (eta <= 1e-7) f |= INST_F_LINE;
       }
     }
   }
@@ -751,7 +805,60 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
   }
 }
 
-template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP>
+// Per-lane constants of the LINE variant: slot 0's record and the step to
+// slot 1 (m_1 - m_0 = -2 kr h, r0_1 - r0_0, c4_1 - c4_0, c4_1), from FP64.
+struct LineRegs {
+  float mx, my, mz, g, r0, c4;
+  float hx, hy, hz, dr0, dc4, c4n;
+};
+
+// Accumulates one ray into the P slots of a lane on a slot line (TILE, UNIN,
+// PAX, SEP): the reduced phase at slot 0 as pair_ray computes it, the step
+// to the next slot
+//   psi_1 - psi_0 = dn rden_1 + num_0 (rden_1 - rden_0),  dn = o.(m_1 - m_0),
+//   rden_1 - rden_0 = (r0_1 - r0_0) - dn c4_1 - num_0 (c4_1 - c4_0),
+// evaluated directly (small, so to full FP32 relative precision), and slot
+// i's e^{2 pi i rev} by i rotations by the step.
+// Per ray and lane: 15 FP32 operations + 2 sin + 2 cos, then 4 per further
+// slot (rotation) and 6 per slot (sums).
+template <int P, bool NOFRACT, bool LIN2>
+__device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
+                                          const LineRegs& L, float h, float (&a)[P][4]) {
+  float lin = r.ox * L.mx;
+  lin = fmaf(r.oy, L.my, lin);
+  if constexpr (!LIN2) lin = fmaf(r.oz, L.mz, lin);
+  const float num = lin + B;
+  const float rden = fmaf(-num, L.c4, L.r0);
+  const float eps = num * rden;
+  const float u = fmaf(-eps, h, L.g);
+  const float rev = fmaf(fmaf(B, u, lin), rden, off);
+  float dn = r.ox * L.hx;
+  dn = fmaf(r.oy, L.hy, dn);
+  if constexpr (!LIN2) dn = fmaf(r.oz, L.hz, dn);
+  const float dr = fmaf(-dn, L.c4n, fmaf(-num, L.dc4, L.dr0));
+  const float step = fmaf(num, dr, dn * (rden + dr));
+  const float f0 = NOFRACT ? rev : __builtin_amdgcn_fractf(rev);
+  const float f1 = NOFRACT ? step : __builtin_amdgcn_fractf(step);
+  float c = __builtin_amdgcn_cosf(f0), s = __builtin_amdgcn_sinf(f0);
+  const float rc = __builtin_amdgcn_cosf(f1), rs = __builtin_amdgcn_sinf(f1);
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    if (i > 0) {
+      const float c2 = fmaf(c, rc, -(s * rs));
+      s = fmaf(c, rs, s * rc);
+      c = c2;
+    }
+    a[i][0] = fmaf(r.cs, c, a[i][0]);
+    a[i][1] = fmaf(r.cs, s, a[i][1]);
+    a[i][2] = fmaf(r.pr, c, a[i][2]);
+    a[i][2] = fmaf(-r.pi, s, a[i][2]);
+    a[i][3] = fmaf(r.pr, s, a[i][3]);
+    a[i][3] = fmaf(r.pi, c, a[i][3]);
+  }
+}
+
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
+          bool LINE = false>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   const uint32_t npad = I.npad;
@@ -775,6 +882,29 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     s.R2[i] = pr.R2; s.R[i] = pr.R; s.g[i] = pr.g; s.nq[i] = pr.nq;
     s.r0[i] = pr.r0; s.c4[i] = pr.c4; s.rinv[i] = pr.rinv; s.c2[i] = pr.c2;
     s.sep[i] = pr.pad;
+  }
+  LineRegs L{};
+  if constexpr (LINE) {
+    // slot 0's record; the step to slot 1 in FP64 from the pixel positions
+    // and |q| (the hi parts tile prep kept)
+    L.mx = s.mx[0]; L.my = s.my[0]; L.mz = s.mz[0];
+    L.g = s.g[0]; L.r0 = s.r0[0]; L.c4 = s.c4[0];
+    const uint32_t s0 = base + lane, s1 = s0 + 64u;
+    const PixF64 p0 = ((const XRTK_CONST PixF64*)I.pix64)[s0];
+    const PixF64 p1 = ((const XRTK_CONST PixF64*)I.pix64)[s1];
+    const double hv[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+    int ax[3];
+    kernel_axes((const InstScratch*)I.scr, ax);
+    L.hx = (float)(-2.0 * kr * pick3(hv, ax[0]));
+    L.hy = (float)(-2.0 * kr * pick3(hv, ax[1]));
+    L.hz = (float)(-2.0 * kr * pick3(hv, ax[2]));
+    const double R0 = ((const XRTK_CONST double*)I.R64)[2 * s0];
+    const double R1 = ((const XRTK_CONST double*)I.R64)[2 * s1];
+    const double i0 = 1.0 / R0, i1 = 1.0 / R1;
+    const double c40 = 0.125 * (i0 * i0 * i0 / kr), c41 = 0.125 * (i1 * i1 * i1 / kr);
+    L.dr0 = (float)(0.5 * i1 - 0.5 * i0);
+    L.dc4 = (float)(c41 - c40);
+    L.c4n = (float)c41;
   }
   // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
   // partials are summed in FP64 by k_merge.
@@ -814,11 +944,20 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
         my[lane] = tr;
       }
       __syncthreads();
+      if constexpr (LINE) {
 #pragma unroll 2
-      for (uint32_t jj = 0; jj < nb; ++jj) {
-        const RayRec32 r = rays[b0 + jj];
-        const TileRay tr = my[jj];
-        pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+        for (uint32_t jj = 0; jj < nb; ++jj) {
+          const RayRec32 r = rays[b0 + jj];
+          const TileRay tr = my[jj];
+          pair_line<P, NOFRACT, LIN2>(r, tr.B, tr.off, L, h, a);
+        }
+      } else {
+#pragma unroll 2
+        for (uint32_t jj = 0; jj < nb; ++jj) {
+          const RayRec32 r = rays[b0 + jj];
+          const TileRay tr = my[jj];
+          pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+        }
       }
     }
   } else {
@@ -846,19 +985,22 @@ __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const 
   const bool tile = (f & INST_F_TILE) != 0u;
   const bool pax = tile && (f & INST_F_PAX);
   const bool sep = pax && (f & INST_F_UNIFORM_N) && (f & INST_F_SEPAMP);
+  const bool line = P >= 4 && sep && (f & INST_F_LINE);
   const uint32_t v = (tile ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
                      ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u) |
-                     (pax ? 16u : 0u) | (sep ? 32u : 0u);
+                     (pax ? 16u : 0u) | (sep ? 32u : 0u) | (line ? 64u : 0u);
   switch (v) {
 #define XRTK_V(V_)                                                                      \
   case V_:                                                                             \
     f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0,         \
-              (V_ & 16) != 0, (V_ & 32) != 0>(I, w, lds, lane, wave);                  \
+              (V_ & 16) != 0, (V_ & 32) != 0, (V_ & 64) != 0 && P >= 4>(I, w, lds, lane,  \
+                                                                         wave);         \
     break;
     XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
     XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
     XRTK_V(24) XRTK_V(25) XRTK_V(26) XRTK_V(27) XRTK_V(28) XRTK_V(29) XRTK_V(30) XRTK_V(31)
     XRTK_V(60) XRTK_V(61) XRTK_V(62) XRTK_V(63)
+    XRTK_V(124) XRTK_V(125) XRTK_V(126) XRTK_V(127)
 #undef XRTK_V
   }
 }
@@ -1104,10 +1246,10 @@ hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx,
 // part: kRedBlocks * 4 doubles of scratch.  P is the pixels-per-lane of the
 // F32R launch (tile = 64 * P slots; npad a multiple of it).
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
-                       InstScratch* scr, double* part, uint32_t mode, double kr, double kr_lo,
-                       double skip, uint32_t force_flags, uint32_t static_flags, int P,
-                       RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
-                       hipStream_t st) {
+                       const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
+                       uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
+                       uint32_t static_flags, int P, RayRec32* rays32, PixRec32* pix32,
+                       TileRec* tiles, double* R64, hipStream_t st) {
   const unsigned nb = nr ? std::min<unsigned>(kRedBlocks, cdiv(nr, 256)) : 1u;
   hipLaunchKernelGGL(k_centroid_partial, dim3(nb), dim3(256), 0, st, rays, nr, part);
   hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(256), 0, st, part, nb, nr, rays, scr);
@@ -1117,13 +1259,13 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
       const unsigned ntile = npad / (64u * P);
       if (P == 2)
         hipLaunchKernelGGL(k_tile_prep<2>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
-                           tiles, R64);
+                           tiles, R64, perm, np);
       else if (P == 4)
         hipLaunchKernelGGL(k_tile_prep<4>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
-                           tiles, R64);
+                           tiles, R64, perm, np);
       else if (P == 8)
         hipLaunchKernelGGL(k_tile_prep<8>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
-                           tiles, R64);
+                           tiles, R64, perm, np);
       else
         return hipErrorInvalidValue;
     } else {
@@ -1131,7 +1273,7 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
     }
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, st, scr, skip, kr, mode, force_flags,
-                     static_flags);
+                     static_flags, (uint32_t)P);
   if (mode == MODE_F32R && nr)
     hipLaunchKernelGGL(k_ray_prep32, dim3(cdiv(nr, 256)), dim3(256), 0, st, rays, nr, scr, kr,
                        kr_lo, npad ? 1u : 0u, rays32);

@@ -112,7 +112,9 @@ struct InstScratch {
   uint32_t nl_lo_key, nl_hi_key;
   uint32_t nu_max_bits;
   uint32_t c_lo_key, c_hi_key;
-  uint32_t pad2;
+  // slot lines (a lane's slots q_0 + i h): max |h|, max |q_0.h| / (|q_0| |h|)
+  // and max |q_i - q_0 - i h| (any axis) over the real slots
+  uint32_t hmax_bits, sigma_bits, ldev_bits;
 };
 
 enum : uint32_t {
@@ -126,6 +128,9 @@ enum : uint32_t {
                           // first-order paraxial expansion (rho <= 1e-4)
   INST_F_SEPAMP = 128u,   // F32R PAX variant: the amplitude as a ray factor times a
                           // slot factor (bound <= 1e-6)
+  INST_F_LINE = 256u,     // F32R SEPAMP variant: each lane's slots lie on a line, the
+                          // phase steps along it by one rotation per slot
+                          // (bound <= 1e-7 rev)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };

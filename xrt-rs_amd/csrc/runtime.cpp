@@ -326,7 +326,8 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
       }
       continue;
     }
-    e = launch_prep(s.rays, s.nr, s.pix, s.npad, reinterpret_cast<InstScratch*>(W + L.scr),
+    e = launch_prep(s.rays, s.nr, s.pix, s.npad, s.perm, s.np,
+                    reinterpret_cast<InstScratch*>(W + L.scr),
                     reinterpret_cast<double*>(W + L.part), s.mode, insts[i].kr_hi,
                     insts[i].kr_lo, s.skip, force_flags, s.static_flags, cfg32.P,
                     reinterpret_cast<RayRec32*>(W + L.rays32),

@@ -28,11 +28,13 @@ hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, uin
 hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,
                            uint32_t th, uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
                            hipStream_t st);
+// perm: slot -> pixel index (~0u for padding), or null when slots [np, npad)
+// are the padding.
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
-                       InstScratch* scr, double* part, uint32_t mode, double kr, double kr_lo,
-                       double skip, uint32_t force_flags, uint32_t static_flags, int P,
-                       RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
-                       hipStream_t st);
+                       const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
+                       uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
+                       uint32_t static_flags, int P, RayRec32* rays32, PixRec32* pix32,
+                       TileRec* tiles, double* R64, hipStream_t st);
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st);

@@ -277,8 +277,8 @@ struct xrtk_server {
     if (e == hipSuccess && s.npad)
       e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, s.npad, pix, perm, stream);
     if (e == hipSuccess && s.nr)
-      e = launch_prep(s.rays, s.nr, pix, s.npad, scr, part, s.mode, I.kr_hi, I.kr_lo, s.skip, 0,
-                      s.static_flags, c32.P, rays32, pix32, tiles, R64, stream);
+      e = launch_prep(s.rays, s.nr, pix, s.npad, perm, s.np, scr, part, s.mode, I.kr_hi, I.kr_lo,
+                      s.skip, 0, s.static_flags, c32.P, rays32, pix32, tiles, R64, stream);
     if (e == hipSuccess && !s.nr && s.np) e = launch_zero_f64(out, (size_t)s.np * 4, stream);
     return map_err(e);
   }
