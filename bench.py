@@ -49,6 +49,9 @@ PEAK_FP64 = 78.6            # TFLOP/s, MI355X FP64 vector
 PEAK_MFMA_FP32 = 157.3      # TFLOP/s, MI355X dense FP32 matrix
 PEAK_MFMA_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0           # GB/s
+# VALU issue rates of gfx950 in wave-instructions per CU per clock, measured
+# by xrt-rs_amd/probes/issue_rate (v_fmac_f32; mean of v_sin/v_cos_f32)
+ISSUE_FMA, ISSUE_TRANS = 1.417, 0.435
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
 
 
@@ -182,7 +185,11 @@ def leg_headline(args, ctx, xrtk, W):
 
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None}
+            "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None,
+            "note": "achieved counts the reference formula's 51 flop per pair; the kernel "
+                    "executes fewer operations per pair (reduced phase, separable amplitude, "
+                    "slot-line rotation), so frac can exceed 1; issue_frac is the share of "
+                    "the VALU issue capacity its own instructions use"}
     t = traffic_record(nr, npx, plan)
     if t is not None:
         roof["traffic"] = t["hbm_bytes_per_launch"]
@@ -190,6 +197,11 @@ def leg_headline(args, ctx, xrtk, W):
         roof["traffic_frac_hbm"] = roof["traffic_gbps"] / PEAK_HBM
         roof["valu_insts_per_pair"] = t.get("valu_insts_per_pair")
         roof["trans_insts_per_pair"] = t.get("trans_insts_per_pair")
+        if t.get("sclk_ghz") and t.get("n_cu"):
+            # wave-instructions per CU per clock over the probe's issue rates
+            waves_s = pairs / (kernel_ms * 1e-3) / 64.0 / (t["n_cu"] * t["sclk_ghz"] * 1e9)
+            vt, tt = t["valu_insts_per_pair"], t["trans_insts_per_pair"]
+            roof["issue_frac"] = waves_s * ((vt - tt) / ISSUE_FMA + tt / ISSUE_TRANS)
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
             "plan": plan, "roofline": roof, "fp64": fp64, "oneshot": oneshot}
     return head, rays, wave, out32, rs
