@@ -116,6 +116,12 @@ def leg_headline(args, ctx, xrtk, W):
     pairs = float(nr) * float(npx)
     rs = xrtk.RaySet(ctx, rays)
     job = xrtk.Job(rs, 10000.0, wave)
+    # launch shape for this job from the autotuner (untimed; it times the
+    # candidate shapes on the job cut to 65,536 rays)
+    t = time.perf_counter()
+    tuned, _ = xrtk.autotune_jobs(ctx, [job])
+    log(f"autotune: {time.perf_counter() - t:.1f}s, chosen P={tuned['pixels_per_lane']} "
+        f"W={tuned['waves_per_group']} persistent={tuned['persistent']}")
 
     for i in range(args.warmup):
         xrtk.solve_jobs(ctx, [job], mode="f32r")
@@ -134,10 +140,13 @@ def leg_headline(args, ctx, xrtk, W):
 
     # per-launch duration of the pair kernel alone (HIP events on the
     # engine's stream around the launch), from eager solves of the instance
-    xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r")
+    # with the tuned launch shape
+    shape = {k: tuned[k] for k in ("pixels_per_lane", "waves_per_group", "persistent",
+                                   "tile_rays")}
+    xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r", **shape)
     k_ms = []
     for _ in range(3):
-        xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r")
+        xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r", **shape)
         k_ms.append(ctx.last_timing()[0])
     kernel_ms = float(np.median(k_ms))
     plan = ctx.last_plan()
@@ -203,7 +212,8 @@ def leg_headline(args, ctx, xrtk, W):
             vt, tt = t["valu_insts_per_pair"], t["trans_insts_per_pair"]
             roof["issue_frac"] = waves_s * ((vt - tt) / ISSUE_FMA + tt / ISSUE_TRANS)
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
-            "plan": plan, "roofline": roof, "fp64": fp64, "oneshot": oneshot}
+            "plan": plan, "autotuned": tuned, "roofline": roof, "fp64": fp64,
+            "oneshot": oneshot}
     return head, rays, wave, out32, rs
 
 

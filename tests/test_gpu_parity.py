@@ -185,7 +185,7 @@ def test_broadband_routes_to_fp64(ctx):
 
 
 # ---- launch-shape invariance (every instantiated kernel) ------------------------------------
-SHAPES_F32 = [(P, W, 1, per) for P in (2, 4, 8) for W in (1, 4) for per in (0, 1)]
+SHAPES_F32 = [(P, W, 1, per) for P in (2, 4, 8, 16) for W in (1, 4) for per in (0, 1)]
 SHAPES_F64 = [(P, W, 1, per) for P in (1, 2) for W in (1, 4) for per in (0, 1)]
 
 
@@ -402,12 +402,16 @@ def _line_case(case):
     return rays, grid(scr)
 
 
-@pytest.mark.parametrize("P", [4, 8])
+@pytest.mark.parametrize("P", [4, 8, 16])
 @pytest.mark.parametrize("case", ["gaussian", "L1000", "L10000", "L30000"])
 def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
     """A lane's slots on a line of the grid: the phase at slot 0 and the step
-    to the next slot, then one rotation per slot, against every slot's own
-    phase (both with the separable amplitude) and the long-double oracle."""
+    to the next slot, then one rotation and the three-term recurrence along
+    the lane, against every slot's own phase (both with the separable
+    amplitude) and the long-double oracle.  At L = 1 m and P = 16 (15 steps
+    of 2 pixels, 0.47 um at 20 keV) the bound on the phase's departure from
+    the line exceeds 1e-7 rev and the line form is refused: the same bits
+    as the per-slot form."""
     rays, g = _line_case(case)
     line = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
     flags = ctx.last_plan()["inst_flags"]
@@ -415,8 +419,12 @@ def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
     per_slot = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
     f2 = ctx.last_plan()["inst_flags"]
     monkeypatch.delenv("XRTK_FORCE_FLAGS")
-    assert flags & VAR_LINE and flags & VAR_SEPAMP, hex(flags)
     assert not f2 & VAR_LINE and f2 & VAR_SEPAMP, hex(f2)
+    if case == "L1000" and P == 16:
+        assert not flags & VAR_LINE, hex(flags)
+        assert np.array_equal(line, per_slot)
+        return
+    assert flags & VAR_LINE and flags & VAR_SEPAMP, hex(flags)
     px, py, pz = xrtk.prepare_wave(g)
     stride = 37
     ld = O.diffraction_ld(rays, px, py, pz, stride=stride)

@@ -883,7 +883,7 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     mtz = (float)(-2.0 * kr * T.qz);
   }
 #pragma unroll
-  for (int i = 0; i < P; ++i) {
+  for (int i = 0; i < (LINE ? 1 : P); ++i) {  // the line form reads slot 0 only
     const PixRec32 pr = I.pix32[base + i * 64u + lane];
     s.mx[i] = TILE ? pr.mx : pr.mx + mtx;
     s.my[i] = TILE ? pr.my : pr.my + mty;
@@ -980,41 +980,61 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
-    const float m = SEP ? s.sep[i] : 1.f;  // the slot factor of the amplitude
+    // the slot factor of the amplitude (the line form reads it only here)
+    const float m = !SEP ? 1.f : (LINE ? I.pix32[p].pad : s.sep[i]);
     part[p] = make_float4(a[i][0] * m, a[i][1] * m, a[i][2] * m, a[i][3] * m);
   }
 }
 
+// Whether an instance runs the slot-line form at this P.
+template <int P>
+__device__ __forceinline__ bool line_variant(uint32_t f) {
+  constexpr uint32_t need = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
+                            INST_F_LINE;
+  return P >= 4 && (f & need) == need;
+}
+
 // Runs one work item with the variant the instance's flags select (PAX only
 // exists in the tile-referenced form, SEPAMP only with PAX and a uniform
-// normal).
-template <int P, int W>
+// normal).  LINE: the slot-line variants only (the lean kernel); otherwise
+// every other variant.
+template <int P, int W, bool LINE>
 __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const WorkItem& w,
                                               TileRay* lds, uint32_t lane, uint32_t wave) {
+  if constexpr (LINE) {
+    const uint32_t v = ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+    switch (v) {
+      case 0: f32r_item<P, W, true, true, false, false, true, true, true>(I, w, lds, lane, wave); break;
+      case 1: f32r_item<P, W, true, true, false, true, true, true, true>(I, w, lds, lane, wave); break;
+      case 2: f32r_item<P, W, true, true, true, false, true, true, true>(I, w, lds, lane, wave); break;
+      case 3: f32r_item<P, W, true, true, true, true, true, true, true>(I, w, lds, lane, wave); break;
+    }
+    return;
+  }
   const bool tile = (f & INST_F_TILE) != 0u;
   const bool pax = tile && (f & INST_F_PAX);
   const bool sep = pax && (f & INST_F_UNIFORM_N) && (f & INST_F_SEPAMP);
-  const bool line = P >= 4 && sep && (f & INST_F_LINE);
   const uint32_t v = (tile ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
                      ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u) |
-                     (pax ? 16u : 0u) | (sep ? 32u : 0u) | (line ? 64u : 0u);
+                     (pax ? 16u : 0u) | (sep ? 32u : 0u);
   switch (v) {
 #define XRTK_V(V_)                                                                      \
   case V_:                                                                             \
     f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0,         \
-              (V_ & 16) != 0, (V_ & 32) != 0, (V_ & 64) != 0 && P >= 4>(I, w, lds, lane,  \
-                                                                         wave);         \
+              (V_ & 16) != 0, (V_ & 32) != 0>(I, w, lds, lane, wave);                  \
     break;
     XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
     XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
     XRTK_V(24) XRTK_V(25) XRTK_V(26) XRTK_V(27) XRTK_V(28) XRTK_V(29) XRTK_V(30) XRTK_V(31)
     XRTK_V(60) XRTK_V(61) XRTK_V(62) XRTK_V(63)
-    XRTK_V(124) XRTK_V(125) XRTK_V(126) XRTK_V(127)
 #undef XRTK_V
   }
 }
 
-template <int P, int W, bool PERSIST>
+// LINE: the lean kernel, which runs the slot-line items and skips the rest
+// (its register allocation is that of the slot-line code alone); the other
+// kernel runs every other item.  Each has its own queue counter.
+template <int P, int W, bool PERSIST, bool LINE>
 __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     const KInst* __restrict__ insts, const WorkItem* __restrict__ items,
     uint32_t n_items, uint32_t* __restrict__ queue, uint32_t remap) {
@@ -1036,7 +1056,7 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
     const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
-    f32r_dispatch<P, W>(f, I, w, lds, lane, wave);
+    if (line_variant<P>(f) == LINE) f32r_dispatch<P, W, LINE>(f, I, w, lds, lane, wave);
     if constexpr (!PERSIST) break;
   }
 }
@@ -1275,6 +1295,9 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
       else if (P == 8)
         hipLaunchKernelGGL(k_tile_prep<8>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
                            tiles, R64, perm, np);
+      else if (P == 16)
+        hipLaunchKernelGGL(k_tile_prep<16>, dim3(ntile), dim3(64), 0, st, pix, kr, scr, pix32,
+                           tiles, R64, perm, np);
       else
         return hipErrorInvalidValue;
     } else {
@@ -1289,11 +1312,16 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
   return hipGetLastError();
 }
 
+// F32R: the lean slot-line kernel (P >= 4) on queue[1], then the kernel of
+// every other variant on queue[0].
 template <int P, int W, bool PERSIST>
 static void launch_f32r_t(const KInst* insts, const WorkItem* items, uint32_t n_items,
                           uint32_t* queue, uint32_t grid, uint32_t remap, hipStream_t st) {
-  hipLaunchKernelGGL((k_pair_f32r<P, W, PERSIST>), dim3(grid), dim3(64 * W), 0, st, insts, items,
-                     n_items, queue, remap);
+  if constexpr (P >= 4)
+    hipLaunchKernelGGL((k_pair_f32r<P, W, PERSIST, true>), dim3(grid), dim3(64 * W), 0, st,
+                       insts, items, n_items, queue + 1, remap);
+  hipLaunchKernelGGL((k_pair_f32r<P, W, PERSIST, false>), dim3(grid), dim3(64 * W), 0, st, insts,
+                     items, n_items, queue, remap);
 }
 template <int P, int W, bool PERSIST>
 static void launch_f64_t(const KInst* insts, const WorkItem* items, uint32_t n_items,
@@ -1307,7 +1335,7 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st) {
   if (!n_items) return hipSuccess;
-  if (cfg.persistent) hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 1u);
+  if (cfg.persistent) hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 2u);
   const bool ps = cfg.persistent != 0;
 #define XRTK_F32(P_, W_)                                                                \
   if (cfg.P == P_ && cfg.W == W_) {                                                     \
@@ -1323,6 +1351,7 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
   }
   if (mode == MODE_F32R) {
     XRTK_F32(2, 1) XRTK_F32(2, 4) XRTK_F32(4, 1) XRTK_F32(4, 4) XRTK_F32(8, 1) XRTK_F32(8, 4)
+    XRTK_F32(16, 1) XRTK_F32(16, 4)
   } else {
     XRTK_F64(1, 1) XRTK_F64(1, 4) XRTK_F64(2, 1) XRTK_F64(2, 4)
   }
@@ -1333,7 +1362,8 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c) {
   if (c.ray_src != 1) return false;
-  if (mode == MODE_F32R) return (c.P == 2 || c.P == 4 || c.P == 8) && (c.W == 1 || c.W == 4);
+  if (mode == MODE_F32R)
+    return (c.P == 2 || c.P == 4 || c.P == 8 || c.P == 16) && (c.W == 1 || c.W == 4);
   return (c.P == 1 || c.P == 2) && (c.W == 1 || c.W == 4);
 }
 

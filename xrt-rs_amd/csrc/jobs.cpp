@@ -379,7 +379,7 @@ extern "C" int xrtk_autotune(xrtk_ctx* ctx, size_t n_rays, size_t n_pixels, int 
     std::vector<LaunchCfg> cands;
     if (mode == MODE_F32R) {
       for (int src : {1})
-        for (int P : {2, 4, 8})
+        for (int P : {2, 4, 8, 16})
           for (int W : {1, 4})
             for (int per : {0, 1}) {
               LaunchCfg c; c.P = P; c.W = W; c.ray_src = src; c.persistent = per; c.tile = 8192;
@@ -451,7 +451,7 @@ extern "C" int xrtk_autotune_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_j
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
     std::vector<LaunchCfg> cands;
-    for (int P : mode == MODE_F32R ? std::vector<int>{2, 4, 8} : std::vector<int>{1, 2})
+    for (int P : mode == MODE_F32R ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1, 2})
       for (int W : {1, 4})
         for (int per : {0, 1}) {
           LaunchCfg c;
