@@ -117,20 +117,28 @@ def leg_headline(args, ctx, xrtk, W):
     rs = xrtk.RaySet(ctx, rays)
     job = xrtk.Job(rs, 10000.0, wave)
     # launch shape for this job from the autotuner (untimed; it times the
-    # candidate shapes on the job cut to 65,536 rays)
-    t = time.perf_counter()
-    tuned, _ = xrtk.autotune_jobs(ctx, [job])
-    log(f"autotune: {time.perf_counter() - t:.1f}s, chosen P={tuned['pixels_per_lane']} "
-        f"W={tuned['waves_per_group']} persistent={tuned['persistent']}")
+    # candidate shapes on the job cut to 65,536 rays), or as given
+    if args.shape:
+        P, Wv, per = (int(x) for x in args.shape.split(","))
+        tuned = {"pixels_per_lane": P, "waves_per_group": Wv, "persistent": per,
+                 "tile_rays": 8192}
+        log(f"shape given: P={P} W={Wv} persistent={per}")
+    else:
+        t = time.perf_counter()
+        tuned, _ = xrtk.autotune_jobs(ctx, [job])
+        log(f"autotune: {time.perf_counter() - t:.1f}s, chosen P={tuned['pixels_per_lane']} "
+            f"W={tuned['waves_per_group']} persistent={tuned['persistent']}")
+    shape = {k: tuned[k] for k in ("pixels_per_lane", "waves_per_group", "persistent",
+                                   "tile_rays")}
 
     for i in range(args.warmup):
-        xrtk.solve_jobs(ctx, [job], mode="f32r")
+        xrtk.solve_jobs(ctx, [job], mode="f32r", **shape)
         log(f"warmup {i + 1}/{args.warmup}: pair {ctx.last_timing()[0]:.1f} ms")
     device_sync()
     dev_ms = []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        (out32,) = xrtk.solve_jobs(ctx, [job], mode="f32r")
+        (out32,) = xrtk.solve_jobs(ctx, [job], mode="f32r", **shape)
         dev_ms.append(ctx.last_timing()[1])
     device_sync()
     t_total = time.perf_counter() - t_start
@@ -140,9 +148,7 @@ def leg_headline(args, ctx, xrtk, W):
 
     # per-launch duration of the pair kernel alone (HIP events on the
     # engine's stream around the launch), from eager solves of the instance
-    # with the tuned launch shape
-    shape = {k: tuned[k] for k in ("pixels_per_lane", "waves_per_group", "persistent",
-                                   "tile_rays")}
+    # with the same launch shape
     xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r", **shape)
     k_ms = []
     for _ in range(3):
@@ -549,6 +555,8 @@ def main():
     ap.add_argument("--no-serve-compare", action="store_true")
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--shape", default="", metavar="P,W,PERSISTENT",
+                    help="headline launch shape instead of the autotuner's")
     ap.add_argument("--no-fresnel", action="store_true")
     ap.add_argument("--no-undulator", action="store_true")
     ap.add_argument("--und-energies", type=int, default=1024)
