@@ -94,7 +94,8 @@ typedef struct XrtkWaveParams {
 
 typedef struct XrtkOpts {
     int32_t mode;             /* XRTK_MODE_*                                */
-    int32_t pixels_per_lane;  /* 0 = autotune, else 1, 2, 4 or 8            */
+    int32_t pixels_per_lane;  /* 0 = autotune, else 2, 4, 8 or 16 (F32R), 1
+                                 or 2 (FP64)                                 */
     int32_t waves_per_group;  /* 0 = autotune, else 1, 2 or 4               */
     int32_t tile_rays;        /* 0 = autotune: rays per FP32 partial sum    */
     int32_t split;            /* 0 = autotune: ray chunks per pixel block   */
@@ -326,6 +327,10 @@ typedef struct XrtkServerCfg {
     double quantum_ms;         /* target device time of one batched launch  */
     int32_t mode;              /* XRTK_MODE_*                               */
     int32_t policy;            /* XRTK_SCHED_*                              */
+    const XrtkOpts *opts;      /* F32R launch shape (the FP64 kernel takes
+                                  it where it has that shape); NULL or 0
+                                  fields: the context's batch configuration
+                                  (xrtk_autotune_jobs)                      */
 } XrtkServerCfg;
 
 /* Which requests fill a launch.  Every active request with work left gets

@@ -52,6 +52,36 @@ def test_server_results_equal_standalone(ctx, sets):
         srv.close()
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_server_launch_shapes(ctx, sets, persistent):
+    """The server on a given launch shape (P = 16, one wave per group;
+    one group per work item, or persistent groups on the queue) equals the
+    batched solve on the same shape bit for bit, with every request's items
+    run (more items per launch than resident groups)."""
+    specs, rsets = sets
+    big_spec = {"n_rays": 1 << 20, "side": 512, "energy": 10000.0, "distance": 1000.0,
+                "seed": 78, "half": 0.2 * 0.1 * (1000.0 + W.SRC_DIST) / W.SRC_DIST}
+    big_rs = xrtk.RaySet(ctx, W.instance_rays(big_spec))
+    try:
+        # the 1 M x 512^2 request has 32768 work items (128 chunks x 256 pixel
+        # blocks): a 100 ms quantum holds ~2e4 of them, against 8192 groups
+        jobs = jobs_of(specs, rsets) + [
+            xrtk.Job(big_rs, big_spec["energy"], grid(W.instance_screen(big_spec)))]
+        shape = dict(pixels_per_lane=16, waves_per_group=1, persistent=persistent)
+        alone = [xrtk.solve_jobs(ctx, [j], **shape)[0] for j in jobs]
+        srv = xrtk.Server(ctx, hbm_budget_bytes=16 << 30, quantum_ms=100.0, **shape)
+        try:
+            got = [srv.wait(t, timeout_ms=60000)[0] for t in [srv.submit(j) for j in jobs]]
+            st = srv.stats()
+        finally:
+            srv.close()
+        for k, (g, a) in enumerate(zip(got, alone)):
+            assert np.array_equal(g, a), k
+        assert st["completed"] == len(jobs)
+    finally:
+        big_rs.release()
+
+
 def test_short_quantum_chunks_match_standalone(ctx, sets):
     """A short quantum cuts the requests' ray chunks (one work item stays
     within half a quantum): the FP32 partial sums change, the result stays

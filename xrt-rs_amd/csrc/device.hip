@@ -1335,6 +1335,8 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st) {
   if (!n_items) return hipSuccess;
+  // a non-persistent launch runs one item per group: every item needs one
+  if (!cfg.persistent && grid < n_items) return hipErrorInvalidValue;
   if (cfg.persistent) hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 2u);
   const bool ps = cfg.persistent != 0;
 #define XRTK_F32(P_, W_)                                                                \

@@ -438,7 +438,9 @@ struct xrtk_server {
     uint32_t* d_q = reinterpret_cast<uint32_t*>(M + a_ins + a_32 + a_64);
     hipError_t e = hipMemcpyAsync(M, H, a_ins + a_32 + a_64, hipMemcpyHostToDevice, stream);
     (void)hipEventRecord(L.start, stream);
+    // persistent groups pull items from the queue; otherwise one group per item
     auto grid_of = [&](const LaunchCfg& c, uint32_t mode, size_t n) {
+      if (!c.persistent) return (uint32_t)n;
       return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n, grid_res(c, mode)));
     };
     if (e == hipSuccess && !L.h_items32.empty())
@@ -569,8 +571,12 @@ int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** ou
     if (s->cfg.quantum_ms <= 0) s->cfg.quantum_ms = 10.0;
     {
       std::lock_guard<std::mutex> lk(ctx->mu);
-      s->c32 = batch_cfg(ctx, MODE_F32R, nullptr);
-      s->c64 = batch_cfg(ctx, MODE_F64, nullptr);
+      // the options shape the F32R kernel; the FP64 kernel takes them where
+      // it has that shape, else the context's batch configuration
+      s->c32 = batch_cfg(ctx, MODE_F32R, cfg->opts);
+      s->c64 = batch_cfg(ctx, MODE_F64, cfg->opts);
+      if (!pair_cfg_supported(MODE_F32R, s->c32)) return XRTK_EINVAL;
+      if (!pair_cfg_supported(MODE_F64, s->c64)) s->c64 = batch_cfg(ctx, MODE_F64, nullptr);
     }
     auto cap = [&](const LaunchCfg& c, double wave_rate) -> uint32_t {
       const double rays = 0.5e-3 * s->cfg.quantum_ms * wave_rate / (64.0 * c.P);

@@ -372,10 +372,14 @@ class Server:
     """Continuous-batching solver (xrtk_server_*)."""
 
     def __init__(self, ctx: GpuContext, hbm_budget_bytes: int, max_active: int = 64,
-                 quantum_ms: float = 10.0, mode="f32r", policy="shortest"):
+                 quantum_ms: float = 10.0, mode="f32r", policy="shortest", **shape):
+        """shape: launch-shape options (pixels_per_lane, waves_per_group,
+        persistent, ...) instead of the context's batch configuration."""
         pol = {"shortest": 0, "arrival": 1}[policy]
+        self._shape = _lib.opts(_mode(mode), **shape) if shape else None
         cfg = _lib.XrtkServerCfg(int(hbm_budget_bytes), int(max_active), float(quantum_ms),
-                                 _mode(mode), pol)
+                                 _mode(mode), pol,
+                                 C.pointer(self._shape) if self._shape is not None else None)
         h = C.c_void_p()
         check(lib.xrtk_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "xrtk_server_create")
         self._h = h

@@ -94,7 +94,8 @@ class XrtkUndulator(C.Structure):
 
 class XrtkServerCfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int32),
-                ("quantum_ms", C.c_double), ("mode", C.c_int32), ("policy", C.c_int32)]
+                ("quantum_ms", C.c_double), ("mode", C.c_int32), ("policy", C.c_int32),
+                ("opts", C.POINTER(XrtkOpts))]
 
 
 class XrtkServerStats(C.Structure):
