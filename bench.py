@@ -483,15 +483,22 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
                 time.sleep(min(d, 0.002))
             tickets.append((srv.submit(xrtk.Job(sets[i], e, screens[i])), t0 + ta, pairs_of[i]))
         t_sub = xrtk.now() - t0
-        lat, queue, done = [], [], []
+        lat, queue, done, outs = [], [], [], []
         for tk, ta, _ in tickets:
-            _, ts = srv.wait(tk, timeout_ms=600000)
+            out, ts = srv.wait(tk, timeout_ms=600000)
+            outs.append(out)
             lat.append(ts[3] - ta)
             queue.append(ts[1] - ts[0])
             done.append(ts[3])
         st = srv.stats()
     finally:
         srv.close()
+    # the served results of the 4 smallest requests against the batched solve
+    # (FP32 rounding apart: the server may cut the ray chunks shorter)
+    small = sorted(range(len(tickets)), key=lambda k: tickets[k][2])[:4]
+    check = max(W.rel_l2(outs[k], xrtk.solve_jobs(ctx, [xrtk.Job(sets[reqs[k][0]], reqs[k][1],
+                                                                    screens[reqs[k][0]])])[0])
+                for k in small)
     total = sum(p for _, _, p in tickets)
     span = max(done) - t0
     lat_ms = np.array(lat) * 1e3
@@ -519,12 +526,16 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
            "max_concurrent": st.get("max_concurrent"), "launches": st.get("launches"),
            "pair_kernel_busy_frac": st.get("busy_ms", 0.0) * 1e-3 / span,
            "hbm_peak_gb": st.get("hbm_peak", 0) / 1e9,
-           "hbm_budget_gb": budget / 1e9, "quantum_ms": args.quantum_ms, "policy": policy}
+           "hbm_budget_gb": budget / 1e9, "quantum_ms": args.quantum_ms, "policy": policy,
+           "check_max_rel_l2_vs_batched_4_smallest": check}
     log(f"serving ({policy} first) at {rate:.1f} req/s for {seconds:.1f}s: {len(tickets)} requests, "
         f"p50 {res['p50_ms']:.0f} ms, p99 {res['p99_ms']:.0f} ms (largest request: "
         f"{res['largest_request_latency_ms']:.0f} ms, {res['alone_ms_max']:.0f} ms alone), "
         f"slowdown p50 {res['slowdown_p50']:.1f} p99 {res['slowdown_p99']:.1f}, sustained "
-        f"{res['sustained_gpairs_per_s']:.0f} Gpair/s (offered {res['offered_gpairs_per_s']:.0f})")
+        f"{res['sustained_gpairs_per_s']:.0f} Gpair/s (offered {res['offered_gpairs_per_s']:.0f}), "
+        f"served vs batched on the 4 smallest: {check:.1e}")
+    if not check < 1e-5:
+        raise RuntimeError(f"served results differ from the batched solve: {check}")
     return res
 
 
