@@ -415,7 +415,6 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
   float aq[3] = {0.f, 0.f, 0.f}, ad[3] = {0.f, 0.f, 0.f};
   float rmin = __uint_as_float(0x7f800000u), rmax = 0.f;
   float clo = __uint_as_float(0x7f800000u), chi = -__uint_as_float(0x7f800000u);
-  const double nlb = nl_ref(scr);
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     double Rh, Rl;
@@ -435,11 +434,11 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     r.c4 = (float)(0.125 * ri3);
     r.rinv = (float)ri;
     r.c2 = (float)(0.5 * ri3);
-    // separable amplitude: slot factor (nl_ref + C) / |q|, C = n0.q / |q|
+    // separable amplitude: range of C = n0.q / |q| (the slot factor
+    // (nl_ref + C) / |q| is applied by k_merge)
     r.pad = 0.f;
     if (uni && Rh > 0.0) {
       const double C = (n0x * qh[i][0] + n0y * qh[i][1] + n0z * qh[i][2]) * ri;
-      r.pad = (float)((nlb + C) * ri);
       clo = fminf(clo, __double2float_rd(C));
       chi = fmaxf(chi, __double2float_ru(C));
     }
@@ -731,7 +730,7 @@ static constexpr uint32_t kBlockRays = 64;  // rays staged per LDS block (one pe
 template <int P>
 struct SlotRegs {
   float mx[P], my[P], mz[P], R2[P], R[P], g[P], nq[P];
-  float r0[P], c4[P], rinv[P], c2[P], sep[P];
+  float r0[P], c4[P], rinv[P], c2[P];
 };
 
 // Accumulates one ray into a[P][4] for the P slots of this lane.
@@ -742,8 +741,8 @@ struct SlotRegs {
 //   PAX:   (tile variant) 1/path and 1/(path + R) from the paraxial
 //          expansion: one FMA each instead of rsq and rcp.
 //   SEP:   (PAX variant) the amplitude is the ray factor already in the
-//          ray's polarisation terms times the slot factor applied to the
-//          sums afterwards: no per-pair amplitude.
+//          ray's polarisation terms times the slot factor that k_merge
+//          applies to the summed partials: no per-pair amplitude.
 // Per pair (TILE, UNIN, NOFRACT, LIN2, PAX): 21 FP32 add/mul/FMA + sin, cos;
 // with SEP: 14 + sin, cos; without PAX: 20 + rsq, rcp, sin, cos.
 template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP>
@@ -890,7 +889,6 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     s.mz[i] = TILE ? pr.mz : pr.mz + mtz;
     s.R2[i] = pr.R2; s.R[i] = pr.R; s.g[i] = pr.g; s.nq[i] = pr.nq;
     s.r0[i] = pr.r0; s.c4[i] = pr.c4; s.rinv[i] = pr.rinv; s.c2[i] = pr.c2;
-    s.sep[i] = pr.pad;
   }
   LineRegs L{};
   if constexpr (LINE) {
@@ -976,13 +974,12 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
       pair_ray<P, false, UNIN, NOFRACT, LIN2, false, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
+  // (SEP: k_merge applies the slot factor of the amplitude)
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     const uint32_t p = base + i * 64u + lane;
-    // the slot factor of the amplitude (the line form reads it only here)
-    const float m = !SEP ? 1.f : (LINE ? I.pix32[p].pad : s.sep[i]);
-    part[p] = make_float4(a[i][0] * m, a[i][1] * m, a[i][2] * m, a[i][3] * m);
+    part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
   }
 }
 
@@ -1208,7 +1205,17 @@ __global__ __launch_bounds__(256) void k_merge(const KInst* __restrict__ insts,
   if (I.mode == MODE_F32R) {
     double c, sn;
     cis_rev_dd2(I.kr_hi, I.kr_lo, I.R64[2 * p], I.R64[2 * p + 1], &c, &sn);
-    const double kf = I.k / (4.0 * kPI);
+    double kf = I.k / (4.0 * kPI);
+    const InstScratch* scr = (const InstScratch*)I.scr;
+    if (scr->flags & INST_F_SEPAMP) {
+      // the slot factor of the separable amplitude, (nl_ref + C) / |q| with
+      // C = n0.q / |q| (the ray factor is in the sums)
+      const PixF64 px = I.pix64[p];
+      const double R = I.R64[2 * p];
+      const double C = (scr->n0x * (px.x - scr->cx) + scr->n0y * (px.y - scr->cy) +
+                        scr->n0z * (px.z - scr->cz)) / R;
+      kf *= (nl_ref(scr) + C) / R;
+    }
     c *= kf; sn *= kf;
     const double a0 = o[0] * c - o[1] * sn, a1 = o[0] * sn + o[1] * c;
     const double a2 = o[2] * c - o[3] * sn, a3 = o[2] * sn + o[3] * c;
