@@ -136,7 +136,9 @@ def test_energy_scan_reuses_resident_rays(ctx, sets):
     specs, rsets = sets
     s, r = specs[0], rsets[0]
     energies = [8000.0, 9000.0, 12000.0]
-    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30)
+    # a quantum long enough for the configuration's own ray chunks: the
+    # batched solve's plan, so the same bits
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, quantum_ms=100.0)
     try:
         ts = [srv.submit(xrtk.Job(r, e, grid(W.instance_screen(s)))) for e in energies]
         outs = [srv.wait(t, timeout_ms=60000)[0] for t in ts]

@@ -145,6 +145,8 @@ struct Launch {
 // Nominal pair rates of one resident wave of the pair kernels on gfx950
 // (32 waves per CU for F32R, 16 for FP64); they size the server's work items.
 static constexpr double kWavePairsF32 = 2.9e8, kWavePairsF64 = 3.2e7;
+// Most ray chunks (partial sums per slot) the item cap may give a request.
+static constexpr uint64_t kMaxChunks = 512;
 
 struct xrtk_server {
   xrtk_ctx* ctx = nullptr;
@@ -203,7 +205,13 @@ struct xrtk_server {
     r->cfg = s.mode == MODE_F32R ? c32 : c64;
     r->chunk_len = chunk_len_for(ctx, r->cfg, s.nr, s.npad, 0);
     const uint32_t cap = s.mode == MODE_F32R ? cap32 : cap64;
-    if (cap && r->chunk_len > cap) r->chunk_len = cap;
+    if (cap && r->chunk_len > cap) {
+      // short items, but at most kMaxChunks partial sums per slot (a large
+      // request then runs longer items rather than holding GBs of partials)
+      const uint64_t floor_len = ((uint64_t)s.nr + kMaxChunks - 1) / kMaxChunks;
+      const uint32_t min_len = (uint32_t)((floor_len + 63) / 64 * 64);
+      r->chunk_len = std::min(r->chunk_len, std::max(cap, min_len));
+    }
     r->n_chunks = s.nr ? (s.nr + r->chunk_len - 1) / r->chunk_len : 0;
     const uint32_t B = 64u * r->cfg.W * r->cfg.P;
     r->pblocks = s.npad / B;
