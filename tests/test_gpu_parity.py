@@ -328,6 +328,7 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
 
 
 VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP, VAR_LINE = 2, 16, 32, 64, 128, 256
+VAR_LINEDEV = 1024
 
 
 @pytest.mark.parametrize("distance", [100.0, 1000.0, 30000.0])
@@ -436,17 +437,45 @@ def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
     assert e_line < STRESS_BAR and e_line < e_ps + 3e-6
 
 
-def test_slot_line_needs_a_line(ctx):
-    """Pixel lists keep their order: a lane's slots are 64 list entries apart.
-    On a 256-wide grid in prepare_wave order that is a line of the grid,
-    allowed with a 0.01 mm screen (64 pixels = 2.5 um); with a 0.04 mm screen
-    the 10 um step leaves the phase 3e-7 rev off the line within four slots
-    (the bound refuses it).  On a 96-wide grid the step wraps rows, and a
-    shuffled list has no line at all (both refused).  At L = 100 mm the
-    separable amplitude, and with it the line, is refused; at P = 2 there is
-    no line variant."""
+@pytest.mark.parametrize("P", [4, 8])
+def test_slot_line_with_float32_offsets(ctx, monkeypatch, P):
+    """kirchhoff_gpu's float32 pixels: a grid in prepare_wave order, each
+    point off the line by float32 rounding (~3e-9 mm, 7e-6 rev of phase at
+    the aperture's edge).  The line form takes each slot's offset into the
+    phase to first order; against the per-slot form (the offsets exact) and
+    the FP64 kernel on the same float32 inputs."""
     rays = W.gaussian_source(65536, seed=2)
-    for side, half, line in ((256, 0.005, True), (256, 0.02, False), (96, 0.02, False)):
+    g = grid(W.screen(1000.0, 0.02, 256))
+    px, py, pz = xrtk.prepare_wave(g)
+    r32 = xrtk.gpu_rays(rays["x"].shape[0])
+    for f in xrtk.RAY_FIELDS:
+        r32[f] = rays[f]
+    p32 = xrtk.gpu_pixels(px, py, pz)
+    line = xrtk.kirchhoff_gpu(ctx, r32, p32, pixels_per_lane=P)
+    flags = ctx.last_plan()["inst_flags"]
+    monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x4000")
+    per_slot = xrtk.kirchhoff_gpu(ctx, r32, p32, pixels_per_lane=P)
+    f2 = ctx.last_plan()["inst_flags"]
+    monkeypatch.delenv("XRTK_FORCE_FLAGS")
+    ref = xrtk.kirchhoff_gpu(ctx, r32, p32, mode="f64")
+    between, e_line, e_ps = rel_l2(line, per_slot), rel_l2(line, ref), rel_l2(per_slot, ref)
+    print(f"P={P}: flags {flags:#x}; line vs per-slot {between:.2e}; vs f64: line {e_line:.2e}, "
+          f"per-slot {e_ps:.2e}")
+    assert flags & VAR_LINE and flags & VAR_LINEDEV, hex(flags)
+    assert not f2 & VAR_LINE, hex(f2)
+    assert between < 3e-6 and e_line < e_ps + 3e-6
+
+
+def test_slot_line_needs_a_line(ctx):
+    """Pixel lists: a lane holds P consecutive list entries.  In
+    prepare_wave order those are neighbours along a row of the grid (a
+    line, allowed) unless a lane straddles the end of a row: rows of 98
+    entries at P = 4 (refused), and a shuffled list has no line at all
+    (refused).  At L = 100 mm the separable amplitude, and with it the line,
+    is refused; at P = 2 there is no line variant."""
+    rays = W.gaussian_source(65536, seed=2)
+    for side, half, line in ((256, 0.005, True), (256, 0.02, True), (96, 0.02, True),
+                             (98, 0.02, False)):
         g = grid(W.screen(1000.0, half, side))
         px, py, pz = xrtk.prepare_wave(g)
         got = xrtk.diffraction_integral(rays, (px, py, pz), ctx=ctx, pixels_per_lane=4)

@@ -147,7 +147,7 @@ static_assert(XRTK_VAR_CHECK == INST_F_CHECK && XRTK_VAR_TILE == INST_F_TILE &&
                   XRTK_VAR_PATHUP == INST_F_PATHUP && XRTK_VAR_UNIFORM_N == INST_F_UNIFORM_N &&
                   XRTK_VAR_NOFRACT == INST_F_NOFRACT && XRTK_VAR_LIN2 == INST_F_LIN2 &&
                   XRTK_VAR_PAX == INST_F_PAX && XRTK_VAR_SEPAMP == INST_F_SEPAMP &&
-                  XRTK_VAR_LINE == INST_F_LINE,
+                  XRTK_VAR_LINE == INST_F_LINE && XRTK_VAR_LINEDEV == INST_F_LINEDEV,
               "variant flags of the ABI match the device planner's");
 
 void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
@@ -328,7 +328,9 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
     e = hipMemcpyAsync(S + ray_stage, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice,
                        ctx->stream);
     if (e == hipSuccess)
-      e = launch_pix_from_f32aos(S + ray_stage, ctx->pix64.as<PixF64>(), np, npad, ctx->stream);
+      e = launch_pix_from_f32aos(S + ray_stage, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(),
+                                 np, npad, (uint32_t)c32.P, ctx->stream);
+    d_perm = ctx->perm.as<uint32_t>();
   } else {
     double* P3 = reinterpret_cast<double*>(S + ray_stage);
     const size_t cb = sizeof(double) * np;
@@ -337,7 +339,9 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
     if (e == hipSuccess)
       e = hipMemcpyAsync(P3 + 2 * np, ps.z, cb, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-      e = launch_pix_from_cols(P3, ctx->pix64.as<PixF64>(), np, npad, ctx->stream);
+      e = launch_pix_from_cols(P3, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(), np, npad,
+                               (uint32_t)c32.P, ctx->stream);
+    d_perm = ctx->perm.as<uint32_t>();
   }
   if (e != hipSuccess) return map_err(e);
   // ---- instances

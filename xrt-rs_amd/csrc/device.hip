@@ -206,22 +206,37 @@ __global__ void k_rays_from_cols(const double* __restrict__ cols,
   dst[i] = r;
 }
 
-// Explicit pixel lists keep their order; slots [n, npad) repeat pixel n-1.
-__global__ void k_pix_from_f32aos(const float4* __restrict__ src,
-                                  PixF64* __restrict__ dst, uint32_t n, uint32_t npad) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npad) return;
-  const float4 p = src[i < n ? i : n - 1u];
-  dst[i] = PixF64{(double)p.x, (double)p.y, (double)p.z, 0.0};
+// Explicit pixel lists: within each tile of 64 P slots, lane l's P slots
+// (slot t*64P + i*64 + l) take the list entries t*64P + l*P + i, so that a
+// lane holds P consecutive entries (neighbours along a row of a grid-ordered
+// list); perm[slot] is the entry, or ~0u for a padding slot, which repeats
+// entry n-1.
+__device__ __forceinline__ uint32_t list_entry(uint32_t s, uint32_t P) {
+  const uint32_t tsz = 64u * P, t = s / tsz, r = s - t * tsz;
+  return t * tsz + (r & 63u) * P + (r >> 6);
 }
 
-__global__ void k_pix_from_cols(const double* __restrict__ cols,
-                                PixF64* __restrict__ dst, uint32_t n, uint32_t npad) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npad) return;
-  const uint32_t j = i < n ? i : n - 1u;
+__global__ void k_pix_from_f32aos(const float4* __restrict__ src, PixF64* __restrict__ dst,
+                                  uint32_t* __restrict__ perm, uint32_t n, uint32_t npad,
+                                  uint32_t P) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  const uint32_t j = list_entry(s, P);
+  const float4 p = src[j < n ? j : n - 1u];
+  dst[s] = PixF64{(double)p.x, (double)p.y, (double)p.z, 0.0};
+  perm[s] = j < n ? j : ~0u;
+}
+
+__global__ void k_pix_from_cols(const double* __restrict__ cols, PixF64* __restrict__ dst,
+                                uint32_t* __restrict__ perm, uint32_t n, uint32_t npad,
+                                uint32_t P) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  const uint32_t e = list_entry(s, P);
+  const uint32_t j = e < n ? e : n - 1u;
   const size_t N = n;
-  dst[i] = PixF64{cols[j], cols[N + j], cols[2 * N + j], 0.0};
+  dst[s] = PixF64{cols[j], cols[N + j], cols[2 * N + j], 0.0};
+  perm[s] = e < n ? e : ~0u;
 }
 
 // prepare_wave (src/waves/prepare_wave.rs:40-68), evaluated in the same
@@ -552,8 +567,14 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //                       + 15/8 kr rho^2 |h|^2 / R,
 //          R = min |q|, s = sigma + (P-1) |h| / R bounds |q^.h^| along the line.
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
+//          LINEDEV: the same with the slots' offsets delta_i = q_i - q_0 - i h
+//          from the line (a pixel list rounded to float32) taken into the
+//          phase to first order, kr (d^ - q^).delta_i ~ o.w_i with
+//          w_i = -kr (delta_i - q^ (q^.delta_i)) / |q_i|; what is left is
+//          bounded by kr sqrt(3) max|delta| (2 |o| / min|q|)^2, and the first-
+//          order rotation by that angle needs it below 1e-4 rev.
 // 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP, 0x2000
-// forbids LINE.
+// forbids LINE, 0x4000 forbids LINEDEV.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags, uint32_t P) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -609,8 +630,14 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         const double d2 = 2.0 * kr * om * h2 * s / (R * R) + 0.5 * kr * rho * h2 * (1.0 + 2.0 * s * s) / R +
                           kr * om * om * h2 / (R * R * R) + 3.0 * kr * rho * om * h2 * s / (R * R) +
                           1.875 * kr * rho * rho * h2 / R;
-        const double eta = 0.5 * (P - 1.0) * (P - 2.0) * d2 + kr * dv * 1.7320508075688772 * 2.0 * om / R;
-        if (eta <= 1e-7) f |= INST_F_LINE;
+        const double bend = 0.5 * (P - 1.0) * (P - 2.0) * d2;
+        const double dev1 = kr * dv * 1.7320508075688772 * 2.0 * om / R;  // first order
+        const double dev2 = dev1 * 2.0 * om / R;                           // after it
+        if (bend + dev1 <= 1e-7) {
+          f |= INST_F_LINE;
+        } else if (bend + dev2 <= 1e-7 && dev1 <= 1e-4 && !(force_flags & 0x4000u)) {
+          f |= INST_F_LINE | INST_F_LINEDEV;
+        }
       }
     }
   }
@@ -823,9 +850,13 @@ struct LineRegs {
 // magnitude of slots 0 and 1).
 // Per ray and lane: 15 FP32 operations + 2 sin + 2 cos, 5 for slot 1, 2 per
 // further slot, and 6 per slot (sums).
-template <int P, bool NOFRACT, bool LIN2>
+// DEV: slot i lies off the line by its own offset; its phase gains
+// o.w_i radians (w_i from the item prologue), applied to the slot's
+// e^{i phase} to first order: 4 more operations per slot (3 with LIN2).
+template <int P, bool NOFRACT, bool LIN2, bool DEV = false>
 __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
-                                          const LineRegs& L, float h, float (&a)[P][4]) {
+                                          const LineRegs& L, float h, float (&a)[P][4],
+                                          const float (&dw)[P][3]) {
   float lin = r.ox * L.mx;
   lin = fmaf(r.oy, L.my, lin);
   if constexpr (!LIN2) lin = fmaf(r.oz, L.mz, lin);
@@ -856,17 +887,25 @@ __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
       cp = c; sp = s;
       c = cn; s = sn;
     }
-    a[i][0] = fmaf(r.cs, c, a[i][0]);
-    a[i][1] = fmaf(r.cs, s, a[i][1]);
-    a[i][2] = fmaf(r.pr, c, a[i][2]);
-    a[i][2] = fmaf(-r.pi, s, a[i][2]);
-    a[i][3] = fmaf(r.pr, s, a[i][3]);
-    a[i][3] = fmaf(r.pi, c, a[i][3]);
+    float cc = c, ss = s;
+    if constexpr (DEV) {
+      float th = r.ox * dw[i][0];
+      th = fmaf(r.oy, dw[i][1], th);
+      if constexpr (!LIN2) th = fmaf(r.oz, dw[i][2], th);
+      cc = fmaf(-th, s, c);
+      ss = fmaf(th, c, s);
+    }
+    a[i][0] = fmaf(r.cs, cc, a[i][0]);
+    a[i][1] = fmaf(r.cs, ss, a[i][1]);
+    a[i][2] = fmaf(r.pr, cc, a[i][2]);
+    a[i][2] = fmaf(-r.pi, ss, a[i][2]);
+    a[i][3] = fmaf(r.pr, ss, a[i][3]);
+    a[i][3] = fmaf(r.pi, cc, a[i][3]);
   }
 }
 
 template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
-          bool LINE = false>
+          bool LINE = false, bool DEV = false>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   const uint32_t npad = I.npad;
@@ -913,6 +952,35 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     L.dc4 = (float)(c41 - c40);
     L.c4n = (float)c41;
   }
+  // DEV: each slot's offset from the line and its phase gradient (radians
+  // per unit o, kernel axis order): w_i = -2 pi kr (d_i - q^ (q^.d_i)) / |q_i|
+  float dw[P][3];
+#pragma unroll
+  for (int i = 0; i < P; ++i) dw[i][0] = dw[i][1] = dw[i][2] = 0.f;
+  if constexpr (LINE && DEV) {
+    const InstScratch* sc = (const InstScratch*)I.scr;
+    const uint32_t s0 = base + lane;
+    const PixF64 p0 = ((const XRTK_CONST PixF64*)I.pix64)[s0];
+    const PixF64 p1 = ((const XRTK_CONST PixF64*)I.pix64)[s0 + 64u];
+    const double hv[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+    int ax[3];
+    kernel_axes(sc, ax);
+#pragma unroll
+    for (int i = 2; i < P; ++i) {
+      const uint32_t si = s0 + 64u * i;
+      const PixF64 pi = ((const XRTK_CONST PixF64*)I.pix64)[si];
+      const double d[3] = {(pi.x - p0.x) - i * hv[0], (pi.y - p0.y) - i * hv[1],
+                           (pi.z - p0.z) - i * hv[2]};
+      const double R = ((const XRTK_CONST double*)I.R64)[2 * si];
+      const double q[3] = {(pi.x - sc->cx) / R, (pi.y - sc->cy) / R, (pi.z - sc->cz) / R};
+      const double qd = q[0] * d[0] + q[1] * d[1] + q[2] * d[2];
+      const double f = -2.0 * kPI * kr / R;
+      const double w[3] = {f * (d[0] - q[0] * qd), f * (d[1] - q[1] * qd), f * (d[2] - q[2] * qd)};
+      dw[i][0] = (float)pick3(w, ax[0]);
+      dw[i][1] = (float)pick3(w, ax[1]);
+      dw[i][2] = (float)pick3(w, ax[2]);
+    }
+  }
   // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
   // partials are summed in FP64 by k_merge.
   float a[P][4];
@@ -956,7 +1024,7 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
           const TileRay tr = my[jj];
-          pair_line<P, NOFRACT, LIN2>(r, tr.B, tr.off, L, h, a);
+          pair_line<P, NOFRACT, LIN2, DEV>(r, tr.B, tr.off, L, h, a, dw);
         }
       } else {
 #pragma unroll 2
@@ -988,6 +1056,15 @@ template <int P>
 __device__ __forceinline__ bool line_variant(uint32_t f) {
   constexpr uint32_t need = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
                             INST_F_LINE;
+  return P >= 4 && (f & need) == need && !(f & INST_F_LINEDEV);
+}
+
+// The slot-line form with per-slot offsets (run by the other kernel, whose
+// register budget already holds P slot records).
+template <int P>
+__device__ __forceinline__ bool linedev_variant(uint32_t f) {
+  constexpr uint32_t need = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
+                            INST_F_LINE | INST_F_LINEDEV;
   return P >= 4 && (f & need) == need;
 }
 
@@ -1007,6 +1084,18 @@ __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const 
       case 3: f32r_item<P, W, true, true, true, true, true, true, true>(I, w, lds, lane, wave); break;
     }
     return;
+  }
+  if constexpr (P >= 4) {
+    if (linedev_variant<P>(f)) {
+      const uint32_t v = ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+      switch (v) {
+        case 0: f32r_item<P, W, true, true, false, false, true, true, true, true>(I, w, lds, lane, wave); break;
+        case 1: f32r_item<P, W, true, true, false, true, true, true, true, true>(I, w, lds, lane, wave); break;
+        case 2: f32r_item<P, W, true, true, true, false, true, true, true, true>(I, w, lds, lane, wave); break;
+        case 3: f32r_item<P, W, true, true, true, true, true, true, true, true>(I, w, lds, lane, wave); break;
+      }
+      return;
+    }
   }
   const bool tile = (f & INST_F_TILE) != 0u;
   const bool pax = tile && (f & INST_F_PAX);
@@ -1255,18 +1344,20 @@ hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, 
   hipLaunchKernelGGL(k_rays_from_cols, dim3(cdiv(n, 256)), dim3(256), 0, st, cols, dst, n);
   return hipGetLastError();
 }
-hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, uint32_t npad,
-                                  hipStream_t st) {
+hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t* perm, uint32_t n,
+                                  uint32_t npad, uint32_t P, hipStream_t st) {
   if (!n) return hipSuccess;
+  if (npad % (64u * P)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_pix_from_f32aos, dim3(cdiv(npad, 256)), dim3(256), 0, st,
-                     (const float4*)src, dst, n, npad);
+                     (const float4*)src, dst, perm, n, npad, P);
   return hipGetLastError();
 }
-hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, uint32_t npad,
-                                hipStream_t st) {
+hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t* perm, uint32_t n,
+                                uint32_t npad, uint32_t P, hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_pix_from_cols, dim3(cdiv(npad, 256)), dim3(256), 0, st, cols, dst, n,
-                     npad);
+  if (npad % (64u * P)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pix_from_cols, dim3(cdiv(npad, 256)), dim3(256), 0, st, cols, dst, perm,
+                     n, npad, P);
   return hipGetLastError();
 }
 hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,

@@ -131,6 +131,8 @@ enum : uint32_t {
   INST_F_LINE = 256u,     // F32R SEPAMP variant: each lane's slots lie on a line, the
                           // phase steps along it by one rotation per slot
                           // (bound <= 1e-7 rev)
+  INST_F_LINEDEV = 1024u, // LINE with each slot's own offset from the line applied
+                          // to first order (pixel lists off a line by rounding)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };

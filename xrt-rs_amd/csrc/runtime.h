@@ -21,10 +21,12 @@ namespace xrtk {
 // ---- device launchers (device.hip) ----
 hipError_t launch_rays_from_f32aos(const float* src, RayRec64* dst, uint32_t n, hipStream_t st);
 hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, hipStream_t st);
-hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t n, uint32_t npad,
-                                  hipStream_t st);
-hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t n, uint32_t npad,
-                                hipStream_t st);
+// Explicit pixel lists: slots in tiles of 64 P, each lane holding P
+// consecutive list entries; perm[slot] = entry or ~0u (padding).
+hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t* perm, uint32_t n,
+                                  uint32_t npad, uint32_t P, hipStream_t st);
+hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t* perm, uint32_t n,
+                                uint32_t npad, uint32_t P, hipStream_t st);
 hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,
                            uint32_t th, uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
                            hipStream_t st);
