@@ -334,9 +334,10 @@ typedef struct XrtkServerCfg {
                                   (xrtk_autotune_jobs)                      */
 } XrtkServerCfg;
 
-/* Which requests fill a launch.  Every active request with work left gets
- * at least one work item in every launch under either policy; the results
- * do not depend on it. */
+/* Which requests fill a launch, and which queued request is admitted next
+ * (the one with the fewest planned pairs, or the oldest).  Every active
+ * request with work left gets at least one work item in every launch under
+ * either policy; the results do not depend on it. */
 #define XRTK_SCHED_SHORTEST 0  /* fewest remaining pairs first (default)    */
 #define XRTK_SCHED_ARRIVAL  1  /* oldest request first                      */
 

@@ -222,3 +222,32 @@ def test_invalid_policy_rejected(ctx):
     h = ctypes.c_void_p()
     rc = xrtk._lib.lib.xrtk_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h))
     assert rc == xrtk._lib.EINVAL and not h.value
+
+
+@pytest.mark.parametrize("policy", ["shortest", "arrival"])
+def test_admission_follows_policy(ctx, sets, policy):
+    """One active request at a time: while the largest runs, a medium and
+    then a small request queue.  Shortest-remaining-first admits the small
+    one first, arrival order the medium one; both equal the standalone
+    solve."""
+    specs, rsets = sets
+    jobs = sorted(jobs_of(specs, rsets), key=lambda j: j.rays.n * j.screen.nx * j.screen.nz)
+    small, medium, big = jobs[0], jobs[len(jobs) // 2], jobs[-1]
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=1, quantum_ms=50.0,
+                      policy=policy)
+    try:
+        tb = srv.submit(big)
+        tm = srv.submit(medium)
+        tsm = srv.submit(small)
+        outs = {k: srv.wait(t, timeout_ms=60000) for k, t in
+                (("big", tb), ("medium", tm), ("small", tsm))}
+        assert srv.stats()["max_concurrent"] == 1
+    finally:
+        srv.close()
+    for k, j in (("big", big), ("medium", medium), ("small", small)):
+        assert np.array_equal(outs[k][0], xrtk.solve_jobs(ctx, [j])[0]), k
+    t_small, t_medium = outs["small"][1], outs["medium"][1]
+    if policy == "shortest":
+        assert t_small[1] < t_medium[1] and t_small[3] < t_medium[3]
+    else:
+        assert t_medium[1] < t_small[1] and t_medium[3] < t_small[3]

@@ -4,8 +4,10 @@
 // one energy on one prepare_wave screen) queue on submit.  Each scheduler
 // iteration
 //   1. admits queued requests while the HBM budget and the active-request
-//      cap allow (their buffers come from a caching pool; their grid and
-//      prep kernels are enqueued at once),
+//      cap allow, the one with the fewest pairs first (oldest first under
+//      XRTK_SCHED_ARRIVAL), so that under overload small requests pass large
+//      queued ones; their buffers come from a caching pool, their grid and
+//      prep kernels are enqueued at once,
 //   2. fills one "quantum" launch of the persistent pair kernel with work
 //      items of the active requests, shortest remaining work first (the
 //      request with the fewest pairs left fills the budget first; or oldest
@@ -31,7 +33,6 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -162,7 +163,9 @@ struct xrtk_server {
   std::condition_variable cv_work, cv_done;
   bool stop = false;
   uint64_t next_id = 1;
-  std::deque<Req*> queue;               // submitted, not admitted
+  // submitted, not admitted, in admission order: key (planned pairs, id)
+  // under shortest-remaining-first, (0, id) in arrival order
+  std::map<std::pair<double, uint64_t>, Req*> queue;
   std::vector<Req*> active;             // admitted, items left or merging
   std::map<uint64_t, Req*> all;         // every request not yet collected
   Pool dpool, hpool;
@@ -293,7 +296,7 @@ struct xrtk_server {
 
   void admit() {
     while (!queue.empty() && (int)active.size() < cfg.max_active) {
-      Req* r = queue.front();
+      Req* r = queue.begin()->second;
       if (hbm_in_use + Pool::bucket(r->bytes) > cfg.hbm_budget_bytes) break;  // wait
       size_t got = 0;
       void* d = dpool.get(r->bytes, cfg.hbm_budget_bytes, &got);
@@ -305,7 +308,7 @@ struct xrtk_server {
         dpool.put(d, got);
         break;
       }
-      queue.pop_front();
+      queue.erase(queue.begin());
       r->dblk = d; r->dblk_size = got; r->hblk = h; r->hblk_size = hgot;
       hbm_in_use += got;
       hbm_peak = std::max(hbm_peak, hbm_in_use);
@@ -641,7 +644,9 @@ int xrtk_server_submit(xrtk_server* s, const XrtkJob* job, xrtk_ticket* ticket) 
     *ticket = r->id;
     Req* raw = r.release();
     s->all[raw->id] = raw;
-    s->queue.push_back(raw);
+    const double work =
+        s->cfg.policy == XRTK_SCHED_SHORTEST ? (double)raw->n_items * raw->item_pairs : 0.0;
+    s->queue.emplace(std::make_pair(work, raw->id), raw);
     s->cv_work.notify_all();
     return XRTK_OK;
   } catch (...) {
