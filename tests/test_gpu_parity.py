@@ -437,13 +437,15 @@ def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
     assert e_line < STRESS_BAR and e_line < e_ps + 3e-6
 
 
-@pytest.mark.parametrize("P", [4, 8])
+@pytest.mark.parametrize("P", [4, 8, 16])
 def test_slot_line_with_float32_offsets(ctx, monkeypatch, P):
     """kirchhoff_gpu's float32 pixels: a grid in prepare_wave order, each
-    point off the line by float32 rounding (~3e-9 mm, 7e-6 rev of phase at
-    the aperture's edge).  The line form takes each slot's offset into the
-    phase to first order; against the per-slot form (the offsets exact) and
-    the FP64 kernel on the same float32 inputs."""
+    point off the lane's line (the secant to its last slot) by float32
+    rounding (~2e-9 mm, ~1e-5 rev of phase at the aperture's edge).  The
+    line form takes each slot's offset into the phase to first order (at
+    P = 16 in two passes of 8 slots, each anchored at its first slot);
+    against the per-slot form (the offsets exact) and the FP64 kernel on the
+    same float32 inputs."""
     rays = W.gaussian_source(65536, seed=2)
     g = grid(W.screen(1000.0, 0.02, 256))
     px, py, pz = xrtk.prepare_wave(g)

@@ -39,7 +39,7 @@ def test_server_results_equal_standalone(ctx, sets):
     specs, rsets = sets
     jobs = jobs_of(specs, rsets)
     alone = [xrtk.solve_jobs(ctx, [j])[0] for j in jobs]
-    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=50.0)
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, quantum_ms=100.0)
     try:
         tickets = [srv.submit(j) for j in jobs]
         for t, a in zip(tickets, alone):
@@ -233,7 +233,7 @@ def test_admission_follows_policy(ctx, sets, policy):
     specs, rsets = sets
     jobs = sorted(jobs_of(specs, rsets), key=lambda j: j.rays.n * j.screen.nx * j.screen.nz)
     small, medium, big = jobs[0], jobs[len(jobs) // 2], jobs[-1]
-    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=1, quantum_ms=50.0,
+    srv = xrtk.Server(ctx, hbm_budget_bytes=8 << 30, max_active=1, quantum_ms=100.0,
                       policy=policy)
     try:
         tb = srv.submit(big)

@@ -231,7 +231,9 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
   const uint32_t want = mode_from(opts);
   const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
   // the F32R shape fixes the slot layout even when only the FP64 kernel runs
-  LaunchCfg c32 = choose_cfg(ctx, MODE_F32R, nr, np, want == MODE_F32R ? opts : nullptr);
+  const bool f32_pixels = ps.f32 != nullptr;
+  LaunchCfg c32 =
+      choose_cfg(ctx, MODE_F32R, nr, np, want == MODE_F32R ? opts : nullptr, f32_pixels);
   LaunchCfg c64 = cfg64_for(c32);
   if (want == MODE_F64) c64 = choose_cfg(ctx, MODE_F64, nr, np, opts);
   if (!pair_cfg_supported(MODE_F32R, c32) || !pair_cfg_supported(MODE_F64, c64))

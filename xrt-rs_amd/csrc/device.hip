@@ -391,6 +391,19 @@ __device__ __forceinline__ double nl_ref(const InstScratch* scr) {
 // doubles) and the tile record, and reduces the bounds of |q|, |dq|, R and
 // of the lanes' slot lines (real slots only: perm[s] != ~0u, or s < np
 // without perm; a lane's real slots are a prefix of its P slots).
+// Last slot of a lane that holds a pixel (0 if no slot past slot 0 does);
+// s0 is the lane's slot 0.  The lane's slot line runs from slot 0 to it.
+template <int P>
+__device__ __forceinline__ int line_last(const uint32_t* __restrict__ perm, uint32_t np,
+                                         uint32_t s0) {
+#pragma unroll 1
+  for (int i = P - 1; i >= 1; --i) {
+    const uint32_t s = s0 + i * 64u;
+    if (perm ? perm[s] != ~0u : s < np) return i;
+  }
+  return 0;
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix, double kr,
                                                   InstScratch* __restrict__ scr,
@@ -474,23 +487,32 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
   }
   rmin = wave_min(rmin); rmax = wave_max(rmax);
   clo = wave_min(clo); chi = wave_max(chi);
-  // the lane's slot line q_i = q_0 + i h
+  // the lane's slot line q_i = q_0 + i h, h the secant to the lane's last
+  // real slot (so rounded positions depart from it by their own rounding,
+  // not by the rounding of one step times i)
   float hm = 0.f, sg = 0.f, dv = 0.f;
   auto real = [&](int i) {
     const uint32_t s = base + i * 64u + lane;
     return perm ? perm[s] != ~0u : s < np;
   };
-  if (P >= 2 && real(0) && real(1)) {
+  const int kl = line_last<P>(perm, np, base + lane);
+  if (P >= 2 && real(0) && kl >= 1) {
+    const PixF64 pk = pix[base + kl * 64u + lane];
+    const double pkv[3] = {pk.x, pk.y, pk.z};
     double h[3];
-    for (int k = 0; k < 3; ++k) h[k] = (qh[1][k] - qh[0][k]) + (ql[1][k] - ql[0][k]);
+    for (int k = 0; k < 3; ++k) {
+      double qkh, qkl;
+      two_sum(pkv[k], -c[k], &qkh, &qkl);
+      h[k] = ((qkh - qh[0][k]) + (qkl - ql[0][k])) / (double)kl;
+    }
     const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
     const double r0 = sqrt(qh[0][0] * qh[0][0] + qh[0][1] * qh[0][1] + qh[0][2] * qh[0][2]);
     hm = __double2float_ru(hn);
     if (hn > 0.0 && r0 > 0.0)
       sg = __double2float_ru(fabs(qh[0][0] * h[0] + qh[0][1] * h[1] + qh[0][2] * h[2]) / (r0 * hn));
 #pragma unroll
-    for (int i = 2; i < P; ++i) {
-      if (!real(i)) break;
+    for (int i = 1; i < P; ++i) {
+      if (i > kl || !real(i)) continue;
       for (int k = 0; k < 3; ++k) {
         const double d = ((qh[i][k] - qh[0][k]) + (ql[i][k] - ql[0][k])) - (double)i * h[k];
         dv = fmaxf(dv, __double2float_ru(fabs(d)));
@@ -556,23 +578,26 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //            - n.o (1/|q| - 1/R_ref) - (nl - nl_ref - n.o/R_ref) (C - C_ref) / (nl_ref + C_ref),
 //          bounded term by term from the ranges of nl, C, |q| and max |n.o|.
 //          LINE (SEPAMP, P >= 4) when every lane's real slots lie on a line
-//          q_i = q_0 + i h and the ray's reduced phase along it,
+//          q_i = q_0 + i h (h the secant from slot 0 to the lane's last real
+//          slot) and the ray's reduced phase along it,
 //            psi(t) = n(t) (r0(t) - n(t) c4(t)),  n(t) = kr (|o|^2 - 2 o.q(t)),
 //            r0 = 1/(2|q|), c4 = 1/(8 kr |q|^3)  (the PAX form),
-//          departs from the line through slots 0 and 1 by at most
+//          departs from its linear extrapolation from q_0 and q_0 + h by at most
 //            (P-1)(P-2)/2 max |psi''| + kr |q_i - q_0 - i h| 2 |o| / min|q|
 //          <= 1e-7 rev, with |n'| <= 2 kr |o| |h|, |q|' <= sigma |h|, |q|'' <= |h|^2 / |q|:
 //            |psi''| <= 2 kr |o| |h|^2 s / R^2 + kr rho |h|^2 (1 + 2 s^2) / (2 R)
 //                       + kr |o|^2 |h|^2 / R^3 + 3 kr rho |o| |h|^2 s / R^2
 //                       + 15/8 kr rho^2 |h|^2 / R,
 //          R = min |q|, s = sigma + (P-1) |h| / R bounds |q^.h^| along the line.
-// force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
+//          (A float32 pixel list departs from the secant by its own rounding,
+//          ~2e-9 mm, a few 1e-8 rev at 1 m and 10 keV.)
 //          LINEDEV: the same with the slots' offsets delta_i = q_i - q_0 - i h
-//          from the line (a pixel list rounded to float32) taken into the
-//          phase to first order, kr (d^ - q^).delta_i ~ o.w_i with
+//          from the line (a list whose points stray further from it) taken
+//          into the phase to first order, kr (d^ - q^).delta_i ~ o.w_i with
 //          w_i = -kr (delta_i - q^ (q^.delta_i)) / |q_i|; what is left is
 //          bounded by kr sqrt(3) max|delta| (2 |o| / min|q|)^2, and the first-
 //          order rotation by that angle needs it below 1e-4 rev.
+// force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
 // 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP, 0x2000
 // forbids LINE, 0x4000 forbids LINEDEV.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
@@ -633,9 +658,13 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         const double bend = 0.5 * (P - 1.0) * (P - 2.0) * d2;
         const double dev1 = kr * dv * 1.7320508075688772 * 2.0 * om / R;  // first order
         const double dev2 = dev1 * 2.0 * om / R;                           // after it
+        // LINEDEV runs in passes of at most 8 slots, each anchored at its
+        // first slot: past P = 8 the offsets from an anchor's line reach
+        // twice the offsets from the lane's line
+        const double fd = P > 8u ? 2.0 : 1.0;
         if (bend + dev1 <= 1e-7) {
           f |= INST_F_LINE;
-        } else if (bend + dev2 <= 1e-7 && dev1 <= 1e-4 && !(force_flags & 0x4000u)) {
+        } else if (bend + fd * dev2 <= 1e-7 && fd * dev1 <= 1e-4 && !(force_flags & 0x4000u)) {
           f |= INST_F_LINE | INST_F_LINEDEV;
         }
       }
@@ -831,16 +860,17 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
   }
 }
 
-// Per-lane constants of the LINE variant: slot 0's record and the step to
-// slot 1 (m_1 - m_0 = -2 kr h, r0_1 - r0_0, c4_1 - c4_0, c4_1), from FP64.
+// Per-lane constants of the LINE variant: the anchor slot's record and the
+// step along the lane's line to the point q + h after it (-2 kr h, and r0,
+// c4 there minus at the anchor, c4 there), from FP64.
 struct LineRegs {
   float mx, my, mz, g, r0, c4;
   float hx, hy, hz, dr0, dc4, c4n;
 };
 
 // Accumulates one ray into the P slots of a lane on a slot line (TILE, UNIN,
-// PAX, SEP): the reduced phase at slot 0 as pair_ray computes it, the step
-// to the next slot
+// PAX, SEP): the reduced phase at slot 0 (the anchor) as pair_ray computes
+// it, the step to the next point of the line
 //   psi_1 - psi_0 = dn rden_1 + num_0 (rden_1 - rden_0),  dn = o.(m_1 - m_0),
 //   rden_1 - rden_0 = (r0_1 - r0_0) - dn c4_1 - num_0 (c4_1 - c4_0),
 // evaluated directly (small, so to full FP32 relative precision); slot 1
@@ -904,25 +934,28 @@ __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
   }
 }
 
-template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
-          bool LINE = false, bool DEV = false>
-__device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
-                                          uint32_t lane, uint32_t wave) {
+// Slots first .. first + PH - 1 of each lane of one work item (the wave's
+// tile holds P slots per lane).
+template <int P, int PH, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX,
+          bool SEP, bool LINE, bool DEV>
+__device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, TileRay* lds,
+                                          uint32_t lane, uint32_t wave, uint32_t first) {
   const uint32_t npad = I.npad;
   const uint32_t base = w.pix0 + wave * 64u * P;  // first slot of this wave's tile
+  const uint32_t sbase = base + first * 64u;      // first slot of this pass
   const double kr = I.kr_hi;
   const TileRec T = ((const XRTK_CONST TileRec*)I.tiles)[base / (64u * P)];
   const float ikr = (float)(1.0 / kr);
   const float h = (float)(1.0 / (2.0 * T.R * kr));
-  SlotRegs<P> s;
+  SlotRegs<PH> s;
   float mtx = 0.f, mty = 0.f, mtz = 0.f;
   if constexpr (!TILE) {
     mtx = (float)(-2.0 * kr * T.qx); mty = (float)(-2.0 * kr * T.qy);
     mtz = (float)(-2.0 * kr * T.qz);
   }
 #pragma unroll
-  for (int i = 0; i < (LINE ? 1 : P); ++i) {  // the line form reads slot 0 only
-    const PixRec32 pr = I.pix32[base + i * 64u + lane];
+  for (int i = 0; i < (LINE ? 1 : PH); ++i) {  // the line form reads slot 0 only
+    const PixRec32 pr = I.pix32[sbase + i * 64u + lane];
     s.mx[i] = TILE ? pr.mx : pr.mx + mtx;
     s.my[i] = TILE ? pr.my : pr.my + mty;
     s.mz[i] = TILE ? pr.mz : pr.mz + mtz;
@@ -930,43 +963,54 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
     s.r0[i] = pr.r0; s.c4[i] = pr.c4; s.rinv[i] = pr.rinv; s.c2[i] = pr.c2;
   }
   LineRegs L{};
+  double hv[3] = {0.0, 0.0, 0.0};  // the lane's step along its slot line
   if constexpr (LINE) {
-    // slot 0's record; the step to slot 1 in FP64 from the pixel positions
-    // and |q| (the hi parts tile prep kept)
+    // the line: the secant from the lane's slot 0 to its last real slot
+    // (the statistics tile prep bounded), anchored at the pass's first slot
+    // a; slot a's record, and the step from q_a to q_a + h in FP64
     L.mx = s.mx[0]; L.my = s.my[0]; L.mz = s.mz[0];
     L.g = s.g[0]; L.r0 = s.r0[0]; L.c4 = s.c4[0];
-    const uint32_t s0 = base + lane, s1 = s0 + 64u;
-    const PixF64 p0 = ((const XRTK_CONST PixF64*)I.pix64)[s0];
-    const PixF64 p1 = ((const XRTK_CONST PixF64*)I.pix64)[s1];
-    const double hv[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+    const XRTK_CONST PixF64* px = (const XRTK_CONST PixF64*)I.pix64;
+    const InstScratch* sc = (const InstScratch*)I.scr;
+    const uint32_t l0 = base + lane;
+    const int kl = line_last<P>(I.perm, I.np, l0);
+    const PixF64 p0 = px[l0];
+    const PixF64 pk = px[l0 + 64u * (uint32_t)(kl > 0 ? kl : 1)];
+    const double dk = kl > 0 ? (double)kl : 1.0;
+    hv[0] = (pk.x - p0.x) / dk; hv[1] = (pk.y - p0.y) / dk; hv[2] = (pk.z - p0.z) / dk;
     int ax[3];
-    kernel_axes((const InstScratch*)I.scr, ax);
+    kernel_axes(sc, ax);
     L.hx = (float)(-2.0 * kr * pick3(hv, ax[0]));
     L.hy = (float)(-2.0 * kr * pick3(hv, ax[1]));
     L.hz = (float)(-2.0 * kr * pick3(hv, ax[2]));
-    const double R0 = ((const XRTK_CONST double*)I.R64)[2 * s0];
-    const double R1 = ((const XRTK_CONST double*)I.R64)[2 * s1];
-    const double i0 = 1.0 / R0, i1 = 1.0 / R1;
-    const double c40 = 0.125 * (i0 * i0 * i0 / kr), c41 = 0.125 * (i1 * i1 * i1 / kr);
-    L.dr0 = (float)(0.5 * i1 - 0.5 * i0);
-    L.dc4 = (float)(c41 - c40);
-    L.c4n = (float)c41;
+    // r0 = 1/(2|q|) and c4 = 1/(8 kr |q|^3) at q_a + h minus at q_a, from
+    // |q_a + h|^2 - |q_a|^2 = 2 q_a.h + |h|^2 without cancellation
+    const uint32_t sa = sbase + lane;
+    const PixF64 pa = px[sa];
+    const double qa[3] = {pa.x - sc->cx, pa.y - sc->cy, pa.z - sc->cz};
+    const double Ra = ((const XRTK_CONST double*)I.R64)[2 * sa];
+    const double t = 2.0 * (qa[0] * hv[0] + qa[1] * hv[1] + qa[2] * hv[2]) +
+                     (hv[0] * hv[0] + hv[1] * hv[1] + hv[2] * hv[2]);
+    const double Rb = sqrt(Ra * Ra + t);
+    const double ia = 1.0 / Ra, ib = 1.0 / Rb;
+    const double di = -(t / (Ra + Rb)) * ia * ib;  // 1/|q_a + h| - 1/|q_a|
+    L.dr0 = (float)(0.5 * di);
+    L.dc4 = (float)(0.125 / kr * di * (ib * ib + ib * ia + ia * ia));
+    L.c4n = (float)(0.125 * (ib * ib * ib / kr));
   }
   // DEV: each slot's offset from the line and its phase gradient (radians
   // per unit o, kernel axis order): w_i = -2 pi kr (d_i - q^ (q^.d_i)) / |q_i|
-  float dw[P][3];
+  float dw[PH][3];
 #pragma unroll
-  for (int i = 0; i < P; ++i) dw[i][0] = dw[i][1] = dw[i][2] = 0.f;
+  for (int i = 0; i < PH; ++i) dw[i][0] = dw[i][1] = dw[i][2] = 0.f;
   if constexpr (LINE && DEV) {
     const InstScratch* sc = (const InstScratch*)I.scr;
-    const uint32_t s0 = base + lane;
+    const uint32_t s0 = sbase + lane;
     const PixF64 p0 = ((const XRTK_CONST PixF64*)I.pix64)[s0];
-    const PixF64 p1 = ((const XRTK_CONST PixF64*)I.pix64)[s0 + 64u];
-    const double hv[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
     int ax[3];
     kernel_axes(sc, ax);
 #pragma unroll
-    for (int i = 2; i < P; ++i) {
+    for (int i = 1; i < PH; ++i) {
       const uint32_t si = s0 + 64u * i;
       const PixF64 pi = ((const XRTK_CONST PixF64*)I.pix64)[si];
       const double d[3] = {(pi.x - p0.x) - i * hv[0], (pi.y - p0.y) - i * hv[1],
@@ -983,9 +1027,9 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
   }
   // FP32 sums over this item's ray chunk (at most I.tile rays); the chunk
   // partials are summed in FP64 by k_merge.
-  float a[P][4];
+  float a[PH][4];
 #pragma unroll
-  for (int i = 0; i < P; ++i) a[i][0] = a[i][1] = a[i][2] = a[i][3] = 0.f;
+  for (int i = 0; i < PH; ++i) a[i][0] = a[i][1] = a[i][2] = a[i][3] = 0.f;
 
   const XRTK_CONST RayRec32* rays = (const XRTK_CONST RayRec32*)I.rays32;
   const uint32_t r0 = w.chunk * I.chunk_len;
@@ -1024,14 +1068,14 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
           const TileRay tr = my[jj];
-          pair_line<P, NOFRACT, LIN2, DEV>(r, tr.B, tr.off, L, h, a, dw);
+          pair_line<PH, NOFRACT, LIN2, DEV>(r, tr.B, tr.off, L, h, a, dw);
         }
       } else {
 #pragma unroll 2
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
           const TileRay tr = my[jj];
-          pair_ray<P, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+          pair_ray<PH, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
         }
       }
     }
@@ -1039,16 +1083,33 @@ __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, Til
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
-      pair_ray<P, false, UNIN, NOFRACT, LIN2, false, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
+      pair_ray<PH, false, UNIN, NOFRACT, LIN2, false, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
   // (SEP: k_merge applies the slot factor of the amplitude)
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * npad;
 #pragma unroll
-  for (int i = 0; i < P; ++i) {
-    const uint32_t p = base + i * 64u + lane;
+  for (int i = 0; i < PH; ++i) {
+    const uint32_t p = sbase + i * 64u + lane;
     part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
   }
+}
+
+// One work item: the wave's 64 P slots, in one pass for the slot-line form
+// of the lean kernel and in passes of at most 8 slots per lane for the other
+// forms (so that their registers at P = 16 are those of P = 8; each pass
+// reads the ray chunk again).  (The slot line with per-slot offsets in one
+// pass of 16 would need ~256 VGPRs.)
+template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
+          bool LINE = false, bool DEV = false>
+__device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
+                                          uint32_t lane, uint32_t wave) {
+  constexpr int PH = ((LINE && !DEV) || P <= 8) ? P : 8;
+  static_assert(P % PH == 0, "passes must tile the lane's slots");
+#pragma unroll 1
+  for (uint32_t first = 0; first < (uint32_t)P; first += PH)
+    f32r_pass<P, PH, W, TILE, UNIN, NOFRACT, LIN2, PAX, SEP, LINE, DEV>(I, w, lds, lane, wave,
+                                                                         first);
 }
 
 // Whether an instance runs the slot-line form at this P.

@@ -46,7 +46,7 @@ namespace xrtk {
 // not depend on which other jobs share the launch.
 LaunchCfg batch_cfg(const xrtk_ctx* ctx, uint32_t mode, const XrtkOpts* o) {
   LaunchCfg c;
-  if (mode == MODE_F32R) { c.P = 8; c.W = 1; } else { c.P = 1; c.W = 4; }
+  if (mode == MODE_F32R) { c.P = 16; c.W = 1; } else { c.P = 1; c.W = 4; }
   c.ray_src = 1;
   c.persistent = 1;
   c.tile = mode == MODE_F32R ? 8192 : 0;

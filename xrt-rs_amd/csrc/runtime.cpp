@@ -66,13 +66,15 @@ static int ilog2(uint64_t v) {
   return l;
 }
 
-static LaunchCfg default_cfg(uint32_t mode, uint64_t nr, uint64_t np) {
+static LaunchCfg default_cfg(uint32_t mode, uint64_t nr, uint64_t np, bool f32_pixels) {
   LaunchCfg c;
   if (mode == MODE_F32R) {
-    // 8 slots per lane, one wave per group: on 1 M x 512^2 the per-slot
-    // forms run 5 % and the slot-line form 25 % faster than 4 x 4 waves
-    // (16 slots suit only the slot-line form; the autotuners try it)
-    c.P = 8; c.W = 1;
+    // one wave per group; 16 slots per lane: on 1 M x 512^2 the slot-line
+    // form takes 57 ms against 64 ms at 8 slots, and the per-slot forms run
+    // 16 slots as two passes of 8 at 8 slots' speed (107 ms).  float32
+    // pixels sit off the slot line by their rounding: the line with per-slot
+    // offsets takes 85.4 ms at 8 slots, 86.9 ms at 16 (two passes)
+    c.P = f32_pixels ? 8 : 16; c.W = 1;
     if (np < 64ull * 4 * 4 * 64) { c.P = 2; c.W = 4; }
   } else {
     c.P = 1; c.W = 4;
@@ -89,8 +91,9 @@ LaunchCfg cfg64_for(const LaunchCfg& c32) {
   return c;
 }
 
-LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o) {
-  LaunchCfg c = default_cfg(mode, nr, np);
+LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o,
+                     bool f32_pixels) {
+  LaunchCfg c = default_cfg(mode, nr, np, f32_pixels);
   auto it = ctx->tune.find(std::make_tuple((int)mode, ilog2(nr), ilog2(np)));
   if (it != ctx->tune.end()) c = it->second;
   if (o) {

@@ -147,8 +147,11 @@ struct xrtk_ctx {
 
 namespace xrtk {
 
-// Chooses the launch configuration for a solve of nr rays on np pixels.
-LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o);
+// Chooses the launch configuration for a solve of nr rays on np pixels
+// (f32_pixels: a float32 pixel list, which sits off the slot line by its
+// rounding).
+LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o,
+                     bool f32_pixels = false);
 // Default FP64 configuration paired with a given F32R configuration.
 LaunchCfg cfg64_for(const LaunchCfg& c32);
 // Tune-table key (both log2 fields) of the batched-solve configuration.
