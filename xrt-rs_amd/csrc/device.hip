@@ -404,6 +404,42 @@ __device__ __forceinline__ int line_last(const uint32_t* __restrict__ perm, uint
   return 0;
 }
 
+// The lane's slot-line step h: the secant from its slot 0 (slot s0) to its
+// last real slot (to slot 1 when none past slot 0 is real).
+template <int P>
+__device__ __forceinline__ void line_secant(const PixF64* __restrict__ pix,
+                                            const uint32_t* __restrict__ perm, uint32_t np,
+                                            uint32_t s0, double (&hv)[3]) {
+  const int kl = line_last<P>(perm, np, s0);
+  const PixF64 p0 = pix[s0];
+  const PixF64 pk = pix[s0 + 64u * (uint32_t)(kl > 0 ? kl : 1)];
+  const double dk = kl > 0 ? (double)kl : 1.0;
+  hv[0] = (pk.x - p0.x) / dk; hv[1] = (pk.y - p0.y) / dk; hv[2] = (pk.z - p0.z) / dk;
+}
+
+// The line's step constants at the anchor q_a (relative to the centroid,
+// |q_a| = Ra) for the step h: -2 kr h in kernel axis order, and r0 and c4 at
+// q_a + h minus at q_a, from |q_a + h|^2 - |q_a|^2 = 2 q_a.h + |h|^2 without
+// cancellation, and c4 at q_a + h.
+__device__ __forceinline__ LineRec line_step(const double (&qa)[3], double Ra,
+                                             const double (&hv)[3], double kr,
+                                             const int (&ax)[3]) {
+  LineRec r;
+  r.hx = (float)(-2.0 * kr * pick3(hv, ax[0]));
+  r.hy = (float)(-2.0 * kr * pick3(hv, ax[1]));
+  r.hz = (float)(-2.0 * kr * pick3(hv, ax[2]));
+  const double t = 2.0 * (qa[0] * hv[0] + qa[1] * hv[1] + qa[2] * hv[2]) +
+                   (hv[0] * hv[0] + hv[1] * hv[1] + hv[2] * hv[2]);
+  const double Rb = sqrt(Ra * Ra + t);
+  const double ia = 1.0 / Ra, ib = 1.0 / Rb;
+  const double di = -(t / (Ra + Rb)) * ia * ib;  // 1/|q_a + h| - 1/|q_a|
+  r.dr0 = (float)(0.5 * di);
+  r.dc4 = (float)(0.125 / kr * di * (ib * ib + ib * ia + ia * ia));
+  r.c4n = (float)(0.125 * (ib * ib * ib / kr));
+  r.pad0 = r.pad1 = 0.f;
+  return r;
+}
+
 template <int P>
 __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix, double kr,
                                                   InstScratch* __restrict__ scr,
@@ -496,15 +532,16 @@ __global__ __launch_bounds__(64) void k_tile_prep(const PixF64* __restrict__ pix
     return perm ? perm[s] != ~0u : s < np;
   };
   const int kl = line_last<P>(perm, np, base + lane);
+  double h[3];
+  line_secant<P>(pix, perm, np, base + lane, h);
+  {
+    // the lane's line record for the slot-line kernel (anchored at slot 0)
+    const PixF64 p0 = pix[base + lane];
+    const double qa[3] = {p0.x - c[0], p0.y - c[1], p0.z - c[2]};
+    LineRec* lines = reinterpret_cast<LineRec*>(tiles + gridDim.x);
+    lines[blockIdx.x * 64u + lane] = line_step(qa, R64[2 * (base + lane)], h, kr, ax);
+  }
   if (P >= 2 && real(0) && kl >= 1) {
-    const PixF64 pk = pix[base + kl * 64u + lane];
-    const double pkv[3] = {pk.x, pk.y, pk.z};
-    double h[3];
-    for (int k = 0; k < 3; ++k) {
-      double qkh, qkl;
-      two_sum(pkv[k], -c[k], &qkh, &qkl);
-      h[k] = ((qkh - qh[0][k]) + (qkl - ql[0][k])) / (double)kl;
-    }
     const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
     const double r0 = sqrt(qh[0][0] * qh[0][0] + qh[0][1] * qh[0][1] + qh[0][2] * qh[0][2]);
     hm = __double2float_ru(hn);
@@ -967,36 +1004,27 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
   if constexpr (LINE) {
     // the line: the secant from the lane's slot 0 to its last real slot
     // (the statistics tile prep bounded), anchored at the pass's first slot
-    // a; slot a's record, and the step from q_a to q_a + h in FP64
+    // a; slot a's record and the step from q_a to q_a + h
     L.mx = s.mx[0]; L.my = s.my[0]; L.mz = s.mz[0];
     L.g = s.g[0]; L.r0 = s.r0[0]; L.c4 = s.c4[0];
-    const XRTK_CONST PixF64* px = (const XRTK_CONST PixF64*)I.pix64;
-    const InstScratch* sc = (const InstScratch*)I.scr;
-    const uint32_t l0 = base + lane;
-    const int kl = line_last<P>(I.perm, I.np, l0);
-    const PixF64 p0 = px[l0];
-    const PixF64 pk = px[l0 + 64u * (uint32_t)(kl > 0 ? kl : 1)];
-    const double dk = kl > 0 ? (double)kl : 1.0;
-    hv[0] = (pk.x - p0.x) / dk; hv[1] = (pk.y - p0.y) / dk; hv[2] = (pk.z - p0.z) / dk;
-    int ax[3];
-    kernel_axes(sc, ax);
-    L.hx = (float)(-2.0 * kr * pick3(hv, ax[0]));
-    L.hy = (float)(-2.0 * kr * pick3(hv, ax[1]));
-    L.hz = (float)(-2.0 * kr * pick3(hv, ax[2]));
-    // r0 = 1/(2|q|) and c4 = 1/(8 kr |q|^3) at q_a + h minus at q_a, from
-    // |q_a + h|^2 - |q_a|^2 = 2 q_a.h + |h|^2 without cancellation
-    const uint32_t sa = sbase + lane;
-    const PixF64 pa = px[sa];
-    const double qa[3] = {pa.x - sc->cx, pa.y - sc->cy, pa.z - sc->cz};
-    const double Ra = ((const XRTK_CONST double*)I.R64)[2 * sa];
-    const double t = 2.0 * (qa[0] * hv[0] + qa[1] * hv[1] + qa[2] * hv[2]) +
-                     (hv[0] * hv[0] + hv[1] * hv[1] + hv[2] * hv[2]);
-    const double Rb = sqrt(Ra * Ra + t);
-    const double ia = 1.0 / Ra, ib = 1.0 / Rb;
-    const double di = -(t / (Ra + Rb)) * ia * ib;  // 1/|q_a + h| - 1/|q_a|
-    L.dr0 = (float)(0.5 * di);
-    L.dc4 = (float)(0.125 / kr * di * (ib * ib + ib * ia + ia * ia));
-    L.c4n = (float)(0.125 * (ib * ib * ib / kr));
+    LineRec lr;
+    if constexpr (!DEV) {
+      // one pass: anchored at slot 0, the record tile prep wrote
+      const uint32_t nt = npad / (64u * P);
+      lr = ((const XRTK_CONST LineRec*)(I.tiles + nt))[(base / (64u * P)) * 64u + lane];
+    } else {
+      const XRTK_CONST PixF64* px = (const XRTK_CONST PixF64*)I.pix64;
+      const InstScratch* sc = (const InstScratch*)I.scr;
+      line_secant<P>(px, I.perm, I.np, base + lane, hv);
+      int ax[3];
+      kernel_axes(sc, ax);
+      const uint32_t sa = sbase + lane;
+      const PixF64 pa = px[sa];
+      const double qa[3] = {pa.x - sc->cx, pa.y - sc->cy, pa.z - sc->cz};
+      lr = line_step(qa, ((const XRTK_CONST double*)I.R64)[2 * sa], hv, kr, ax);
+    }
+    L.hx = lr.hx; L.hy = lr.hy; L.hz = lr.hz;
+    L.dr0 = lr.dr0; L.dc4 = lr.dc4; L.c4n = lr.c4n;
   }
   // DEV: each slot's offset from the line and its phase gradient (radians
   // per unit o, kernel axis order): w_i = -2 pi kr (d_i - q^ (q^.d_i)) / |q_i|
@@ -1064,6 +1092,10 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
       }
       __syncthreads();
       if constexpr (LINE) {
+        // the ray loop starts on a 64-byte instruction line: a backward
+        // branch to the last bytes of a line costs a fetch on every trip
+        // (~1.5 % of this loop's time, measured)
+        asm volatile(".p2align 6");
 #pragma unroll 2
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];

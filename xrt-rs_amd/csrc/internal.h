@@ -90,6 +90,23 @@ struct TileRec {
   double qx, qy, qz, R;
 };
 
+// One lane's slot line of one tile, anchored at the lane's slot 0 (from
+// k_tile_prep): the step to q_0 + h, h the secant to the lane's last real
+// slot: -2 kr h in the kernel axis order, and r0 = 1/(2|q|), c4 =
+// 1/(8 kr |q|^3) at q_0 + h minus at q_0, and c4 at q_0 + h.
+struct LineRec {
+  float hx, hy, hz, dr0;
+  float dc4, c4n, pad0, pad1;
+};
+static_assert(sizeof(LineRec) == 32, "LineRec is 32 B");
+
+// A tile table: the TileRecs of npad / (64 P) tiles, then 64 LineRecs per
+// tile (tile-major).
+inline size_t tile_table_bytes(uint32_t npad, uint32_t P) {
+  const size_t nt = npad / (64u * P);
+  return nt * (sizeof(TileRec) + 64 * sizeof(LineRec));
+}
+
 // Per-instance values computed on the device (centroid, bounds, flags).
 struct InstScratch {
   double cx, cy, cz;       // FP64 ray centroid

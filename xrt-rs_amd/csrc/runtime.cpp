@@ -224,7 +224,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     L.rays32 = off; off = align_up(off + (f32 ? sizeof(RayRec32) * s.nr : 0), 256);
     L.pix32 = off; off = align_up(off + (f32 ? sizeof(PixRec32) * s.npad : 0), 256);
     L.tiles = off;
-    off = align_up(off + (f32 ? sizeof(TileRec) * (s.npad / (64u * cfg32.P)) : 0), 256);
+    off = align_up(off + (f32 ? tile_table_bytes(s.npad, cfg32.P) : 0), 256);
     L.R64 = off; off = align_up(off + (f32 ? 2 * sizeof(double) * s.npad : 0), 256);
     L.partial = off;
     off = align_up(off + (f32 ? sizeof(float) : sizeof(double)) * 4 * (size_t)s.npad * L.n_chunks,

@@ -226,7 +226,7 @@ struct xrtk_server {
     size_t b = 0;
     b += au(sizeof(PixF64) * s.npad) + au(sizeof(uint32_t) * s.npad);
     b += au(f32 ? sizeof(RayRec32) * s.nr : 0) + au(f32 ? sizeof(PixRec32) * s.npad : 0);
-    b += au(f32 ? sizeof(TileRec) * (s.npad / (64u * c32.P)) : 0);
+    b += au(f32 ? tile_table_bytes(s.npad, c32.P) : 0);
     b += au(f32 ? 2 * sizeof(double) * s.npad : 0);
     b += au((f32 ? 16 : 32) * (size_t)s.npad * r->n_chunks);
     b += au(sizeof(InstScratch)) + au(sizeof(double) * 4 * kPartBlocks);
@@ -245,8 +245,7 @@ struct xrtk_server {
     uint32_t* perm = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * s.npad));
     RayRec32* rays32 = reinterpret_cast<RayRec32*>(take(f32 ? sizeof(RayRec32) * s.nr : 0));
     PixRec32* pix32 = reinterpret_cast<PixRec32*>(take(f32 ? sizeof(PixRec32) * s.npad : 0));
-    TileRec* tiles = reinterpret_cast<TileRec*>(
-        take(f32 ? sizeof(TileRec) * (s.npad / (64u * c32.P)) : 0));
+    TileRec* tiles = reinterpret_cast<TileRec*>(take(f32 ? tile_table_bytes(s.npad, c32.P) : 0));
     double* R64 = reinterpret_cast<double*>(take(f32 ? 2 * sizeof(double) * s.npad : 0));
     void* partial = take((f32 ? 16 : 32) * (size_t)s.npad * r->n_chunks);
     InstScratch* scr = reinterpret_cast<InstScratch*>(take(sizeof(InstScratch)));
