@@ -1103,6 +1103,7 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
           pair_line<PH, NOFRACT, LIN2, DEV>(r, tr.B, tr.off, L, h, a, dw);
         }
       } else {
+        asm volatile(".p2align 6");  // (as above)
 #pragma unroll 2
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
@@ -1112,6 +1113,7 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
       }
     }
   } else {
+    asm volatile(".p2align 6");  // (as above)
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
