@@ -1113,7 +1113,6 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
       }
     }
   } else {
-    asm volatile(".p2align 6");  // (as above)
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
