@@ -126,7 +126,7 @@ Box grid_box(const XrtkWaveParams& w) {
 
 void grid_dq(const XrtkWaveParams& w, int P, double dq[3]) {
   uint32_t th, tw;
-  tile_shape(P, &th, &tw);
+  grid_tile(P, w.nx, w.nz, &th, &tw);
   const double px = w.nx > 1 ? 2.0 * std::fabs(w.dx) / (w.nx - 1) : 0.0;
   const double pz = w.nz > 1 ? 2.0 * std::fabs(w.dz) / (w.nz - 1) : 0.0;
   dq[0] = px * th;
@@ -322,7 +322,7 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
   const uint32_t* d_perm = nullptr;
   if (ps.grid) {
     uint32_t th, tw;
-    tile_shape(c32.P, &th, &tw);
+    grid_tile(c32.P, ps.grid->nx, ps.grid->nz, &th, &tw);
     e = launch_pix_grid(ps.grid->center, ps.grid->dx, ps.grid->dz, ps.grid->nx, ps.grid->nz, th,
                         tw, npad, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(), ctx->stream);
     d_perm = ctx->perm.as<uint32_t>();

@@ -104,7 +104,7 @@ int build_job_specs(xrtk_ctx* ctx, uint32_t want, const LaunchCfg& c32, const La
     pb.add(j.screen.center[0] + std::fabs(j.screen.dx), j.screen.center[1],
            j.screen.center[2] + std::fabs(j.screen.dz));
     uint32_t th, tw;
-    tile_shape(c32.P, &th, &tw);
+    grid_tile(c32.P, j.screen.nx, j.screen.nz, &th, &tw);
     const double dq[3] = {
         j.screen.nx > 1 ? 2.0 * std::fabs(j.screen.dx) / (j.screen.nx - 1) * th : 0.0, 0.0,
         j.screen.nz > 1 ? 2.0 * std::fabs(j.screen.dz) / (j.screen.nz - 1) * tw : 0.0};
@@ -127,10 +127,10 @@ int build_job_specs(xrtk_ctx* ctx, uint32_t want, const LaunchCfg& c32, const La
 
 int enqueue_job_grids(xrtk_ctx* ctx, const LaunchCfg& c32, const XrtkJob* jobs,
                       const std::vector<InstSpec>& specs) {
-  uint32_t th, tw;
-  tile_shape(c32.P, &th, &tw);
   for (size_t i = 0; i < specs.size(); ++i) {
     const XrtkWaveParams& w = jobs[i].screen;
+    uint32_t th, tw;
+    grid_tile(c32.P, w.nx, w.nz, &th, &tw);
     hipError_t e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, specs[i].npad,
                                    const_cast<PixF64*>(specs[i].pix),
                                    const_cast<uint32_t*>(specs[i].perm), ctx->stream);

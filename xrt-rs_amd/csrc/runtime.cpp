@@ -150,15 +150,26 @@ uint32_t padded_slots(uint32_t np, const LaunchCfg& c32, const LaunchCfg& c64) {
   return (np + b - 1) / b * b;
 }
 
-void tile_shape(int P, uint32_t* th, uint32_t* tw) {
+void grid_tile(int P, uint32_t nx, uint32_t nz, uint32_t* th, uint32_t* tw) {
   const uint32_t n = 64u * P;
-  *tw = P >= 8 ? 32u : 16u;
-  *th = n / *tw;
+  const uint32_t w0 = P >= 8 ? 32u : 16u, h0 = n / w0;
+  *tw = w0;
+  *th = h0;
+  const uint32_t w1 = 2 * w0, h1 = h0 / 2;
+  if (w1 <= 64u && h1 >= 1u) {
+    auto area = [&](uint64_t h, uint64_t w) {
+      return ((nx + h - 1) / h * h) * ((nz + w - 1) / w * w);
+    };
+    if (area(h1, w1) < area(h0, w0)) {
+      *tw = w1;
+      *th = h1;
+    }
+  }
 }
 
 uint32_t grid_slots(uint32_t nx, uint32_t nz, const LaunchCfg& c32, const LaunchCfg& c64) {
   uint32_t th, tw;
-  tile_shape(c32.P, &th, &tw);
+  grid_tile(c32.P, nx, nz, &th, &tw);
   const uint64_t ntiles = (uint64_t)((nx + th - 1) / th) * ((nz + tw - 1) / tw);
   const uint64_t s = ntiles * th * tw;
   const uint32_t b = slot_block(c32, c64);

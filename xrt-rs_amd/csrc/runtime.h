@@ -165,8 +165,11 @@ uint32_t chunk_len_for(const xrtk_ctx* ctx, const LaunchCfg& cfg, uint32_t nr, u
                        int split_override);
 // Slot count of np pixels: a multiple of both launches' slot blocks.
 uint32_t padded_slots(uint32_t np, const LaunchCfg& c32, const LaunchCfg& c64);
-// Tile shape (rows of ix, columns of iz) of 64 * P slots.
-void tile_shape(int P, uint32_t* th, uint32_t* tw);
+// Tile shape (th rows of ix, tw columns of iz) of 64 * P slots over an
+// nx x nz prepare_wave grid.  tw divides 64, so a lane's P slots lie in one
+// column, 64 / tw rows apart: 16 (P < 8) or 32 columns, or twice as many
+// (half as tall, slots half as far apart) when that pads the grid less.
+void grid_tile(int P, uint32_t nx, uint32_t nz, uint32_t* th, uint32_t* tw);
 // Slots of a tiled nx x nz grid.
 uint32_t grid_slots(uint32_t nx, uint32_t nz, const LaunchCfg& c32, const LaunchCfg& c64);
 

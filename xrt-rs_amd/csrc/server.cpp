@@ -192,7 +192,7 @@ struct xrtk_server {
     pb.add(j.screen.center[0] + std::fabs(j.screen.dx), j.screen.center[1],
            j.screen.center[2] + std::fabs(j.screen.dz));
     uint32_t th, tw;
-    tile_shape(c32.P, &th, &tw);
+    grid_tile(c32.P, j.screen.nx, j.screen.nz, &th, &tw);
     const double dq[3] = {
         j.screen.nx > 1 ? 2.0 * std::fabs(j.screen.dx) / (j.screen.nx - 1) * th : 0.0, 0.0,
         j.screen.nz > 1 ? 2.0 * std::fabs(j.screen.dz) / (j.screen.nz - 1) * tw : 0.0};
@@ -272,9 +272,9 @@ struct xrtk_server {
       I.kr_lo = (double)(kr - (long double)I.kr_hi);
     }
     I.skip = s.skip;
-    uint32_t th, tw;
-    tile_shape(c32.P, &th, &tw);
     const XrtkWaveParams& w = r->job.screen;
+    uint32_t th, tw;
+    grid_tile(c32.P, w.nx, w.nz, &th, &tw);
     // KInst and the one-entry merge list go up through the pinned block (the
     // result lands there later, after these copies in stream order)
     char* hs = static_cast<char*>(r->hblk);
