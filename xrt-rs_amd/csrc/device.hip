@@ -627,7 +627,8 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //                       + 15/8 kr rho^2 |h|^2 / R,
 //          R = min |q|, s = sigma + (P-1) |h| / R bounds |q^.h^| along the line.
 //          (A float32 pixel list departs from the secant by its own rounding,
-//          ~2e-9 mm, a few 1e-8 rev at 1 m and 10 keV.)
+//          ~2e-9 mm: up to ~1.6e-5 rev at 1 m, 10 keV and a 0.2 mm aperture,
+//          so such lists take LINEDEV.)
 //          LINEDEV: the same with the slots' offsets delta_i = q_i - q_0 - i h
 //          from the line (a list whose points stray further from it) taken
 //          into the phase to first order, kr (d^ - q^).delta_i ~ o.w_i with
