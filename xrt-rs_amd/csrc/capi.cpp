@@ -267,6 +267,7 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
   // broadband or F64 requests: one FP64 instance with per-ray energies
   struct G { uint32_t off, n, mode; double energy; };
   std::vector<G> gs;
+  std::vector<Box> boxes;  // each group's ray box
   std::vector<uint32_t> perm;
   if (nr && want == MODE_F32R && !broadband) {
     uint32_t off = 0;
@@ -279,10 +280,38 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
       }
       const double e = rs.energy(gr[0]);
       gs.push_back(G{off, (uint32_t)gr.size(), route_mode(MODE_F32R, rb, pbox, dq, e), e});
+      boxes.push_back(rb);
       off += (uint32_t)gr.size();
     }
     if (groups.size() > 1)
       for (auto& gr : groups) perm.insert(perm.end(), gr.begin(), gr.end());
+    // An untuned grid solve takes smaller tiles while that keeps more
+    // groups off the FP64 kernel: the routing bound grows with the tile's
+    // extent (near-field screens at P = 16 can exceed it where P = 4 does
+    // not).
+    if (ps.grid && !cfg_shape_given(ctx, MODE_F32R, nr, np, opts)) {
+      auto n64 = [&](int P, double* d) {
+        grid_dq(*ps.grid, P, d);
+        int n = 0;
+        for (size_t g = 0; g < gs.size(); ++g)
+          n += route_mode(MODE_F32R, boxes[g], pbox, d, gs[g].energy) == MODE_F64;
+        return n;
+      };
+      int cur = 0;
+      for (const G& g : gs) cur += g.mode == MODE_F64;
+      while (cur > 0 && c32.P > 4) {
+        double d2[3];
+        const int m = n64(c32.P / 2, d2);
+        if (m >= cur) break;
+        c32.P /= 2;
+        cur = m;
+        std::memcpy(dq, d2, sizeof(dq));
+        for (size_t g = 0; g < gs.size(); ++g)
+          gs[g].mode = route_mode(MODE_F32R, boxes[g], pbox, dq, gs[g].energy);
+        c64 = cfg64_for(c32);
+        npad = grid_slots(ps.grid->nx, ps.grid->nz, c32, c64);
+      }
+    }
   } else if (nr) {
     gs.push_back(G{0, nr, MODE_F64, 0.0});
   }

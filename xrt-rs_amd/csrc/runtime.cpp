@@ -91,6 +91,12 @@ LaunchCfg cfg64_for(const LaunchCfg& c32) {
   return c;
 }
 
+bool cfg_shape_given(const xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np,
+                     const XrtkOpts* o) {
+  if (o && o->pixels_per_lane > 0) return true;
+  return ctx->tune.count(std::make_tuple((int)mode, ilog2(nr), ilog2(np))) != 0;
+}
+
 LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o,
                      bool f32_pixels) {
   LaunchCfg c = default_cfg(mode, nr, np, f32_pixels);

@@ -152,6 +152,10 @@ namespace xrtk {
 // rounding).
 LaunchCfg choose_cfg(xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np, const XrtkOpts* o,
                      bool f32_pixels = false);
+// Whether the pixels per lane of such a solve come from the caller's options
+// or the context's tuning table (not the default).
+bool cfg_shape_given(const xrtk_ctx* ctx, uint32_t mode, uint64_t nr, uint64_t np,
+                     const XrtkOpts* o);
 // Default FP64 configuration paired with a given F32R configuration.
 LaunchCfg cfg64_for(const LaunchCfg& c32);
 // Tune-table key (both log2 fields) of the batched-solve configuration.
