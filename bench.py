@@ -502,6 +502,7 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
         th.start()
         t0 = xrtk.now()
         next_log = 10.0
+        t_sub = 0.0
         try:
             for k, (ta, (i, e)) in enumerate(zip(arrivals, reqs)):
                 while True:
@@ -515,6 +516,7 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
                 if ta >= next_log:
                     progress("submitting")
                     next_log += 10.0
+            t_sub = xrtk.now() - t0
         finally:
             pending.put(None)
             while True:
@@ -524,7 +526,6 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
                 progress("draining")
         if failed:
             raise failed[0]
-        t_sub = xrtk.now() - t0
         lat = [recs[k][0] for k in range(len(tickets))]
         queue = [recs[k][1] for k in range(len(tickets))]
         done = [recs[k][2] for k in range(len(tickets))]
