@@ -204,10 +204,13 @@ def leg_headline(args, ctx, xrtk, W):
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None,
-            "note": "achieved counts the reference formula's 51 flop per pair; the kernel "
-                    "executes fewer operations per pair (reduced phase, separable amplitude, "
-                    "slot-line rotation), so frac can exceed 1; issue_frac is the share of "
-                    "the VALU issue capacity its own instructions use"}
+            "note": "bound 'valu': the pair kernel is limited by FP32 vector-ALU issue (no "
+                    "matrix instructions; its HBM traffic is ~0.2 % of 8 TB/s), priced against "
+                    "the 157.3 TFLOP/s FP32 peak; achieved counts the reference formula's 51 "
+                    "flop per pair; the kernel executes fewer operations per pair (reduced "
+                    "phase, separable amplitude, slot-line rotation), so frac can exceed 1; "
+                    "issue_frac is the share of the VALU issue capacity its own instructions "
+                    "use"}
     t = traffic_record(nr, npx, plan)
     if t is not None:
         roof["traffic"] = t["hbm_bytes_per_launch"]
