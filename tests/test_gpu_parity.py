@@ -210,6 +210,27 @@ def test_f32r_result_independent_of_launch_shape(ctx):
         assert rel_l2(by_p[P], by_p[ps[0]]) < 1e-5
 
 
+@pytest.mark.parametrize("nx,nz", [(300, 70), (33, 100), (1, 257), (257, 1)])
+def test_grid_shapes_match_pixel_list(ctx, nx, nz):
+    """Non-square and one-pixel-wide prepare_wave grids (edge tiles padded
+    along one or both axes; 32 x 32 or 16 x 64 tiles at P = 16, chosen per
+    grid): the grid generated on the device gives the FP64 kernel the same
+    bits as the same points passed as a pixel list, the oracle agrees on a
+    sample, and the FP32-reduced solve stays within its bar."""
+    rays = W.point_source_aperture(4096, seed=23)
+    g = xrtk.WaveParams((0.001, 1000.0, -0.002), 0.02, 0.015, nx, nz)
+    px, py, pz = xrtk.prepare_wave(g)
+    f64_grid = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
+    f64_list = xrtk.diffraction_integral(rays, (px, py, pz), ctx=ctx, mode="f64")
+    assert np.array_equal(f64_grid, f64_list)
+    stride = 37
+    ref = O.diffraction_f64(rays, px, py, pz, stride=stride)
+    assert rel_l2(f64_grid[::stride], ref) < REL_F64
+    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    assert ctx.last_plan()["n_fp64"] == 0
+    assert rel_l2(f32, f64_grid) < 1e-5
+
+
 def test_f64_result_independent_of_launch_shape(ctx):
     rays = W.point_source_aperture(3000, seed=10)
     scr = W.screen(1000.0, 0.02, 24)
