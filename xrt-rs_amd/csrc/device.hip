@@ -920,7 +920,7 @@ struct LineRegs {
 // further slot, and 6 per slot (sums).
 // DEV: slot i lies off the line by its own offset; its phase gains
 // o.w_i radians (w_i from the item prologue), applied to the slot's
-// e^{i phase} to first order: 4 more operations per slot (3 with LIN2).
+// e^{i phase} to first order: 5 more operations per slot (4 with LIN2).
 template <int P, bool NOFRACT, bool LIN2, bool DEV = false>
 __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
                                           const LineRegs& L, float h, float (&a)[P][4],
