@@ -58,3 +58,31 @@ def test_random_geometry(ctx, k):
         print(f"  P={P}: flags {plan['inst_flags']:#x} fp64-routed {plan['n_fp64']} "
               f"rel-L2 vs long double {err:.2e}, vs FP64 {W.rel_l2(got, f64):.2e}", flush=True)
         assert err < REL_F32R, (P, info)
+
+
+@pytest.mark.parametrize("k", range(0, N_CASES, 3))
+def test_random_geometry_gpu_types(ctx, k):
+    """The reference's own call shape on the same cases: kirchhoff_gpu on
+    float32 GpuRay / GpuPixel records (a pixel list, off the slot line by
+    float32 rounding), against the long-double oracle evaluated at the
+    float32 values."""
+    rays, g, info = case(k)
+    px, py, pz = xrtk.prepare_wave(g)
+    r32 = xrtk.gpu_rays(rays["x"].shape[0])
+    for f in xrtk.RAY_FIELDS:
+        r32[f] = rays[f]
+    p32 = xrtk.gpu_pixels(px, py, pz)
+    rays32 = {f: r32[f].astype(np.float64) for f in xrtk.RAY_FIELDS}
+    stride = max(1, px.shape[0] // 97)
+    ld = O.diffraction_ld(rays32, p32["x"].astype(np.float64), p32["y"].astype(np.float64),
+                          p32["z"].astype(np.float64), stride=stride)
+    print(f"case {k}: {info}", flush=True)
+    # the default plan, and the shapes with the slot line (with per-slot
+    # offsets where the list allows it)
+    for shape in ({}, {"pixels_per_lane": 8}, {"pixels_per_lane": 16}):
+        got = xrtk.kirchhoff_gpu(ctx, r32, p32, **shape)
+        plan = ctx.last_plan()
+        err = W.rel_l2(got[::stride], ld)
+        print(f"  P={plan['pixels_per_lane']} flags {plan['inst_flags']:#x} "
+              f"fp64-routed {plan['n_fp64']}: rel-L2 vs long double {err:.2e}", flush=True)
+        assert err < REL_F32R, (shape, info)
