@@ -3,9 +3,12 @@
 Headline workload (one step): the Gaussian-source case, 1,048,576 rays x 512^2
 pixels at 1000 mm, 10 keV (2.75e11 ray-pixel pairs), solved from a ray set
 resident in HBM: FP64 centroid + prep, FP32 phase-reduced pair kernel, FP64
-merge and base phase, result copied to the host.  The same solve with the
-plain FP64 pair kernel runs beside it for the speed ratio and the error
-(complex rel-L2 and intensity error of FP32-reduced vs FP64 over all pixels).
+merge and base phase, result copied to the host.  Launch shape: that of the
+committed counter record (profiles/pair_kernel_traffic.json) when it was
+taken on this workload, else the autotuner's (--autotune forces the tuner,
+--shape fixes one).  The same solve with the plain FP64 pair kernel runs
+beside it for the speed ratio and the error (complex rel-L2 and intensity
+error of FP32-reduced vs FP64 over all pixels).
 
 Further legs, reported in the same JSON line:
   batched   64 uneven instances (rays log-uniform 16k-2M, square screens
@@ -102,6 +105,8 @@ def traffic_record(nr, npx, plan):
     w = t.get("workload", {})
     if w.get("rays") != nr or w.get("pixels") != npx:
         return None
+    if plan is None:  # the workload alone
+        return t
     if any(w.get("plan", {}).get(k) != plan.get(k)
            for k in ("pixels_per_lane", "waves_per_group", "tile_rays", "persistent")):
         return None
@@ -119,16 +124,28 @@ def leg_headline(args, ctx, xrtk, W):
     pairs = float(nr) * float(npx)
     rs = xrtk.RaySet(ctx, rays)
     job = xrtk.Job(rs, 10000.0, wave)
-    # launch shape for this job from the autotuner (untimed; it times the
-    # candidate shapes on the job cut to 65,536 rays), or as given
+    # launch shape for this job: as given; else the shape of the committed
+    # counter record when it was taken on this workload (the autotuner's
+    # usual pick here; its timing on the job cut to 65,536 rays leaves one
+    # wave per group and four within noise of each other), so the roofline's
+    # traffic and issue figures describe the kernel that runs; else, or with
+    # --autotune, the autotuner's (untimed)
+    rec = None if args.autotune else traffic_record(nr, npx, None)
     if args.shape:
         P, Wv, per = (int(x) for x in args.shape.split(","))
         tuned = {"pixels_per_lane": P, "waves_per_group": Wv, "persistent": per,
                  "tile_rays": 8192}
+        shape_from = "given"
         log(f"shape given: P={P} W={Wv} persistent={per}")
+    elif rec is not None:
+        tuned = dict(rec["workload"]["plan"])
+        shape_from = "profiled"
+        log(f"shape of the profiled plan: P={tuned['pixels_per_lane']} "
+            f"W={tuned['waves_per_group']} persistent={tuned['persistent']}")
     else:
         t = time.perf_counter()
         tuned, _ = xrtk.autotune_jobs(ctx, [job])
+        shape_from = "autotuned"
         log(f"autotune: {time.perf_counter() - t:.1f}s, chosen P={tuned['pixels_per_lane']} "
             f"W={tuned['waves_per_group']} persistent={tuned['persistent']}")
     shape = {k: tuned[k] for k in ("pixels_per_lane", "waves_per_group", "persistent",
@@ -224,7 +241,7 @@ def leg_headline(args, ctx, xrtk, W):
             vt, tt = t["valu_insts_per_pair"], t["trans_insts_per_pair"]
             roof["issue_frac"] = waves_s * ((vt - tt) / ISSUE_FMA + tt / ISSUE_TRANS)
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
-            "plan": plan, "autotuned": tuned, "roofline": roof, "fp64": fp64,
+            "plan": plan, "shape_from": shape_from, "roofline": roof, "fp64": fp64,
             "oneshot": oneshot}
     return head, rays, wave, out32, rs
 
@@ -610,7 +627,9 @@ def main():
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--shape", default="", metavar="P,W,PERSISTENT",
-                    help="headline launch shape instead of the autotuner's")
+                    help="headline launch shape instead of the profiled plan's")
+    ap.add_argument("--autotune", action="store_true",
+                    help="headline launch shape from the autotuner")
     ap.add_argument("--no-fresnel", action="store_true")
     ap.add_argument("--no-undulator", action="store_true")
     ap.add_argument("--und-energies", type=int, default=1024)
@@ -686,7 +705,7 @@ def main():
         "config": {"workload": "gaussian-source Kirchhoff solve, 1,048,576 rays x 512^2 pixels "
                                "at 1000 mm, 10 keV, resident ray set",
                    "rays": head["nr"], "pixels": head["npx"], "pairs": head["pairs"],
-                   "plan": head["plan"]},
+                   "plan": head["plan"], "shape_from": head["shape_from"]},
         "roofline": head["roofline"],
         "fp64": head["fp64"],
         "oneshot_host_buffers": head["oneshot"],
