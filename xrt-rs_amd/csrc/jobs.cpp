@@ -431,11 +431,16 @@ extern "C" int xrtk_autotune(xrtk_ctx* ctx, size_t n_rays, size_t n_pixels, int 
 // Batch autotuner: times each candidate launch shape on the caller's batch
 // and records the fastest as the context's batched-solve configuration
 // (xrtk_solve_jobs, and servers created afterwards).  Each instance is cut to
-// its first kTuneRays rays for the timing: the pixel slots, tiles and launch
-// shape are those of the real batch, and the per-ray cost does not depend on
-// the ray count once chunks are long.
+// its first R rays for the timing, R = kTuneRays doubled until the cut batch
+// holds kTunePairs pairs: the pixel slots, tiles and launch shape are those of
+// the real batch, and the per-ray cost does not depend on the ray count once
+// chunks are long and the launch fills the device.
 namespace {
 constexpr uint32_t kTuneRays = 65536;
+// Pairs one timed launch should hold: about two rounds of full-size items
+// (1024 slots x 8192 rays) over the 8192 resident one-wave groups, so that
+// every candidate is timed on a full device.
+constexpr double kTunePairs = 1.2e11;
 struct ScratchBufs {
   DBuf pix, perm, outd, work, meta;
   ~ScratchBufs() { pix.release(); perm.release(); outd.release(); work.release(); meta.release(); }
@@ -464,22 +469,36 @@ extern "C" int xrtk_autotune_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_j
     ScratchBufs b;
     double bt = 1e300;
     LaunchCfg bc = cands.front();
+    uint32_t cut = 0;  // rays per instance in the timed launches
     for (const LaunchCfg& c : cands) {
       const LaunchCfg& c32 = mode == MODE_F32R ? c : other;
       const LaunchCfg& c64 = mode == MODE_F32R ? other : c;
       std::vector<InstSpec> specs;
       int rc = build_job_specs(ctx, mode, c32, c64, jobs, n_jobs, b.pix, b.perm, b.outd, specs);
       if (rc != XRTK_OK) return rc;
-      for (InstSpec& s : specs) s.nr = std::min<uint32_t>(s.nr, kTuneRays);
+      if (!cut) {
+        // kTuneRays, doubled while the cut batch holds fewer than kTunePairs
+        uint32_t max_nr = 0;
+        for (const InstSpec& s : specs) max_nr = std::max(max_nr, s.nr);
+        auto pairs_at = [&](uint64_t R) {
+          double p = 0;
+          for (const InstSpec& s : specs) p += (double)std::min<uint64_t>(s.nr, R) * s.np;
+          return p;
+        };
+        cut = kTuneRays;
+        while (cut < max_nr && pairs_at(cut) < kTunePairs) cut = cut > (1u << 30) ? max_nr : 2 * cut;
+      }
+      for (InstSpec& s : specs) s.nr = std::min<uint32_t>(s.nr, cut);
       double t = 1e300;
-      for (int rep = 0; rep < 2; ++rep) {
+      for (int rep = 0; rep < 3; ++rep) {
         rc = enqueue_job_grids(ctx, c32, jobs, specs);
         if (rc == XRTK_OK)
           rc = run_instances(ctx, c32, c64, specs, b.work, b.meta, true, nullptr, 0, 0, true);
         if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
         if (rc != XRTK_OK) return rc;
         float ms = 0.f;
-        if (rep && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess) t = ms;
+        if (rep && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess)
+          t = std::min(t, (double)ms);
       }
       if (t < bt) { bt = t; bc = c; }
     }
