@@ -152,6 +152,18 @@ int xrtk_diffraction_f64(xrtk_ctx *ctx, const XrtkRaysF64 *rays, size_t n_rays,
                          const double *pix_z, size_t n_pixels,
                          double *out4, const XrtkOpts *opts);
 
+/* diffraction_integral (src/waves/diffraction.rs:56-117) with the dispatch
+ * of kirchhoff_auto (src/gpu/fallback.rs:19-26): the plain FP64 device kernel
+ * on the shared context when a device exists, else the reference's own f64
+ * CPU loop (complex Kahan sums, skip below 1e-30 mm), which is what the
+ * Python caller diffraction_integral_rs (crates/xrt-python/src/lib.rs
+ * :321-394) always runs.  XRTK_REQUIRE_GPU=1 turns the CPU branch into
+ * XRTK_ENODEV.  xrtk_auto_last_backend reports which branch ran. */
+int xrtk_diffraction_auto_f64(const XrtkRaysF64 *rays, size_t n_rays,
+                              const double *pix_x, const double *pix_y,
+                              const double *pix_z, size_t n_pixels,
+                              double *out4);
+
 /* Same, with the screen generated on the device from a prepare_wave grid
  * (nx*nz pixels, output in prepare_wave's order). */
 int xrtk_diffraction_grid_f64(xrtk_ctx *ctx, const XrtkRaysF64 *rays,
@@ -371,6 +383,16 @@ void xrtk_server_destroy(xrtk_server *s);
 
 /* Steady-clock seconds, the clock of xrtk_server_wait's timestamps. */
 double xrtk_now(void);
+
+/* ---- testing ------------------------------------------------------------ */
+
+/* Makes the next n pair-kernel launches of this process fail before anything
+ * is enqueued, as a failed HIP launch would: the calls that own them return
+ * XRTK_ELAUNCH (served requests in such a launch complete with XRTK_ELAUNCH)
+ * and the context stays usable.  n = 0 clears it.  No reference counterpart:
+ * it exercises the error path that the reference reaches by panicking on a
+ * failed buffer map (src/gpu/kirchhoff.rs:268-271). */
+void xrtk_debug_fail_launches(uint32_t n);
 
 #ifdef __cplusplus
 }

@@ -62,12 +62,46 @@ def case(name, seed, nr, npx, energy, dist, half, phalf, tilt=False):
     return {"id": name, "rays": rays, "pixels": pix, "expected": integral(rays, pix)}
 
 
+def curved_case(name, seed, nr, npx, energy, dist, half, phalf, radius, tilt):
+    """Rays on a toroidal cap y = x^2/(2 Rx) + z^2/(2 Rz), each with its own
+    normal: the cap's normal at its point tilted at random by up to `tilt`
+    rad about x and z (rays off a curved optic carry per-ray normals,
+    src/oes/reflect.rs:53-56), nl = n . (beam direction from a point source
+    10 m upstream) as beam_to_diffraction_rays forms it
+    (src/waves/prepare_wave.rs:86-87)."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-half, half, nr)
+    z = rng.uniform(-half, half, nr)
+    y = x * x / (2 * radius[0]) + z * z / (2 * radius[1])
+    tx = -x / radius[0] + rng.uniform(-tilt, tilt, nr)
+    tz = -z / radius[1] + rng.uniform(-tilt, tilt, nr)
+    m = np.sqrt(tx * tx + 1.0 + tz * tz)
+    nx, ny, nz = tx / m, 1.0 / m, tz / m
+    d = np.sqrt(x * x + (y + 10000.0) ** 2 + z * z)
+    rays = {
+        "x": x, "y": y, "z": z, "nx": nx, "ny": ny, "nz": nz,
+        "nl": (nx * x + ny * (y + 10000.0) + nz * z) / d,
+        "es_re": rng.uniform(0.5, 1.0, nr), "es_im": rng.uniform(-0.5, 0.5, nr),
+        "ep_re": rng.uniform(-0.3, 0.3, nr), "ep_im": rng.uniform(-0.3, 0.3, nr),
+        "energy": np.full(nr, energy),
+    }
+    pix = {"x": rng.uniform(-phalf, phalf, npx), "y": np.full(npx, dist),
+           "z": rng.uniform(-phalf, phalf, npx)}
+    rays = {k: [float(v) for v in a] for k, a in rays.items()}
+    pix = {k: [float(v) for v in a] for k, a in pix.items()}
+    return {"id": name, "rays": rays, "pixels": pix, "expected": integral(rays, pix)}
+
+
 if __name__ == "__main__":
     cases = [
         case("micro_1kev_2um", 10, 12, 6, 1000.0, 0.002, 0.001, 0.001),
         case("near_1kev_0p5mm", 11, 16, 8, 1000.0, 0.5, 0.05, 0.05),
         case("mid_5kev_20mm", 12, 24, 8, 5000.0, 20.0, 0.05, 0.02, tilt=True),
         case("far_10kev_1m", 13, 32, 8, 10000.0, 1000.0, 0.1, 0.02),
+        curved_case("curved_10kev_1m", 14, 32, 8, 10000.0, 1000.0, 0.05, 0.02,
+                    (50.0, 200.0), 0.3),
+        curved_case("curved_8kev_50mm", 15, 24, 8, 8000.0, 50.0, 0.05, 0.01,
+                    (20.0, 80.0), 0.2),
     ]
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "mp_small.json"), "w") as f:

@@ -113,6 +113,78 @@ def test_require_gpu_turns_fallback_into_enodev(monkeypatch):
     assert ei.value.code == _lib.ENODEV
 
 
+def _run_no_device(code: str, **env):
+    """Runs `code` in a fresh interpreter with the device hidden
+    (XRTK_FORCE_NO_DEVICE=1: the shared context reports no device), so the
+    auto entry points take their CPU branch on any machine."""
+    import subprocess
+    import sys
+    e = dict(os.environ, XRTK_FORCE_NO_DEVICE="1", **env)
+    e["PYTHONPATH"] = os.pathsep.join([os.path.join(ROOT, "xrt-rs_amd"),
+                                       os.path.join(ROOT, "oracle"),
+                                       os.path.join(ROOT, "tests")])
+    r = subprocess.run([sys.executable, "-c", code], env=e, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_diffraction_integral_rs_reference_semantics_without_device():
+    """The Python caller's own test (validation/test_diffraction.py:36-59):
+    the fixture to 1e-10 absolute per component.  Without a device the f64
+    entry runs the reference's f64 Kahan loop on the CPU (what
+    diffraction_integral_rs always runs, crates/xrt-python/src/lib.rs:386);
+    it equals the oracle's restatement of that loop."""
+    out = _run_no_device("""
+import numpy as np, xrtk, pyoracle as O
+from helpers import fixture_case
+assert xrtk.GpuContext.shared() is None
+rays, pix, exp = fixture_case()
+args = [rays[k] for k in xrtk.RAY_FIELDS] + list(pix)
+es_re, es_im, ep_re, ep_im = xrtk.diffraction_integral_rs(*args)
+assert xrtk.auto_last_backend() == 0
+got = np.array(es_re) + 1j * np.array(es_im)
+assert np.max(np.abs(got.real - exp[:, 0].real)) < 1e-10
+assert np.max(np.abs(got.imag - exp[:, 0].imag)) < 1e-10
+gp = np.array(ep_re) + 1j * np.array(ep_im)
+assert np.max(np.abs(gp - exp[:, 1])) < 1e-10
+ref = O.diffraction_f64(rays, *pix)
+both = np.stack([got, gp], axis=1)
+assert np.max(np.abs(both - ref) / np.abs(ref)) < 1e-14
+# the same through diffraction_integral's default mode
+d = xrtk.diffraction_integral(rays, pix)
+assert np.array_equal(d, both)
+print("ok")
+""")
+    assert out.strip() == "ok"
+
+
+def test_require_gpu_turns_f64_fallback_into_enodev():
+    out = _run_no_device("""
+import xrtk
+from helpers import fixture_case
+rays, pix, _ = fixture_case()
+try:
+    xrtk.diffraction_integral_rs(*([rays[k] for k in xrtk.RAY_FIELDS] + list(pix)))
+except xrtk.XrtkError as e:
+    print(e.code)
+""", XRTK_REQUIRE_GPU="1")
+    assert out.strip() == str(_lib.ENODEV)
+
+
+def test_f32r_mode_needs_a_device():
+    out = _run_no_device("""
+import xrtk
+from helpers import fixture_case
+rays, pix, _ = fixture_case()
+try:
+    xrtk.diffraction_integral(rays, pix, mode="f32r")
+except xrtk.XrtkError as e:
+    print(e.code)
+""")
+    assert out.strip() == str(_lib.ENODEV)
+
+
 def test_device_entry_points_report_enodev_without_context():
     lib = _lib.lib
     h = C.c_uint64()
@@ -151,5 +223,11 @@ def test_workload_generators_deterministic():
     inst = W.batch_instances()
     assert len(inst) == 64
     assert all(16384 <= s["n_rays"] <= 2097152 and 64 <= s["side"] <= 1024 for s in inst)
+    c = W.curved_aperture(5000, tilt=0.01, seed=3)
+    nn = np.sqrt(c["nx"] ** 2 + c["ny"] ** 2 + c["nz"] ** 2)
+    assert np.allclose(nn, 1.0) and np.ptp(c["nx"]) > 0.01 and np.ptp(c["y"]) > 0
+    assert np.all(np.abs(c["nl"]) <= 1.0) and np.all(c["nl"] > 0.99)
+    m = W.grazing_mirror(5000, length=100.0, bend_radius=50000.0, seed=3)
+    assert np.ptp(m["y"]) > 99.0 and np.ptp(m["nz"]) > 0 and np.all(m["nl"] > 0)
     u = W.SplitMix64(1).uniform(100000)
     assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.01

@@ -32,7 +32,7 @@ def test_autotune_bucket_used_by_oneshot(fresh_ctx):
     assert ms > 0
     rays = W.point_source_aperture(200000, seed=5)
     wave = grid(W.screen(1000.0, 0.02, 256))
-    out = xrtk.diffraction_integral(rays, wave, ctx=c)
+    out = xrtk.diffraction_integral(rays, wave, ctx=c, mode="f32r")
     plan = c.last_plan()
     assert plan["pixels_per_lane"] == opts["pixels_per_lane"]
     assert plan["waves_per_group"] == opts["waves_per_group"]

@@ -346,9 +346,9 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
     term of o.m gives the same bits as the full dot product."""
     rays = W.point_source_aperture(20000, seed=3)
     g = grid(W.screen(1000.0, 0.02, 64))
-    lin2 = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    lin2 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x400")
-    full = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    full = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     assert np.array_equal(lin2, full)
 
 
@@ -362,10 +362,10 @@ def test_paraxial_expansion_matches_exact_form(ctx, monkeypatch, distance):
     against rsq/rcp of the exact path; both against the long-double oracle."""
     rays, scr, _ = W.stress_cell(20000.0, distance, n_rays=32768, side=96)
     g = grid(scr)
-    pax = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    pax = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     flags = ctx.last_plan()["inst_flags"]
     monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x800")
-    exact = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    exact = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     assert not ctx.last_plan()["inst_flags"] & VAR_PAX
     monkeypatch.delenv("XRTK_FORCE_FLAGS")
     px, py, pz = xrtk.prepare_wave(g)
@@ -382,7 +382,7 @@ def test_paraxial_expansion_matches_exact_form(ctx, monkeypatch, distance):
 def test_gaussian_case_variant(ctx):
     """The headline case runs the tile variant with every specialisation."""
     rays = W.gaussian_source(65536, seed=2)
-    xrtk.diffraction_integral(rays, grid(W.screen(1000.0, 0.02, 128)), ctx=ctx)
+    xrtk.diffraction_integral(rays, grid(W.screen(1000.0, 0.02, 128)), ctx=ctx, mode="f32r")
     f = ctx.last_plan()["inst_flags"]
     assert f & VAR_TILE and f & VAR_NOFRACT and f & VAR_LIN2 and f & VAR_PAX, hex(f)
     assert f & VAR_SEPAMP, hex(f)
@@ -400,10 +400,10 @@ def test_separable_amplitude_matches_exact_form(ctx, monkeypatch, case):
     else:
         rays, scr, _ = W.stress_cell(20000.0, float(case[1:]), n_rays=32768, side=96)
         g = grid(scr)
-    sep = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    sep = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     flags = ctx.last_plan()["inst_flags"]
     monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x1000")
-    per_pair = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    per_pair = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     assert not ctx.last_plan()["inst_flags"] & VAR_SEPAMP
     monkeypatch.delenv("XRTK_FORCE_FLAGS")
     between = rel_l2(sep, per_pair)
@@ -439,10 +439,10 @@ def test_slot_line_rotation_matches_per_slot_phase(ctx, monkeypatch, case, P):
     the line exceeds 1e-7 rev and the line form is refused: the same bits
     as the per-slot form."""
     rays, g = _line_case(case)
-    line = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
+    line = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", pixels_per_lane=P)
     flags = ctx.last_plan()["inst_flags"]
     monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x2000")
-    per_slot = xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=P)
+    per_slot = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", pixels_per_lane=P)
     f2 = ctx.last_plan()["inst_flags"]
     monkeypatch.delenv("XRTK_FORCE_FLAGS")
     assert not f2 & VAR_LINE and f2 & VAR_SEPAMP, hex(f2)
@@ -505,20 +505,20 @@ def test_slot_line_needs_a_line(ctx):
                              (98, 0.02, False)):
         g = grid(W.screen(1000.0, half, side))
         px, py, pz = xrtk.prepare_wave(g)
-        got = xrtk.diffraction_integral(rays, (px, py, pz), ctx=ctx, pixels_per_lane=4)
+        got = xrtk.diffraction_integral(rays, (px, py, pz), ctx=ctx, mode="f32r", pixels_per_lane=4)
         assert bool(ctx.last_plan()["inst_flags"] & VAR_LINE) == line, (side, half)
         ref = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
         assert rel_l2(got, ref) < 1e-5
     perm = np.random.default_rng(5).permutation(px.shape[0])
-    shuffled = xrtk.diffraction_integral(rays, (px[perm], py[perm], pz[perm]), ctx=ctx,
+    shuffled = xrtk.diffraction_integral(rays, (px[perm], py[perm], pz[perm]), ctx=ctx, mode="f32r",
                                          pixels_per_lane=4)
     assert not ctx.last_plan()["inst_flags"] & VAR_LINE
     assert rel_l2(shuffled, ref[perm]) < 1e-5
     rays, g = _line_case("L100")
-    xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=8)
+    xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", pixels_per_lane=8)
     assert not ctx.last_plan()["inst_flags"] & (VAR_LINE | VAR_SEPAMP)
     rays, g = _line_case("gaussian")
-    xrtk.diffraction_integral(rays, g, ctx=ctx, pixels_per_lane=2)
+    xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", pixels_per_lane=2)
     f = ctx.last_plan()["inst_flags"]
     assert f & VAR_SEPAMP and not f & VAR_LINE
 
@@ -551,8 +551,8 @@ def test_geometry_frames(ctx, frame):
         Rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
         M = Rx @ Rz
     r2, p2 = _transform(rays, pix, M)
-    base = xrtk.diffraction_integral(rays, pix, ctx=ctx)
-    got = xrtk.diffraction_integral(r2, p2, ctx=ctx)
+    base = xrtk.diffraction_integral(rays, pix, ctx=ctx, mode="f32r")
+    got = xrtk.diffraction_integral(r2, p2, ctx=ctx, mode="f32r")
     ref = xrtk.diffraction_integral(r2, p2, ctx=ctx, mode="f64")
     e_ref, e_base = rel_l2(got, ref), rel_l2(got, base)
     print(f"{frame}: vs f64 {e_ref:.2e}, vs original frame {e_base:.2e}")
@@ -582,18 +582,18 @@ def test_polarisation_forms(ctx):
     rays.update(es_re=es.real.copy(), es_im=es.imag.copy(), ep_re=ep.real.copy(),
                 ep_im=ep.imag.copy())
     g = grid(W.screen(1000.0, 0.02, 64))
-    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx)
+    f32 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
     f64 = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f64")
     for c in range(2):
         assert rel_l2(f32[:, c], f64[:, c]) < REL_F32R
     # exchanging Es and Ep exchanges the two fields
     sw = dict(rays, es_re=rays["ep_re"], es_im=rays["ep_im"], ep_re=rays["es_re"],
               ep_im=rays["es_im"])
-    f32s = xrtk.diffraction_integral(sw, g, ctx=ctx)
+    f32s = xrtk.diffraction_integral(sw, g, ctx=ctx, mode="f32r")
     assert rel_l2(f32s[:, ::-1], f64) < REL_F32R
     # dark rays alone give exact zeros
     dark = {k: v[2 * q:2 * q + 64] for k, v in rays.items()}
-    assert not np.any(xrtk.diffraction_integral(dark, g, ctx=ctx))
+    assert not np.any(xrtk.diffraction_integral(dark, g, ctx=ctx, mode="f32r"))
 
 
 # ---- resident ray sets, upstream path and batches ----------------------------------------------
@@ -630,7 +630,7 @@ def test_batch_equals_standalone_bitwise(ctx):
         for _ in range(3):
             again = xrtk.solve_jobs(ctx, jobs)
             xrtk.diffraction_integral(W.point_source_aperture(500, seed=1),
-                                      grid(W.screen(1000.0, 0.02, 8)), ctx=ctx)
+                                      grid(W.screen(1000.0, 0.02, 8)), ctx=ctx, mode="f32r")
             for a, m in zip(again, mixed):
                 assert np.array_equal(a, m)
         # and each job against the FP64 kernel

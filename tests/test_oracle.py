@@ -98,6 +98,23 @@ def test_mp_vectors_detect_planted_mistakes(mutation):
     assert rel_l2(_numpy_formula(rays, pix, **kw), exp) > 1e-2
 
 
+@pytest.mark.parametrize("name", ["curved_10kev_1m", "curved_8kev_50mm"])
+def test_mp_vectors_pin_per_ray_normals(name):
+    """The curved-optic vectors carry a normal per ray (diffraction.rs:22-25,
+    86): the oracle meets them, and a restatement that gives every ray the
+    first ray's normal (or the flat aperture's (0, 1, 0)) misses them."""
+    (case,) = [c for c in mp_cases() if c[0] == name]
+    _, rays, pix, exp = case
+    assert np.ptp(rays["nx"]) > 0.1 and np.ptp(rays["nz"]) > 0.1
+    # (FP64 rounding of phases ~5e10 rad leaves ~2.5e-6 at 1 m; within the
+    # bound test_oracle_vs_mpmath holds it to)
+    assert rel_l2(O.diffraction_f64(rays, *pix), exp) < 1e-5
+    for n in ((rays["nx"][0], rays["ny"][0], rays["nz"][0]), (0.0, 1.0, 0.0)):
+        uni = dict(rays, nx=np.full_like(rays["nx"], n[0]), ny=np.full_like(rays["ny"], n[1]),
+                   nz=np.full_like(rays["nz"], n[2]))
+        assert rel_l2(_numpy_formula(uni, pix), exp) > 1e-3
+
+
 def test_single_ray_closed_form():
     """One ray, one pixel: U = i k/(4 pi) (nl + n.d/r) e^{ikr} / r (diffraction.rs:91-98),
     evaluated in 50-digit mpmath."""

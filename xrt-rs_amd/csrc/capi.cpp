@@ -19,6 +19,8 @@ using namespace xrtk;
 namespace xrtk {
 int cpu_fallback_f32(const XrtkGpuRay* rays, size_t nr, const XrtkGpuPixel* px, size_t np,
                      double* out4);
+int cpu_diffraction_f64(const XrtkRaysF64* r, size_t nr, const double* px, const double* py,
+                        const double* pz, size_t np, double* out4);
 void load_tune_cache(xrtk_ctx* ctx);
 }
 
@@ -552,6 +554,39 @@ int xrtk_diffraction_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
     return XRTK_ENOMEM;
   }
 }
+
+int xrtk_diffraction_auto_f64(const XrtkRaysF64* rays, size_t n_rays, const double* pix_x,
+                              const double* pix_y, const double* pix_z, size_t n_pixels,
+                              double* out4) {
+  if ((n_rays && !rays) || (n_pixels && (!out4 || !pix_x || !pix_y || !pix_z)))
+    return XRTK_EINVAL;
+  if (n_rays) {
+    const double* cols[12] = {rays->x,     rays->y,     rays->z,     rays->nx,
+                              rays->ny,    rays->nz,    rays->nl,    rays->es_re,
+                              rays->es_im, rays->ep_re, rays->ep_im, rays->energy};
+    for (const double* c : cols)
+      if (!c) return XRTK_EINVAL;
+  }
+  xrtk_ctx* ctx = xrtk_ctx_shared();
+  if (ctx) {
+    XrtkOpts o;
+    xrtk_opts_default(&o);
+    o.mode = XRTK_MODE_F64;
+    const int rc = xrtk_diffraction_f64(ctx, rays, n_rays, pix_x, pix_y, pix_z, n_pixels, out4,
+                                        &o);
+    g_auto_backend.store(1);
+    return rc;
+  }
+  if (env_flag("XRTK_REQUIRE_GPU")) return XRTK_ENODEV;
+  g_auto_backend.store(0);
+  try {
+    return cpu_diffraction_f64(rays, n_rays, pix_x, pix_y, pix_z, n_pixels, out4);
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+void xrtk_debug_fail_launches(uint32_t n) { g_fail_launches.store(n); }
 
 int xrtk_diffraction_grid_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
                               const XrtkWaveParams* screen, double* out4, const XrtkOpts* opts) {

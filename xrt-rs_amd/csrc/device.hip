@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "internal.h"
 
@@ -637,7 +638,8 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 //          order rotation by that angle needs it below 1e-4 rev.
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
 // 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP, 0x2000
-// forbids LINE, 0x4000 forbids LINEDEV.
+// forbids LINE, 0x4000 forbids LINEDEV, 0x8000 forbids UNIFORM_N (every ray
+// then takes its own normal, as rays off a curved optic do).
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags, uint32_t P) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -648,7 +650,7 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
   const double dz = __uint_as_float(scr->dqmax_bits[2]);
   const double rmin = __uint_as_float(scr->rmin_bits);
   uint32_t f = static_flags;
-  if (scr->n_differs == 0u) f |= INST_F_UNIFORM_N;
+  if (scr->n_differs == 0u && !(force_flags & 0x8000u)) f |= INST_F_UNIFORM_N;
   if (!(rmin - om > skip * (1.0 + 1e-9))) f |= INST_F_CHECK;
   const double den = 2.0 * rmin - om;
   double v0 = 1e300, v2 = 1e300;
@@ -1524,10 +1526,14 @@ static void launch_f64_t(const KInst* insts, const WorkItem* items, uint32_t n_i
 }
 
 // Returns hipErrorInvalidValue for a configuration that is not instantiated.
+std::atomic<uint32_t> g_fail_launches{0};
+
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st) {
   if (!n_items) return hipSuccess;
+  for (uint32_t f = g_fail_launches.load(); f > 0;)
+    if (g_fail_launches.compare_exchange_weak(f, f - 1)) return hipErrorLaunchFailure;
   // a non-persistent launch runs one item per group: every item needs one
   if (!cfg.persistent && grid < n_items) return hipErrorInvalidValue;
   if (cfg.persistent) hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 2u);

@@ -41,6 +41,10 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st);
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c);
+// Pair-kernel launches still to be failed on purpose (xrtk_debug_fail_launches):
+// launch_pair takes one and returns hipErrorLaunchFailure before enqueuing
+// anything.
+extern std::atomic<uint32_t> g_fail_launches;
 hipError_t launch_merge(const KInst* insts, const uint32_t* list, uint32_t n_list,
                         uint32_t max_npad, hipStream_t st);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t st);
@@ -136,6 +140,10 @@ struct xrtk_ctx {
     xrtk::DBuf work, meta, pix, perm, outd;
     std::vector<std::pair<double*, size_t>> outs;  // device out and np of each job
     xrtk::Plan plan;
+    // what an eager replay re-enqueues when there is no graph (XRTK_NO_GRAPH,
+    // or a capture that failed): the instances and each job's screen
+    std::vector<xrtk::InstSpec> specs;
+    std::vector<XrtkJob> jobs;
   };
   std::vector<GraphEntry*> graphs;
   // Fresnel propagation: rocBLAS handle (created on first use) and workspace

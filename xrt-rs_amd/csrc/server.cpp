@@ -180,7 +180,9 @@ struct xrtk_server {
     std::vector<InstSpec> specs;
     // build the spec's pixel/perm/out layout sizes only (no allocation)
     const XrtkJob& j = r->job;
-    const uint32_t np = j.screen.nx * j.screen.nz;
+    const size_t np64 = (size_t)j.screen.nx * j.screen.nz;
+    if (np64 > 0x7fff0000ull) return false;  // the slot and merge indices are 32-bit
+    const uint32_t np = (uint32_t)np64;
     auto it = ctx->raysets.find(j.rays);
     if (it == ctx->raysets.end()) return false;
     const RaySet& rs = it->second;
@@ -326,11 +328,14 @@ struct xrtk_server {
     }
   }
 
+  // A request whose launch failed (rc set) only records its event: its
+  // partial sums were never all computed, so nothing is merged or copied.
   void enqueue_merge(Req* r) {
     r->state = MERGING;
     hipError_t e = hipSuccess;
-    if (r->spec.nr) e = launch_merge(r->d_kinst, r->d_list, 1, r->spec.npad, stream);
-    if (e == hipSuccess && r->spec.np)
+    if (r->rc == XRTK_OK && r->spec.nr)
+      e = launch_merge(r->d_kinst, r->d_list, 1, r->spec.npad, stream);
+    if (e == hipSuccess && r->rc == XRTK_OK && r->spec.np)
       e = hipMemcpyAsync(r->hblk, r->spec.out, sizeof(double) * 4 * r->spec.np,
                          hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipEventRecord(r->done_ev, stream);
@@ -466,7 +471,18 @@ struct xrtk_server {
     st.launches++;
     st.max_concurrent = std::max<uint64_t>(st.max_concurrent, L.n_req);
     if (e != hipSuccess) {
-      for (Req* r : finishing) r->rc = map_err(e);
+      // Every request with items in this launch loses them: each fails now
+      // (its remaining items are dropped), rather than later merging partial
+      // sums that were never computed.
+      L.pairs = 0;
+      for (const Undo& u : undo) {
+        Req* r = u.r;
+        r->rc = map_err(e);
+        if (r->next_item < r->n_items) {
+          r->next_item = r->n_items;
+          finishing.push_back(r);
+        }
+      }
     }
     for (Req* r : finishing) enqueue_merge(r);
     cur ^= 1;
@@ -622,12 +638,21 @@ int xrtk_server_submit(xrtk_server* s, const XrtkJob* job, xrtk_ticket* ticket) 
     r->job = *job;
     r->t[0] = now_s();
     {
+      // The request holds its ray set from the lookup on (taken under the
+      // context lock, which xrtk_rayset_release also takes), so the set
+      // cannot be freed between this lookup and the admission.
       std::lock_guard<std::mutex> lk(s->ctx->mu);
       auto it = s->ctx->raysets.find(job->rays);
       if (it == s->ctx->raysets.end()) return XRTK_EINVAL;
       if (!s->plan_request(r.get())) return XRTK_EINVAL;
       r->users = it->second.users;
+      r->users->fetch_add(1);
     }
+    // gives the hold back on every early return below
+    struct Hold {
+      Req* r;
+      ~Hold() { if (r) r->users->fetch_sub(1); }
+    } hold{r.get()};
     if (r->spec.np && !job->out4) return XRTK_EINVAL;
     std::lock_guard<std::mutex> lk(s->mu);
     s->st.submitted++;
@@ -638,7 +663,7 @@ int xrtk_server_submit(xrtk_server* s, const XrtkJob* job, xrtk_ticket* ticket) 
     }
     if (hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming) != hipSuccess)
       return XRTK_ELAUNCH;
-    r->users->fetch_add(1);
+    hold.r = nullptr;  // kept by the request until it is collected
     r->id = s->next_id++;
     *ticket = r->id;
     Req* raw = r.release();

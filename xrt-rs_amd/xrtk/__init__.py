@@ -12,8 +12,10 @@ Mirrors the Kirchhoff path of physwkim/xrt-rs:
   beam_to_diffraction_rays ...... src/waves/prepare_wave.rs:78-111
 
 Everything computes on the device through libxrtk's C ABI.  The only CPU
-arithmetic is kirchhoff_auto's fallback when no gfx950 device exists, the
-reference's own behaviour (fallback.rs:24-25).
+arithmetic is the auto entries' branch when no gfx950 device exists, the
+reference's own behaviour: kirchhoff_auto's f64 loop (fallback.rs:24-25) and,
+for diffraction_integral / diffraction_integral_rs in their default f64 mode,
+the f64 Kahan loop those functions always run (diffraction.rs:56-117).
 """
 from __future__ import annotations
 
@@ -188,16 +190,26 @@ def _ray_count(rays: dict) -> int:
 
 
 def diffraction_integral(rays: dict, pixels, ctx: Optional[GpuContext] = None,
-                         mode="f32r", **tune) -> np.ndarray:
+                         mode="f64", **tune) -> np.ndarray:
     """diffraction_integral (src/waves/diffraction.rs:56): rays as a dict of
     the DiffractionRay columns (RAY_FIELDS), pixels as (px, py, pz) arrays or a
-    WaveParams.  Returns (n_pixels, 2) complex128 [(Es, Ep)]."""
-    ctx = ctx or GpuContext.shared()
+    WaveParams.  Returns (n_pixels, 2) complex128 [(Es, Ep)].
+
+    mode "f64" (the default, the reference's arithmetic) runs the plain FP64
+    device kernel; without a device (and no ctx given) it runs the
+    reference's f64 Kahan loop on the CPU, like the reference itself.
+    mode "f32r" runs the FP32 phase-reduced kernel (complex rel-L2 ~1e-6 to
+    ~1e-5 against FP64) and needs a device."""
+    m = _mode(mode)
+    if ctx is None:
+        ctx = GpuContext.shared()
+        if ctx is None and m == MODE_F64 and not tune:
+            return _diffraction_auto_f64(rays, pixels)
     if ctx is None:
         raise XrtkError(_lib.ENODEV, "diffraction_integral")
     n = _ray_count(rays)
     rc = _RayCols(rays, n)
-    o = _lib.opts(_mode(mode), **tune)
+    o = _lib.opts(m, **tune)
     if isinstance(pixels, WaveParams):
         w = pixels.to_c()
         out = np.zeros((pixels.nx * pixels.nz, 4), dtype=np.float64)
@@ -214,6 +226,23 @@ def diffraction_integral(rays: dict, pixels, ctx: Optional[GpuContext] = None,
     return _to_complex(out)
 
 
+def _diffraction_auto_f64(rays: dict, pixels) -> np.ndarray:
+    """xrtk_diffraction_auto_f64: the FP64 device kernel on the shared
+    context, or the reference's f64 Kahan loop on the CPU without a device."""
+    n = _ray_count(rays)
+    rc = _RayCols(rays, n)
+    if isinstance(pixels, WaveParams):
+        pixels = prepare_wave(pixels)
+    px, py, pz = (np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+                  for a in pixels)
+    if not (px.shape == py.shape == pz.shape):
+        raise ValueError("pixel coordinate arrays differ in length")
+    out = np.zeros((px.shape[0], 4), dtype=np.float64)
+    check(lib.xrtk_diffraction_auto_f64(C.byref(rc.s), n, dptr(px), dptr(py), dptr(pz),
+                                        px.shape[0], dptr(out)), "xrtk_diffraction_auto_f64")
+    return _to_complex(out)
+
+
 def _check_equal_lengths(pairs):
     """check_equal_lengths, crates/xrt-python/src/lib.rs:33-47."""
     if len(pairs) < 2:
@@ -227,11 +256,14 @@ def _check_equal_lengths(pairs):
 
 def diffraction_integral_rs(ray_x, ray_y, ray_z, ray_nx, ray_ny, ray_nz, ray_nl, ray_es_re,
                             ray_es_im, ray_ep_re, ray_ep_im, ray_energy, pix_x, pix_y, pix_z,
-                            mode="f32r"):
+                            mode="f64"):
     """diffraction_integral_rs (crates/xrt-python/src/lib.rs:321-394): fifteen
     float sequences in, (es_re, es_im, ep_re, ep_im) lists out, ValueError on
-    a length mismatch.  The device call runs without holding any Python lock
-    (ctypes releases the GIL)."""
+    a length mismatch.  The default f64 mode gives the reference's result
+    (plain FP64 device kernel, or the reference's f64 Kahan loop on the CPU
+    without a device); mode="f32r" opts into the FP32 phase-reduced kernel.
+    The library call runs without holding any Python lock (ctypes releases
+    the GIL)."""
     names = ["ray_x", "ray_y", "ray_z", "ray_nx", "ray_ny", "ray_nz", "ray_nl", "ray_es_re",
              "ray_es_im", "ray_ep_re", "ray_ep_im", "ray_energy"]
     cols = [ray_x, ray_y, ray_z, ray_nx, ray_ny, ray_nz, ray_nl, ray_es_re, ray_es_im,

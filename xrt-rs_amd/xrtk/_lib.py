@@ -130,6 +130,8 @@ PROTOTYPES = {
     "xrtk_kirchhoff_auto_f32": (_i, [_p, _sz, _p, _sz, _dp]),
     "xrtk_diffraction_f64": (_i, [_p, C.POINTER(XrtkRaysF64), _sz, _dp, _dp, _dp, _sz, _dp,
                                   C.POINTER(XrtkOpts)]),
+    "xrtk_diffraction_auto_f64": (_i, [C.POINTER(XrtkRaysF64), _sz, _dp, _dp, _dp, _sz, _dp]),
+    "xrtk_debug_fail_launches": (None, [C.c_uint32]),
     "xrtk_diffraction_grid_f64": (_i, [_p, C.POINTER(XrtkRaysF64), _sz,
                                        C.POINTER(XrtkWaveParams), _dp, C.POINTER(XrtkOpts)]),
     "xrtk_prepare_wave": (_i, [C.POINTER(XrtkWaveParams), _dp, _dp, _dp]),

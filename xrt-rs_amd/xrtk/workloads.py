@@ -16,6 +16,11 @@ normals, so a (seed, draw) pair gives the same value everywhere.
                          Fresnel zones for (E, L)
   batch_instances        64 uneven instances (16k-2M rays, 64^2-1024^2,
                          6-25 keV, 200-20,000 mm)
+  curved_aperture        rays on a toroidal cap, each with its own surface
+                         normal (the input a curved optic hands on)
+  grazing_mirror         the footprint of a grazing-incidence mirror, spread
+                         along the beam (a bundle that is not thin along the
+                         line of sight)
 """
 from __future__ import annotations
 
@@ -118,6 +123,65 @@ def gaussian_source(n_rays=1048576, half=0.2, energy=10000.0, sigma=(0.010, 0.00
     x = np.concatenate(xa)[:n_rays]
     z = np.concatenate(za)[:n_rays]
     return _rays_from_positions(x, z, sx, sz, energy)
+
+
+def _point_source_rays(x, y, z, n, dirn, energy):
+    """Rays at (x, y, z) with normals n (3, N) lit by a point source 10 m
+    upstream: Es = exp(i k d), Ep = 0, nl = n . dirn (dirn (3, N) the beam
+    direction the obliquity takes, prepare_wave.rs:86-87)."""
+    nr = x.shape[0]
+    d = np.sqrt(x * x + (y + SRC_DIST) ** 2 + z * z)
+    ph = k_of(energy) * d
+    zeros = np.zeros(nr)
+    return {
+        "x": x, "y": y, "z": z, "nx": n[0], "ny": n[1], "nz": n[2],
+        "nl": n[0] * dirn[0] + n[1] * dirn[1] + n[2] * dirn[2],
+        "es_re": np.cos(ph), "es_im": np.sin(ph),
+        "ep_re": zeros.copy(), "ep_im": zeros.copy(),
+        "energy": np.full(nr, float(energy)),
+    }
+
+
+def curved_aperture(n_rays=65536, half=0.1, energy=10000.0, radius=(2000.0, 5000.0), tilt=0.0,
+                    seed=1):
+    """Rays uniform on |x|, |z| <= half of the toroidal cap
+    y = x^2 / (2 Rx) + z^2 / (2 Rz), each carrying the surface normal at its
+    point, (-x/Rx, 1, -z/Rz) normalised (per-ray normals, as rays taken off a
+    curved optic carry, src/oes/reflect.rs:53-56), tilted further at random
+    by up to `tilt` rad about x and z (so that the obliquity term n.d/|d|
+    differs from ray to ray by ~tilt^2 / 2).  Lit by a point source 10 m
+    upstream."""
+    rng = SplitMix64(seed)
+    x = (rng.uniform(n_rays) * 2.0 - 1.0) * half
+    z = (rng.uniform(n_rays) * 2.0 - 1.0) * half
+    y = x * x / (2.0 * radius[0]) + z * z / (2.0 * radius[1])
+    tx = -x / radius[0] + (rng.uniform(n_rays) * 2.0 - 1.0) * tilt
+    tz = -z / radius[1] + (rng.uniform(n_rays) * 2.0 - 1.0) * tilt
+    m = np.sqrt(tx * tx + 1.0 + tz * tz)
+    n = (tx / m, 1.0 / m, tz / m)
+    d = np.sqrt(x * x + (y + SRC_DIST) ** 2 + z * z)
+    return _point_source_rays(x, y, z, n, (x / d, (y + SRC_DIST) / d, z / d), energy)
+
+
+def grazing_mirror(n_rays=65536, length=100.0, width=0.5, theta=3e-3, energy=10000.0,
+                   bend_radius=None, seed=1):
+    """The footprint of a mirror at grazing angle theta (rad) to the beam,
+    which runs along +y: rays at (x, s cos theta, s sin theta), s uniform on
+    +-length/2 and x on +-width/2, so the bundle spans `length` mm along the
+    line of sight.  Normals (0, -sin theta, cos theta), or, with
+    `bend_radius` (mm), those of the mirror bent to that radius along s
+    (per ray).  nl = n . (0, cos 2 theta, sin 2 theta), the reflected beam
+    direction.  Lit by a point source 10 m upstream.  The reflected beam
+    reaches y = L at z = L tan(2 theta)."""
+    rng = SplitMix64(seed)
+    s = (rng.uniform(n_rays) * 2.0 - 1.0) * (0.5 * length)
+    x = (rng.uniform(n_rays) * 2.0 - 1.0) * (0.5 * width)
+    c, sn = math.cos(theta), math.sin(theta)
+    y, z = s * c, s * sn
+    a = s / bend_radius if bend_radius else np.zeros(n_rays)  # local slope of the bend
+    n = (np.zeros(n_rays), -np.sin(theta + a), np.cos(theta + a))
+    refl = (0.0, math.cos(2 * theta), math.sin(2 * theta))
+    return _point_source_rays(x, y, z, n, refl, energy)
 
 
 def screen(distance=1000.0, half=0.02, side=256) -> Screen:
