@@ -274,7 +274,10 @@ def diffraction_integral_rs(ray_x, ray_y, ray_z, ray_nx, ray_ny, ray_nz, ray_nl,
     _check_equal_lengths([(nm, a.shape[0]) for nm, a in zip(("pix_x", "pix_y", "pix_z"), pxs)])
     rays = dict(zip(RAY_FIELDS, [arrs[0], arrs[1], arrs[2], arrs[3], arrs[4], arrs[5], arrs[6],
                                  arrs[7], arrs[8], arrs[9], arrs[10], arrs[11]]))
-    res = diffraction_integral(rays, tuple(pxs), mode=mode)
+    if _mode(mode) == MODE_F64:
+        res = _diffraction_auto_f64(rays, tuple(pxs))
+    else:
+        res = diffraction_integral(rays, tuple(pxs), mode=mode)
     es, ep = res[:, 0], res[:, 1]
     return (es.real.tolist(), es.imag.tolist(), ep.real.tolist(), ep.imag.tolist())
 
