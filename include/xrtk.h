@@ -136,6 +136,23 @@ int xrtk_kirchhoff_gpu_f32(xrtk_ctx *ctx,
                            const XrtkGpuPixel *pixels, size_t n_pixels,
                            double *out4, const XrtkOpts *opts);
 
+/* The same solve as xrtk_kirchhoff_gpu_f32 (same arguments, same result bit
+ * for bit), with every device buffer taken from a fixed 64 MiB block that
+ * the library holds on each device for the life of the process: the call
+ * makes no device allocation (no hipMalloc / hipFree).  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * A solve that needs more than the block is refused with XRTK_ENOMEM; a
+ * monochromatic solve of 65,536 rays on 65,536 pixels needs ~43 MiB
+ * (about 240 B per ray plus 150 B per pixel and 16 B per pixel per 4096-ray
+ * chunk).  Calls on one device share the block: calls through one context
+ * are serialised, calls through different contexts on one device must not
+ * overlap. */
+int xrtk_kirchhoff_gpu_f32_noalloc(xrtk_ctx *ctx,
+                                   const XrtkGpuRay *rays, size_t n_rays,
+                                   const XrtkGpuPixel *pixels, size_t n_pixels,
+                                   double *out4, const XrtkOpts *opts);
+
 /* kirchhoff_auto (src/gpu/fallback.rs:19-26): the shared context when a
  * device exists, else the f64 CPU loop of fallback.rs:29-71 (no Kahan, skip
  * below 1e-20 mm).  Setting XRTK_REQUIRE_GPU=1 in the environment turns the

@@ -56,6 +56,7 @@ int create_ctx(int device, xrtk_ctx** out) {
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return XRTK_ELAUNCH;
   load_tune_cache(c.get());
+  c->scratch = noalloc_scratch(device, &c->scratch_bytes);  // device is current here
   *out = c.release();
   return XRTK_OK;
 }
@@ -194,9 +195,9 @@ struct RaySrc {
   }
 };
 
-// Uploads 13 f64 columns (path_up may be null: zeros) into ctx->stage at
+// Uploads 13 f64 columns (path_up may be null: zeros) into `stage` at
 // offset 0, optionally permuted, and converts them into dst.
-int upload_cols(xrtk_ctx* ctx, const XrtkRaysF64* r, const double* path_up,
+int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* path_up,
                 const std::vector<uint32_t>* perm, uint32_t nr, RayRec64* dst) {
   if (!nr) return XRTK_OK;
   const double* cols[13] = {r->x,     r->y,     r->z,     r->nx,    r->ny,
@@ -205,8 +206,8 @@ int upload_cols(xrtk_ctx* ctx, const XrtkRaysF64* r, const double* path_up,
   for (int c = 0; c < 12; ++c)
     if (!cols[c]) return XRTK_EINVAL;
   const size_t cb = sizeof(double) * nr;
-  if (ctx->stage.cap < 13 * cb) return XRTK_EINVAL;  // caller sizes the stage
-  char* S = ctx->stage.as<char>();
+  if (stage.cap < 13 * cb) return XRTK_EINVAL;  // caller sizes the stage
+  char* S = stage.as<char>();
   std::vector<double> tmp;
   hipError_t e = hipSuccess;
   for (int c = 0; c < 13 && e == hipSuccess; ++c) {
@@ -228,8 +229,8 @@ int upload_cols(xrtk_ctx* ctx, const XrtkRaysF64* r, const double* path_up,
 }
 
 // The one-shot pipeline: group, route, upload, solve, copy back.
-int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip, double* out4,
-                  const XrtkOpts* opts) {
+int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& ps, double skip,
+                  double* out4, const XrtkOpts* opts) {
   const uint32_t want = mode_from(opts);
   const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
   // the F32R shape fixes the slot layout even when only the FP64 kernel runs
@@ -322,13 +323,13 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
                                   : align256(13 * sizeof(double) * (size_t)nr);
   const size_t pix_stage = ps.grid ? 0 : (ps.f32 ? sizeof(XrtkGpuPixel) * np
                                                  : 3 * sizeof(double) * (size_t)np);
-  hipError_t e = ctx->stage.ensure(ray_stage + pix_stage + 256);
-  if (e == hipSuccess) e = ctx->rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
-  if (e == hipSuccess) e = ctx->pix64.ensure(sizeof(PixF64) * npad);
-  if (e == hipSuccess) e = ctx->perm.ensure(sizeof(uint32_t) * npad);
-  if (e == hipSuccess) e = ctx->outbuf.ensure(sizeof(double) * 4 * np);
+  hipError_t e = ws.stage.ensure(ray_stage + pix_stage + 256);
+  if (e == hipSuccess) e = ws.rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
+  if (e == hipSuccess) e = ws.pix64.ensure(sizeof(PixF64) * npad);
+  if (e == hipSuccess) e = ws.perm.ensure(sizeof(uint32_t) * npad);
+  if (e == hipSuccess) e = ws.outbuf.ensure(sizeof(double) * 4 * np);
   if (e != hipSuccess) return XRTK_ENOMEM;
-  char* S = ctx->stage.as<char>();
+  char* S = ws.stage.as<char>();
   // ---- rays
   if (rs.f32 && nr) {
     std::vector<XrtkGpuRay> tmp;
@@ -340,13 +341,13 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
     }
     e = hipMemcpyAsync(S, src, sizeof(XrtkGpuRay) * nr, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-      e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ctx->rays64.as<RayRec64>(),
+      e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ws.rays64.as<RayRec64>(),
                                   nr, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp goes out of scope
     if (e != hipSuccess) return map_err(e);
   } else if (nr) {
-    const int rc = upload_cols(ctx, rs.f64, nullptr, perm.empty() ? nullptr : &perm, nr,
-                               ctx->rays64.as<RayRec64>());
+    const int rc = upload_cols(ctx, ws.stage, rs.f64, nullptr, perm.empty() ? nullptr : &perm, nr,
+                               ws.rays64.as<RayRec64>());
     if (rc != XRTK_OK) return rc;
   }
   // ---- pixels
@@ -355,15 +356,15 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
     uint32_t th, tw;
     grid_tile(c32.P, ps.grid->nx, ps.grid->nz, &th, &tw);
     e = launch_pix_grid(ps.grid->center, ps.grid->dx, ps.grid->dz, ps.grid->nx, ps.grid->nz, th,
-                        tw, npad, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(), ctx->stream);
-    d_perm = ctx->perm.as<uint32_t>();
+                        tw, npad, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), ctx->stream);
+    d_perm = ws.perm.as<uint32_t>();
   } else if (ps.f32) {
     e = hipMemcpyAsync(S + ray_stage, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice,
                        ctx->stream);
     if (e == hipSuccess)
-      e = launch_pix_from_f32aos(S + ray_stage, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(),
+      e = launch_pix_from_f32aos(S + ray_stage, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(),
                                  np, npad, (uint32_t)c32.P, ctx->stream);
-    d_perm = ctx->perm.as<uint32_t>();
+    d_perm = ws.perm.as<uint32_t>();
   } else {
     double* P3 = reinterpret_cast<double*>(S + ray_stage);
     const size_t cb = sizeof(double) * np;
@@ -372,9 +373,9 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
     if (e == hipSuccess)
       e = hipMemcpyAsync(P3 + 2 * np, ps.z, cb, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-      e = launch_pix_from_cols(P3, ctx->pix64.as<PixF64>(), ctx->perm.as<uint32_t>(), np, npad,
+      e = launch_pix_from_cols(P3, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), np, npad,
                                (uint32_t)c32.P, ctx->stream);
-    d_perm = ctx->perm.as<uint32_t>();
+    d_perm = ws.perm.as<uint32_t>();
   }
   if (e != hipSuccess) return map_err(e);
   // ---- instances
@@ -382,35 +383,35 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
   for (size_t gi = 0; gi < gs.size(); ++gi) {
     InstSpec s;
     s.mode = gs[gi].mode;
-    s.rays = ctx->rays64.as<RayRec64>() + gs[gi].off;
+    s.rays = ws.rays64.as<RayRec64>() + gs[gi].off;
     s.nr = gs[gi].n;
-    s.pix = ctx->pix64.as<PixF64>();
+    s.pix = ws.pix64.as<PixF64>();
     s.perm = d_perm;
     s.np = np;
     s.npad = npad;
     s.energy = s.mode == MODE_F32R ? gs[gi].energy : 0.0;  // F64: per-ray energies
     s.skip = skip;
-    s.out = ctx->outbuf.as<double>();
+    s.out = ws.outbuf.as<double>();
     s.accumulate = gi > 0;
     specs.push_back(s);
   }
   if (specs.empty()) {  // no rays: zeros
     InstSpec s;
     s.mode = want;
-    s.pix = ctx->pix64.as<PixF64>();
+    s.pix = ws.pix64.as<PixF64>();
     s.np = np;
     s.npad = npad;
-    s.out = ctx->outbuf.as<double>();
+    s.out = ws.outbuf.as<double>();
     specs.push_back(s);
   }
   Plan plan;
   const int split = opts && opts->split > 0 ? opts->split : 0;
   uint32_t force = 0;
   if (const char* f = std::getenv("XRTK_FORCE_FLAGS")) force = (uint32_t)std::strtoul(f, nullptr, 0);
-  int rc = run_instances(ctx, c32, c64, specs, ctx->work, ctx->meta, true, &plan, split, force,
+  int rc = run_instances(ctx, c32, c64, specs, ws.work, ws.meta, true, &plan, split, force,
                          false);
   if (rc != XRTK_OK) return rc;
-  e = hipMemcpyAsync(out4, ctx->outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
+  e = hipMemcpyAsync(out4, ws.outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
                      ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return map_err(e);
@@ -425,6 +426,32 @@ int solve_oneshot(xrtk_ctx* ctx, const RaySrc& rs, const PixSrc& ps, double skip
 }
 
 }  // namespace
+
+namespace xrtk {
+
+int kirchhoff_f32_ws(xrtk_ctx* ctx, Workspace& ws, const XrtkGpuRay* rays, size_t n_rays,
+                     const XrtkGpuPixel* pixels, size_t n_pixels, double* out4,
+                     const XrtkOpts* opts) {
+  if ((n_rays && !rays) || (n_pixels && (!pixels || !out4))) return XRTK_EINVAL;
+  if (n_rays > 0xffffffffull || n_pixels > 0x7fff0000ull) return XRTK_EINVAL;
+  if (n_pixels == 0) return XRTK_OK;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    RaySrc rs;
+    rs.f32 = rays;
+    rs.nr = n_rays;
+    PixSrc ps;
+    ps.f32 = pixels;
+    ps.np = n_pixels;
+    return solve_oneshot(ctx, ws, rs, ps, 1e-20 /* kirchhoff.wgsl:85, fallback.rs:48 */, out4,
+                         opts);
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
+}  // namespace xrtk
 
 extern "C" {
 
@@ -473,8 +500,7 @@ void xrtk_ctx_destroy(xrtk_ctx* ctx) {
     ge->outd.release();
     delete ge;
   }
-  ctx->stage.release(); ctx->rays64.release(); ctx->pix64.release(); ctx->perm.release();
-  ctx->work.release(); ctx->meta.release(); ctx->outbuf.release();
+  ctx->ws.release();
   fresnel_release(ctx);
   ctx->undulator.release();
   for (auto& kv : ctx->raysets) kv.second.rays.release();
@@ -493,22 +519,7 @@ int xrtk_kirchhoff_gpu_f32(xrtk_ctx* ctx, const XrtkGpuRay* rays, size_t n_rays,
                            const XrtkGpuPixel* pixels, size_t n_pixels, double* out4,
                            const XrtkOpts* opts) {
   if (!ctx) return XRTK_ENODEV;
-  if ((n_rays && !rays) || (n_pixels && (!pixels || !out4))) return XRTK_EINVAL;
-  if (n_rays > 0xffffffffull || n_pixels > 0x7fff0000ull) return XRTK_EINVAL;
-  if (n_pixels == 0) return XRTK_OK;
-  try {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    DeviceGuard g(ctx->device);
-    RaySrc rs;
-    rs.f32 = rays;
-    rs.nr = n_rays;
-    PixSrc ps;
-    ps.f32 = pixels;
-    ps.np = n_pixels;
-    return solve_oneshot(ctx, rs, ps, 1e-20 /* kirchhoff.wgsl:85, fallback.rs:48 */, out4, opts);
-  } catch (...) {
-    return XRTK_ENOMEM;
-  }
+  return kirchhoff_f32_ws(ctx, ctx->ws, rays, n_rays, pixels, n_pixels, out4, opts);
 }
 
 int xrtk_kirchhoff_auto_f32(const XrtkGpuRay* rays, size_t n_rays, const XrtkGpuPixel* pixels,
@@ -549,7 +560,7 @@ int xrtk_diffraction_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_rays,
     PixSrc ps;
     ps.x = pix_x; ps.y = pix_y; ps.z = pix_z;
     ps.np = n_pixels;
-    return solve_oneshot(ctx, rs, ps, 1e-30 /* diffraction.rs:81 */, out4, opts);
+    return solve_oneshot(ctx, ctx->ws, rs, ps, 1e-30 /* diffraction.rs:81 */, out4, opts);
   } catch (...) {
     return XRTK_ENOMEM;
   }
@@ -606,7 +617,7 @@ int xrtk_diffraction_grid_f64(xrtk_ctx* ctx, const XrtkRaysF64* rays, size_t n_r
     PixSrc ps;
     ps.grid = screen;
     ps.np = np;
-    return solve_oneshot(ctx, rs, ps, 1e-30, out4, opts);
+    return solve_oneshot(ctx, ctx->ws, rs, ps, 1e-30, out4, opts);
   } catch (...) {
     return XRTK_ENOMEM;
   }
@@ -682,10 +693,10 @@ int xrtk_rayset_upload(xrtk_ctx* ctx, const XrtkRaysF64* rays, const double* pat
         rs.box.add(rays->x[i], rays->y[i], rays->z[i]);
       }
     }
-    if (ctx->stage.ensure(13 * sizeof(double) * std::max<size_t>(n_rays, 1) + 256) != hipSuccess ||
+    if (ctx->ws.stage.ensure(13 * sizeof(double) * std::max<size_t>(n_rays, 1) + 256) != hipSuccess ||
         rs.rays.ensure(sizeof(RayRec64) * std::max<size_t>(n_rays, 1)) != hipSuccess)
       return XRTK_ENOMEM;
-    int rc = upload_cols(ctx, rays, path_up, nullptr, rs.nr, rs.rays.as<RayRec64>());
+    int rc = upload_cols(ctx, ctx->ws.stage, rays, path_up, nullptr, rs.nr, rs.rays.as<RayRec64>());
     if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
     if (rc != XRTK_OK) {
       rs.rays.release();

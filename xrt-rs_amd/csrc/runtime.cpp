@@ -30,6 +30,15 @@ int map_err(hipError_t e) {
 
 hipError_t DBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return hipSuccess;
+  if (arena) {
+    if (p) return hipErrorOutOfMemory;  // a carved block does not grow
+    const size_t off = (arena->off + 255) & ~size_t(255);
+    if (off > arena->cap || bytes > arena->cap - off) return hipErrorOutOfMemory;
+    p = arena->base + off;
+    cap = bytes;
+    arena->off = off + bytes;
+    return hipSuccess;
+  }
   release();
   size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
   want = (want + 4095) & ~size_t(4095);
@@ -52,7 +61,7 @@ hipError_t DBuf::ensure(size_t bytes) {
 }
 
 void DBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !arena) (void)hipFree(p);
   p = nullptr;
   cap = 0;
 }

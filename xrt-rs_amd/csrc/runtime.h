@@ -55,14 +55,38 @@ constexpr int kPartBlocks = 240;                 // centroid partial blocks (dev
 // FP32 residual phase would carry more than ~1e-4 rad of rounding per pair.
 constexpr double kMaxResidualRev = 512.0;
 
-// Grow-only device buffer.
+// A fixed block of device memory handed out front to back (256-byte
+// aligned); nothing is ever allocated or freed through it.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t off = 0;
+};
+
+// Grow-only device buffer.  With `arena` set it is carved once from that
+// arena instead (ensure fails with hipErrorOutOfMemory when the arena is
+// short, release does nothing).
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;
   hipError_t ensure(size_t bytes);
   void release();
   template <class T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// Device buffers of one-shot solves: staged inputs, resident-form rays and
+// slots, per-instance work (FP32 records, tiles, partial sums), launch
+// metadata and the merged output.
+struct Workspace {
+  DBuf stage, rays64, pix64, perm, work, meta, outbuf;
+  void release() {
+    for (DBuf* b : {&stage, &rays64, &pix64, &perm, &work, &meta, &outbuf}) b->release();
+  }
+  void use_arena(Arena* a) {
+    for (DBuf* b : {&stage, &rays64, &pix64, &perm, &work, &meta, &outbuf}) b->arena = a;
+  }
 };
 
 // Axis-aligned bounding box.
@@ -124,7 +148,10 @@ struct xrtk_ctx {
   int n_cu = 256;
   bool shared = false;
   // one-shot workspace
-  xrtk::DBuf stage, rays64, pix64, perm, work, meta, outbuf;
+  xrtk::Workspace ws;
+  // this device's fixed scratch block (xrtk_kirchhoff_gpu_f32_noalloc)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
   // resident ray sets
   std::map<uint64_t, xrtk::RaySet> raysets;
   uint64_t next_rs = 1;
@@ -205,6 +232,15 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   bool upload = true);
 
 int map_err(hipError_t e);
+// kirchhoff_gpu's one-shot solve with its device buffers in `ws` (the
+// context's own workspace, or one over a fixed arena); takes the context
+// lock.
+int kirchhoff_f32_ws(xrtk_ctx* ctx, Workspace& ws, const XrtkGpuRay* rays, size_t n_rays,
+                     const XrtkGpuPixel* pixels, size_t n_pixels, double* out4,
+                     const XrtkOpts* opts);
+// The device address and size of device `device`'s fixed scratch block,
+// looked up once per device (noalloc.hip); nullptr when the lookup failed.
+void* noalloc_scratch(int device, size_t* bytes);
 double now_s();
 // Releases the Fresnel workspace and rocBLAS handle of a context.
 void fresnel_release(xrtk_ctx* ctx);

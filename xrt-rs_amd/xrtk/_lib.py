@@ -127,6 +127,7 @@ PROTOTYPES = {
     "xrtk_ctx_destroy": (None, [_p]),
     "xrtk_ctx_device_name": (_i, [_p, C.c_char_p, _sz]),
     "xrtk_kirchhoff_gpu_f32": (_i, [_p, _p, _sz, _p, _sz, _dp, C.POINTER(XrtkOpts)]),
+    "xrtk_kirchhoff_gpu_f32_noalloc": (_i, [_p, _p, _sz, _p, _sz, _dp, C.POINTER(XrtkOpts)]),
     "xrtk_kirchhoff_auto_f32": (_i, [_p, _sz, _p, _sz, _dp]),
     "xrtk_diffraction_f64": (_i, [_p, C.POINTER(XrtkRaysF64), _sz, _dp, _dp, _dp, _sz, _dp,
                                   C.POINTER(XrtkOpts)]),
