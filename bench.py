@@ -4,26 +4,29 @@ Headline workload (one step): the Gaussian-source case, 1,048,576 rays x 512^2
 pixels at 1000 mm, 10 keV (2.75e11 ray-pixel pairs), solved from a ray set
 resident in HBM: FP64 centroid + prep, FP32 phase-reduced pair kernel, FP64
 merge and base phase, result copied to the host.  Launch shape: that of the
-committed counter record (profiles/pair_kernel_traffic.json) when it was
+committed counter record (profiles/pair_kernel_counters.json) when it was
 taken on this workload, else the autotuner's (--autotune forces the tuner,
 --shape fixes one).  The same solve with the plain FP64 pair kernel runs
 beside it for the speed ratio and the error (complex rel-L2 and intensity
-error of FP32-reduced vs FP64 over all pixels).
+error of FP32-reduced vs FP64 over all pixels), and the same solve with
+Ep != 0 on every ray (no Ep = 0 specialisation) for the general rate.
 
 Further legs, reported in the same JSON line:
   batched   64 uneven instances (rays log-uniform 16k-2M, square screens
             log-uniform 64^2-1024^2, E ~ U(6, 25) keV, L log-uniform
             200-20000 mm) in one persistent launch replayed from a hipGraph,
-            launch shape picked by the batch autotuner;
+            launch shape picked by the batch autotuner; every instance
+            checked against the FP64 kernel on a pixel sample;
   serving   energy scans over those resident ray sets (1-50 energies per
             scan) arriving as a Poisson stream, admitted under a 200 GB HBM
             budget and batched continuously; p50/p99 latency from arrival to
-            result and the sustained pair rate, at the stated 200 req/s (20 s
-            window by default, --serve-seconds 60 for the full minute) and at
-            a fraction of the measured batched capacity;
+            result and the sustained pair rate, at the stated 200 req/s for
+            60 s and for 1000 requests arriving at half the measured batched
+            capacity;
   cpu       the oracle (the reference's f64 Kahan loop restated in C, OpenMP
             over pixels) timed on a strided pixel sample of the headline
-            workload, all threads and one thread;
+            workload, all threads and one thread, sin()/cos() and sincos(),
+            median of 3 runs;
   fresnel   fresnel_propagate of the same rays onto a 1024^2 screen (complex
             GEMM of FP64-phase factors), FP32 and FP64 factors, with the
             oracle's loop on a pixel sample;
@@ -55,10 +58,15 @@ PEAK_FP64 = 78.6            # TFLOP/s, MI355X FP64 vector
 PEAK_MFMA_FP32 = 157.3      # TFLOP/s, MI355X dense FP32 matrix
 PEAK_MFMA_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0           # GB/s
-# VALU issue rates of gfx950 in wave-instructions per CU per clock, measured
-# by xrt-rs_amd/probes/issue_rate (v_fmac_f32; mean of v_sin/v_cos_f32)
+# One VALU wave-instruction occupies the issue slot of 64 lanes x one FMA
+# (2 flops): the FP32 datasheet peak is 2 such wave-instructions per CU per
+# clock (256 CUs at 2.4 GHz).
+FLOP_PER_WAVE_INST = 128
+# Issue rates of gfx950 measured by a microbenchmark (16 independent chains
+# per lane, 8 waves per SIMD), wave-instructions per CU per clock: v_fmac_f32
+# 1.417, v_sin_f32 / v_cos_f32 ~0.435 (packed FP32 excluded).
 ISSUE_FMA, ISSUE_TRANS = 1.417, 0.435
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+COUNTER_FILE = os.path.join(ROOT, "profiles", "pair_kernel_counters.json")
 
 
 def log(*a):
@@ -94,18 +102,21 @@ def copy_probe(mib):
     log(f"copy probe: 3 launches, each {mib} MiB read + {mib} MiB written")
 
 
-def traffic_record(nr, npx, plan):
-    """HBM bytes per launch of the pair kernel from the committed rocprofv3
-    counter summary, when it was taken on this workload and plan."""
+def counter_record(nr, npx, plan, key="headline"):
+    """The committed rocprofv3 counter record of a pair kernel (instructions
+    and HBM bytes per launch), when it was taken on this workload, launch
+    shape and variant; None otherwise.  plan None: match the workload only."""
     try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
+        with open(COUNTER_FILE) as f:
+            t = json.load(f).get(key)
+    except (OSError, ValueError, AttributeError):
+        return None
+    if not t:
         return None
     w = t.get("workload", {})
     if w.get("rays") != nr or w.get("pixels") != npx:
         return None
-    if plan is None:  # the workload alone
+    if plan is None:
         return t
     if any(w.get("plan", {}).get(k) != plan.get(k)
            for k in ("pixels_per_lane", "waves_per_group", "tile_rays", "persistent")):
@@ -113,6 +124,49 @@ def traffic_record(nr, npx, plan):
     if t.get("variant_flags") != plan.get("inst_flags"):
         return None
     return t
+
+
+def valu_roofline(rec, kernel_ms, pairs):
+    """The pair kernel against the FP32 VALU roofline, by the work it
+    executes: its VALU wave-instructions per launch (committed counters),
+    each an FMA issue slot of 64 lanes (128 flops), over its measured launch
+    time, against the 157.3 TFLOP/s datasheet peak (2 wave-instructions per
+    CU per clock).  frac <= 1 by construction."""
+    roof = {"bound": "valu", "achieved": None, "peak": PEAK_FP32, "unit": "TFLOP/s",
+            "frac": None, "traffic": None, "kernel_ms": kernel_ms,
+            "equivalent_tflops_51": pairs * F_PAIR / (kernel_ms * 1e-3) / 1e12,
+            "note": "achieved = VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU, "
+                    "committed record) x 128 flops (64 lanes x FMA) / measured launch time, "
+                    "against the FP32 datasheet peak; equivalent_tflops_51 counts the "
+                    "reference formula's 51 flops per pair instead (the kernel executes "
+                    "fewer operations per pair than the formula, so that figure is not a "
+                    "utilisation)"}
+    if rec is None:
+        roof["note"] += "; no counter record matches this build's plan: achieved unmeasured"
+        return roof
+    vi = rec["valu_insts_per_dispatch"]
+    roof["achieved"] = vi * FLOP_PER_WAVE_INST / (kernel_ms * 1e-3) / 1e12
+    roof["frac"] = roof["achieved"] / PEAK_FP32
+    roof["valu_insts_per_launch"] = vi
+    roof["valu_insts_per_pair"] = vi * 64 / pairs
+    ti = rec.get("trans_insts_per_dispatch")
+    if ti is not None:
+        roof["trans_insts_per_pair"] = ti * 64 / pairs
+    if rec.get("hbm_bytes_per_launch"):
+        roof["traffic"] = rec["hbm_bytes_per_launch"]
+        roof["algorithmic_bytes_per_launch"] = rec.get("algorithmic_bytes_per_launch")
+        roof["traffic_gbps"] = rec["hbm_bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9
+        roof["traffic_frac_hbm"] = roof["traffic_gbps"] / PEAK_HBM
+    if rec.get("n_cu"):
+        clk = rec.get("sclk_ghz", 2.4) * 1e9
+        rate = vi / (kernel_ms * 1e-3) / rec["n_cu"] / clk
+        roof["valu_issue_per_cu_per_clk"] = rate
+        if ti is not None:
+            # share of the issue capacity measured for these instruction kinds
+            t_rate = ti / (kernel_ms * 1e-3) / rec["n_cu"] / clk
+            roof["issue_frac_measured_rates"] = (rate - t_rate) / ISSUE_FMA + t_rate / ISSUE_TRANS
+    roof["counter_source"] = rec.get("source")
+    return roof
 
 
 # ---------------------------------------------------------------------------
@@ -130,7 +184,7 @@ def leg_headline(args, ctx, xrtk, W):
     # wave per group and four within noise of each other), so the roofline's
     # traffic and issue figures describe the kernel that runs; else, or with
     # --autotune, the autotuner's (untimed)
-    rec = None if args.autotune else traffic_record(nr, npx, None)
+    rec = None if args.autotune else counter_record(nr, npx, None)
     if args.shape:
         P, Wv, per = (int(x) for x in args.shape.split(","))
         tuned = {"pixels_per_lane": P, "waves_per_group": Wv, "persistent": per,
@@ -176,8 +230,40 @@ def leg_headline(args, ctx, xrtk, W):
         k_ms.append(ctx.last_timing()[0])
     kernel_ms = float(np.median(k_ms))
     plan = ctx.last_plan()
-    achieved = pairs * F_PAIR / (kernel_ms * 1e-3) / 1e12
-    log(f"pair kernel: {kernel_ms:.1f} ms/launch  {achieved:.1f} TFLOP/s  plan {plan}")
+    log(f"pair kernel: {kernel_ms:.1f} ms/launch  plan {plan}")
+
+    # the same rays with Ep != 0 (Ep = 0.5 e^{i pi/5} Es on every ray): the
+    # general sums, without the Ep = 0 specialisation.  Es is the same field
+    # (to FP32 rounding) and Ep = 0.5 e^{i pi/5} Es exactly in exact
+    # arithmetic: both are checked.
+    ep_leg = None
+    if not args.no_ep_leg:
+        ratio = 0.5 * np.exp(1j * np.pi / 5)
+        es = rays["es_re"] + 1j * rays["es_im"]
+        rays_ep = dict(rays, ep_re=(ratio * es).real, ep_im=(ratio * es).imag)
+        rs_ep = xrtk.RaySet(ctx, rays_ep)
+        job_ep = xrtk.Job(rs_ep, 10000.0, wave)
+        xrtk.solve_jobs(ctx, [job_ep], mode="f32r", **shape)
+        device_sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            (oep,) = xrtk.solve_jobs(ctx, [job_ep], mode="f32r", **shape)
+        device_sync()
+        t_ep = (time.perf_counter() - t) / args.steps
+        xrtk.diffraction_integral(rays_ep, wave, ctx=ctx, mode="f32r", **shape)
+        ep_flags = ctx.last_plan()["inst_flags"]
+        kms_ep = ctx.last_timing()[0]
+        rs_ep.release()
+        ep_leg = {"value": pairs / t_ep / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * t_ep,
+                  "kernel_ms": kms_ep, "inst_flags": ep_flags,
+                  "rel_l2_es_vs_ep0_solve": W.rel_l2(oep[:, 0], out32[:, 0]),
+                  "rel_l2_ep_vs_ratio_es": W.rel_l2(oep[:, 1], ratio * oep[:, 0]),
+                  "what": "the headline solve with Ep = 0.5 e^{i pi/5} Es on every ray: "
+                          "Es and Ep sums both formed (no Ep = 0 specialisation)"}
+        log(f"ep != 0: {1e3 * t_ep:.1f} ms/step, {ep_leg['value']:.0f} Gpair/s, pair kernel "
+            f"{kms_ep:.1f} ms, flags {ep_flags:#x}, Es vs Ep=0 solve "
+            f"{ep_leg['rel_l2_es_vs_ep0_solve']:.2e}, Ep vs ratio*Es "
+            f"{ep_leg['rel_l2_ep_vs_ratio_es']:.2e}")
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
@@ -218,35 +304,28 @@ def leg_headline(args, ctx, xrtk, W):
                            "merge, result copy (PCIe-inclusive)"}
         log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call")
 
-    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32,
-            "flop_per_pair": F_PAIR, "kernel_ms": kernel_ms, "traffic": None,
-            "note": "bound 'valu': the pair kernel is limited by FP32 vector-ALU issue (no "
-                    "matrix instructions; its HBM traffic is ~0.2 % of 8 TB/s), priced against "
-                    "the 157.3 TFLOP/s FP32 peak; achieved counts the reference formula's 51 "
-                    "flop per pair; the kernel executes fewer operations per pair (reduced "
-                    "phase, separable amplitude, slot-line rotation), so frac can exceed 1; "
-                    "issue_frac is the share of the VALU issue capacity its own instructions "
-                    "use"}
-    t = traffic_record(nr, npx, plan)
-    if t is not None:
-        roof["traffic"] = t["hbm_bytes_per_launch"]
-        roof["traffic_gbps"] = t["hbm_bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9
-        roof["traffic_frac_hbm"] = roof["traffic_gbps"] / PEAK_HBM
-        roof["valu_insts_per_pair"] = t.get("valu_insts_per_pair")
-        roof["trans_insts_per_pair"] = t.get("trans_insts_per_pair")
-        if t.get("sclk_ghz") and t.get("n_cu"):
-            # wave-instructions per CU per clock over the probe's issue rates
-            waves_s = pairs / (kernel_ms * 1e-3) / 64.0 / (t["n_cu"] * t["sclk_ghz"] * 1e9)
-            vt, tt = t["valu_insts_per_pair"], t["trans_insts_per_pair"]
-            roof["issue_frac"] = waves_s * ((vt - tt) / ISSUE_FMA + tt / ISSUE_TRANS)
+    roof = valu_roofline(counter_record(nr, npx, plan), kernel_ms, pairs)
+    if roof["achieved"] is not None:
+        log(f"roofline: {roof['achieved']:.1f} TFLOP/s of VALU issue slots "
+            f"({roof['frac']:.3f} of {PEAK_FP32}), {roof['valu_insts_per_pair']:.2f} VALU "
+            f"instructions per pair; formula-equivalent {roof['equivalent_tflops_51']:.0f} TFLOP/s")
+    if fp64 is not None:
+        rec64 = counter_record(nr, npx, None, key="fp64")
+        fp64["counters"] = None if rec64 is None else {
+            k: rec64.get(k) for k in ("kernel", "valu_insts_per_dispatch",
+                                      "trans_insts_per_dispatch", "valu_insts_per_pair",
+                                      "trans_insts_per_pair", "source")}
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
             "plan": plan, "shape_from": shape_from, "roofline": roof, "fp64": fp64,
-            "oneshot": oneshot}
+            "oneshot": oneshot, "ep_nonzero": ep_leg}
     return head, rays, wave, out32, rs
 
 
 def leg_cpu(args, xrtk, W, rays, wave, out32, nr, npx):
+    """The oracle (the reference's f64 Kahan loop) on a strided pixel sample
+    of the headline workload: all threads and one thread, each with sin() /
+    cos() and with sincos() (the same bits), median of 3 runs per form, the
+    faster form reported."""
     import pyoracle
     threads = min(16, len(os.sched_getaffinity(0)))
     px, py, pz = xrtk.prepare_wave(wave)
@@ -256,31 +335,47 @@ def leg_cpu(args, xrtk, W, rays, wave, out32, nr, npx):
     pyoracle.diffraction_f64(rays, px, py, pz, stride=stride0, threads=threads)
     n0 = (npx + stride0 - 1) // stride0
     per_px = (time.perf_counter() - t) / n0
-    want = max(threads, int(args.cpu_seconds / per_px))
+    # 2 forms x 3 runs share the all-thread budget
+    want = max(threads, int(args.cpu_seconds / 6 / per_px))
     stride = max(1, npx // want)
     nsamp = (npx + stride - 1) // stride
     log(f"cpu baseline: {threads} threads ({cpu_model()}), {nsamp} pixels (stride {stride}), "
-        f"est {nsamp * per_px:.1f}s")
-    t = time.perf_counter()
-    ref = pyoracle.diffraction_f64(rays, px, py, pz, stride=stride, threads=threads)
-    dt = time.perf_counter() - t
-    rate = nsamp * nr / dt / 1e9
-    # one thread, ~cpu_seconds / 4 of work
-    n1 = max(1, int(args.cpu_seconds / 4 / (per_px * threads)))
+        f"est {6 * nsamp * per_px:.1f}s over 2 forms x 3 runs")
+
+    def timed(stride_, threads_, trig):
+        runs, ref_ = [], None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref_ = pyoracle.diffraction_f64(rays, px, py, pz, stride=stride_, threads=threads_,
+                                            trig=trig)
+            runs.append(time.perf_counter() - t0)
+        return float(np.median(runs)), runs, ref_
+
+    forms = {}
+    for trig in ("sin_cos", "sincos"):
+        dt, runs, ref = timed(stride, threads, trig)
+        forms[trig] = {"value": nsamp * nr / dt / 1e9, "median_s": dt,
+                       "runs_s": [round(r, 3) for r in runs]}
+    best = max(forms, key=lambda k: forms[k]["value"])
+    rate = forms[best]["value"]
+    # one thread, the faster form, ~1/4 of the budget in 3 runs
+    n1 = max(1, int(args.cpu_seconds / 12 / (per_px * threads)))
     stride1 = max(1, npx // n1)
     n1 = (npx + stride1 - 1) // stride1
-    t = time.perf_counter()
-    pyoracle.diffraction_f64(rays, px, py, pz, stride=stride1, threads=1)
-    dt1 = time.perf_counter() - t
+    dt1, runs1, _ = timed(stride1, 1, best)
     rate1 = n1 * nr / dt1 / 1e9
     cpu = {"value": rate, "unit": "Gpair-evals/s", "cores": threads, "kind": "port",
            "sample": f"{nsamp} pixels (every {stride}th of the {npx}-pixel screen) x {nr} rays, "
-                     f"{dt:.1f} s, f64 Kahan oracle, OpenMP over pixels",
+                     f"f64 Kahan oracle, OpenMP over pixels, median of 3 runs "
+                     f"({forms[best]['median_s']:.2f} s), {best} (the faster form)",
+           "trig_forms": forms, "trig_form_reported": best,
            "cpu_model": cpu_model(), "per_core_value": rate1,
-           "per_core_sample": f"{n1} pixels x {nr} rays on 1 thread, {dt1:.1f} s",
+           "per_core_sample": f"{n1} pixels x {nr} rays on 1 thread, median of 3 runs "
+                              f"({dt1:.2f} s), {best}",
            "rel_l2_f32r_vs_oracle_on_sample": W.rel_l2(out32[::stride], ref)}
-    log(f"cpu: {rate:.4f} Gpair/s on {threads} threads, {rate1:.4f} on 1; rel-L2 f32r vs "
-        f"oracle on sample {cpu['rel_l2_f32r_vs_oracle_on_sample']:.3e}")
+    log(f"cpu: {rate:.4f} Gpair/s on {threads} threads ({best}; sin_cos "
+        f"{forms['sin_cos']['value']:.4f}, sincos {forms['sincos']['value']:.4f}), {rate1:.4f} "
+        f"on 1; rel-L2 f32r vs oracle on sample {cpu['rel_l2_f32r_vs_oracle_on_sample']:.3e}")
     return cpu
 
 
@@ -316,12 +411,20 @@ def leg_batched(args, ctx, xrtk, W):
     value = pairs / dt / 1e9
     log(f"batched: {dt * 1e3:.0f} ms/step, {value:.1f} Gpair/s (device graph "
         f"{np.median(dev):.0f} ms), plan {plan}")
-    # error of the smallest instances against the FP64 kernel
-    order = sorted(range(len(specs)), key=lambda i: specs[i]["n_rays"] * specs[i]["side"] ** 2)
+    # every instance against the plain FP64 kernel on a strided sample of its
+    # pixels (~args.batch_check_pixels each), from the same rays
     errs = []
-    for i in order[:args.batch_check]:
-        (ref,) = xrtk.solve_jobs(ctx, [jobs[i]], mode="f64")
-        errs.append(W.rel_l2(outs[i], ref))
+    t = time.perf_counter()
+    for i, s in enumerate(specs):
+        scr = W.instance_screen(s)
+        px, py, pz = xrtk.prepare_wave(grid_of(xrtk, scr))
+        stride = max(1, px.shape[0] // args.batch_check_pixels)
+        ref = xrtk.diffraction_integral(W.instance_rays(s), (px[::stride], py[::stride],
+                                                             pz[::stride]), ctx=ctx, mode="f64")
+        errs.append(W.rel_l2(outs[i][::stride], ref))
+    log(f"batched check: all {len(errs)} instances vs the FP64 kernel on pixel samples in "
+        f"{time.perf_counter() - t:.1f}s, max rel-L2 {max(errs):.2e} (instance "
+        f"{int(np.argmax(errs))}: {specs[int(np.argmax(errs))]})")
     res = {"value": value, "unit": "Gpair-evals/s", "ms_per_step": dt * 1e3,
            "device_ms_per_step": float(np.median(dev)), "steps": args.batch_steps,
            "warmup": args.batch_warmup, "instances": len(specs), "pairs": pairs,
@@ -330,7 +433,11 @@ def leg_batched(args, ctx, xrtk, W):
            "screen_side": "log-uniform 64..1024", "plan": plan, "autotuned": tuned,
            "n_fp64_routed": plan.get("n_fp64"),
            "rel_l2_vs_f64_max": max(errs) if errs else None,
-           "rel_l2_checked_instances": len(errs), "graph": True}
+           "rel_l2_vs_f64_median": float(np.median(errs)) if errs else None,
+           "rel_l2_checked_instances": len(errs),
+           "rel_l2_check": f"every instance, ~{args.batch_check_pixels} strided pixels each, "
+                           "against the plain FP64 kernel on the same rays",
+           "graph": True}
     return res, sets, jobs, specs
 
 
@@ -464,22 +571,18 @@ def energy_scan_stream(rng, n_sets, base_energy, n_req, concurrent=16):
 
 
 def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=None, load=None,
-                capacity=None, policy="shortest"):
+                capacity=None, policy="shortest", n_requests=None):
     """Poisson arrivals at `rate` req/s over `seconds`; or, given `load`, the
-    requests whose pairs fill load x capacity x seconds, arriving as a
-    Poisson stream whose rate makes the offered pair rate exactly
-    load x capacity."""
+    first `n_requests` requests of the stream, arriving as a Poisson stream
+    whose rate makes the offered pair rate exactly load x capacity."""
     rng = np.random.default_rng(seed)
     pairs_of = [float(s["n_rays"] * s["side"] ** 2) for s in specs]
     reqs = energy_scan_stream(rng, len(sets), [s["energy"] for s in specs], 20000)
     mean_pairs = float(np.mean([pairs_of[i] for i, _ in reqs]))
     if rate is None:
-        budget_pairs = load * capacity * seconds
-        n, tot = 0, 0.0
-        while n < len(reqs) and tot < budget_pairs:
-            tot += pairs_of[reqs[n][0]]
-            n += 1
+        n = min(len(reqs), int(n_requests))
         reqs = reqs[:n]
+        tot = sum(pairs_of[i] for i, _ in reqs)
         rate = load * capacity * n / tot
         arrivals = list(np.cumsum(rng.exponential(1.0 / rate, n)))
         seconds = arrivals[-1]
@@ -617,13 +720,15 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-warmup", type=int, default=2)
-    ap.add_argument("--batch-check", type=int, default=4)
+    ap.add_argument("--batch-check-pixels", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-load", type=float, default=0.5)
-    ap.add_argument("--serve-load-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-load-requests", type=int, default=1000)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--no-serve-compare", action="store_true")
+    ap.add_argument("--serve-compare", action="store_true",
+                    help="also serve the sub-saturation stream in arrival order")
+    ap.add_argument("--no-ep-leg", action="store_true")
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--shape", default="", metavar="P,W,PERSISTENT",
@@ -674,14 +779,15 @@ def main():
             cap = batched["value"] * 1e9
             serving = {"stated": leg_serving(args, ctx, xrtk, W, sets, specs, args.serve_seconds,
                                              7, budget, rate=args.serve_rate, capacity=cap)}
-            serving["sustainable"] = leg_serving(args, ctx, xrtk, W, sets, specs,
-                                                 args.serve_load_seconds, 8, budget,
-                                                 load=args.serve_load, capacity=cap)
+            serving["sustainable"] = leg_serving(args, ctx, xrtk, W, sets, specs, None, 8,
+                                                 budget, load=args.serve_load, capacity=cap,
+                                                 n_requests=args.serve_load_requests)
             serving["sustainable"]["load_fraction_of_batched_capacity"] = args.serve_load
-            if not args.no_serve_compare:
+            if args.serve_compare:
                 # the same request stream with requests taken in arrival order
-                s = leg_serving(args, ctx, xrtk, W, sets, specs, args.serve_load_seconds, 8,
-                                budget, load=args.serve_load, capacity=cap, policy="arrival")
+                s = leg_serving(args, ctx, xrtk, W, sets, specs, None, 8, budget,
+                                load=args.serve_load, capacity=cap, policy="arrival",
+                                n_requests=args.serve_load_requests)
                 serving["sustainable_arrival_order"] = {
                     k: s[k] for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "slowdown_p50",
                                       "slowdown_p99", "largest_request_latency_ms",
@@ -708,6 +814,7 @@ def main():
                    "plan": head["plan"], "shape_from": head["shape_from"]},
         "roofline": head["roofline"],
         "fp64": head["fp64"],
+        "ep_nonzero": head["ep_nonzero"],
         "oneshot_host_buffers": head["oneshot"],
         "cpu_baseline": cpu,
         "batched": batched,

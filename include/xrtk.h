@@ -323,6 +323,7 @@ typedef struct XrtkPlan {
 #define XRTK_VAR_SEPAMP    128 /* amplitude = ray factor x pixel factor          */
 #define XRTK_VAR_LINE      256 /* phase along a lane's pixel line by rotation    */
 #define XRTK_VAR_LINEDEV   1024 /* LINE with per-pixel offsets from the line     */
+#define XRTK_VAR_EPZERO    2048 /* every ray has Ep = 0: no Ep sums are formed    */
 int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);
 /* Backend used by the last xrtk_kirchhoff_auto_f32 call of this process:
  * 1 = device, 0 = CPU fallback, -1 = none yet. */

@@ -9,6 +9,10 @@
  *   oracle_diffraction_f64   src/waves/diffraction.rs:56-117  (f64, complex
  *                            Kahan sums, skip path < 1e-30, rayon over pixels
  *                            -> OpenMP over pixels; output order = pixel order)
+ *   oracle_diffraction_f64_sincos  the same with one glibc sincos() call per
+ *                            pair instead of sin() and cos() (the lowering a
+ *                            compiler may pick for Rust's sin_cos); the same
+ *                            bits, timed beside it for the CPU baseline
  *   oracle_fallback_f32      src/gpu/fallback.rs:29-71 and the test helper
  *                            tests/golden_gpu_kirchhoff.rs:68-99 (f32 inputs
  *                            promoted to f64, skip < 1e-20, no Kahan)
@@ -35,6 +39,7 @@
  * ray; Es and Ep are multiplied by exp(i k path_up) before the pixel sum, the
  * term beam_to_diffraction_rays leaves to the caller (prepare_wave.rs:78-111).
  */
+#define _GNU_SOURCE /* sincos */
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -48,7 +53,7 @@ typedef struct {
 
 /* diffraction_at_pixel, diffraction.rs:67-117 */
 static void pixel_f64(const oracle_rays *r, const double *path_up, size_t nr, double px,
-                      double py, double pz, double *o4) {
+                      double py, double pz, int use_sincos, double *o4) {
   double es_re = 0, es_im = 0, ep_re = 0, ep_im = 0;     /* sums         */
   double ces_re = 0, ces_im = 0, cep_re = 0, cep_im = 0; /* compensators */
   for (size_t j = 0; j < nr; ++j) {
@@ -60,7 +65,13 @@ static void pixel_f64(const oracle_rays *r, const double *path_up, size_t nr, do
     const double ns = (r->nx[j] * dx + r->ny[j] * dy + r->nz[j] * dz) / path; /* :86 */
     const double k = r->energy[j] / CHBAR * 1e7;                               /* :89 */
     const double phase = k * path;                                             /* :92 */
-    const double sp = sin(phase), cp = cos(phase);                            /* :93 */
+    double sp, cp;                                                             /* :93 */
+    if (use_sincos) {
+      sincos(phase, &sp, &cp);
+    } else {
+      sp = sin(phase);
+      cp = cos(phase);
+    }
     const double obl = r->nl[j] + ns;                                          /* :96 */
     const double amp = k / (4.0 * PI_F64) * obl / path;                       /* :97 */
     /* u = i * amp * (cos + i sin) = (-amp sin, amp cos)                         :98 */
@@ -98,9 +109,9 @@ static void pixel_f64(const oracle_rays *r, const double *path_up, size_t nr, do
 /* diffraction_integral, diffraction.rs:56-64.  Pixels are independent, so the
  * result does not depend on the thread count.  pix_stride > 1 evaluates every
  * pix_stride-th pixel only (output packed). */
-void oracle_diffraction_f64(const double *const cols[12], const double *path_up, size_t nr,
+static void diffraction_f64(const double *const cols[12], const double *path_up, size_t nr,
                             const double *px, const double *py, const double *pz, size_t np,
-                            size_t pix_stride, int threads, double *out4) {
+                            size_t pix_stride, int threads, int use_sincos, double *out4) {
   oracle_rays r = {cols[0], cols[1], cols[2], cols[3], cols[4], cols[5],
                    cols[6], cols[7], cols[8], cols[9], cols[10], cols[11]};
   if (pix_stride < 1) pix_stride = 1;
@@ -108,8 +119,21 @@ void oracle_diffraction_f64(const double *const cols[12], const double *path_up,
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : 1)
   for (long long q = 0; q < nout; ++q) {
     const size_t p = (size_t)q * pix_stride;
-    pixel_f64(&r, path_up, nr, px[p], py[p], pz[p], out4 + 4 * q);
+    pixel_f64(&r, path_up, nr, px[p], py[p], pz[p], use_sincos, out4 + 4 * q);
   }
+}
+
+void oracle_diffraction_f64(const double *const cols[12], const double *path_up, size_t nr,
+                            const double *px, const double *py, const double *pz, size_t np,
+                            size_t pix_stride, int threads, double *out4) {
+  diffraction_f64(cols, path_up, nr, px, py, pz, np, pix_stride, threads, 0, out4);
+}
+
+void oracle_diffraction_f64_sincos(const double *const cols[12], const double *path_up,
+                                   size_t nr, const double *px, const double *py,
+                                   const double *pz, size_t np, size_t pix_stride, int threads,
+                                   double *out4) {
+  diffraction_f64(cols, path_up, nr, px, py, pz, np, pix_stride, threads, 1, out4);
 }
 
 /* fallback.rs:29-71 over GpuRay-typed (f32) inputs: 12 floats per ray in the

@@ -28,6 +28,8 @@ def _load():
     lib.oracle_diffraction_f64.argtypes = [C.POINTER(dp), dp, C.c_size_t, dp, dp, dp,
                                            C.c_size_t, C.c_size_t, C.c_int, dp]
     lib.oracle_diffraction_f64.restype = None
+    lib.oracle_diffraction_f64_sincos.argtypes = lib.oracle_diffraction_f64.argtypes
+    lib.oracle_diffraction_f64_sincos.restype = None
     lib.oracle_diffraction_ld.argtypes = lib.oracle_diffraction_f64.argtypes
     lib.oracle_diffraction_ld.restype = None
     lib.oracle_fallback_f32.argtypes = [fp, C.c_size_t, fp, C.c_size_t, C.c_int, dp]
@@ -65,9 +67,12 @@ def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def diffraction_f64(rays: dict, px, py, pz, path_up=None, stride=1, threads=None):
+def diffraction_f64(rays: dict, px, py, pz, path_up=None, stride=1, threads=None,
+                    trig="sin_cos"):
     """diffraction_integral (diffraction.rs:56-117) on the CPU; returns
-    (n_out, 2) complex128 for pixels 0, stride, 2*stride, ..."""
+    (n_out, 2) complex128 for pixels 0, stride, 2*stride, ...  trig
+    "sin_cos" calls sin() and cos() per pair, "sincos" one sincos() (the
+    same bits)."""
     cols = [np.ascontiguousarray(np.asarray(rays[f], dtype=np.float64)) for f in FIELDS]
     n = cols[0].shape[0]
     px, py, pz = (np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (px, py, pz))
@@ -78,8 +83,9 @@ def diffraction_f64(rays: dict, px, py, pz, path_up=None, stride=1, threads=None
     pu = (np.ascontiguousarray(np.asarray(path_up, dtype=np.float64))
           if path_up is not None else None)
     threads = threads or os.cpu_count() or 1
-    lib.oracle_diffraction_f64(arr, _d(pu) if pu is not None else None, n, _d(px), _d(py),
-                               _d(pz), npx, stride, int(threads), _d(out))
+    fn = {"sin_cos": lib.oracle_diffraction_f64, "sincos": lib.oracle_diffraction_f64_sincos}[trig]
+    fn(arr, _d(pu) if pu is not None else None, n, _d(px), _d(py), _d(pz), npx, stride,
+       int(threads), _d(out))
     return out.view(np.complex128).reshape(-1, 2)
 
 

@@ -353,7 +353,47 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
 
 
 VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP, VAR_LINE = 2, 16, 32, 64, 128, 256
-VAR_LINEDEV = 1024
+VAR_LINEDEV, VAR_EPZERO, VAR_UNIFORM_N = 1024, 2048, 8
+
+
+@pytest.mark.parametrize("form", ["line", "linedev", "per_slot", "per_ray_normal", "exact"])
+def test_ep_zero_variant_is_exact(ctx, monkeypatch, form):
+    """Rays with Ep = 0 everywhere: the kernels that skip the Ep sums give the
+    same bits as the general sums (EPZERO forbidden with 0x10000), in each
+    form that has them: the slot line, the line with float32 offsets, the
+    per-slot separable form, the per-ray-normal form; the form without the
+    paraxial expansion keeps the general sums (the same bits trivially)."""
+    rays = W.gaussian_source(65536, seed=2)
+    g = grid(W.screen(1000.0, 0.02, 96))
+    extra = {"line": 0, "linedev": 0, "per_slot": 0x2000, "per_ray_normal": 0x8000,
+             "exact": 0x800}[form]
+
+    def run(force):
+        if force:
+            monkeypatch.setenv("XRTK_FORCE_FLAGS", hex(force))
+        else:
+            monkeypatch.delenv("XRTK_FORCE_FLAGS", raising=False)
+        if form == "linedev":
+            px = xrtk.prepare_wave(g)
+            out = xrtk.kirchhoff_gpu(ctx, to_gpu_rays(rays), to_gpu_pixels(px), pixels_per_lane=8)
+        else:
+            out = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", pixels_per_lane=8)
+        monkeypatch.delenv("XRTK_FORCE_FLAGS", raising=False)
+        return out, ctx.last_plan()["inst_flags"]
+
+    fast, f1 = run(extra)
+    full, f2 = run(extra | 0x10000)
+    print(f"{form}: {f1:#x} vs {f2:#x}")
+    assert f1 & VAR_EPZERO and not f2 & VAR_EPZERO
+    want = {"line": VAR_LINE, "linedev": VAR_LINEDEV, "per_slot": VAR_SEPAMP,
+            "per_ray_normal": VAR_PAX, "exact": VAR_TILE}[form]
+    assert f1 & want
+    if form == "per_ray_normal":
+        assert not f1 & VAR_UNIFORM_N
+    if form in ("per_slot", "exact"):
+        assert not f1 & VAR_LINE
+    assert np.array_equal(fast, full)
+    assert np.all(fast[:, 1] == 0)
 
 
 @pytest.mark.parametrize("distance", [100.0, 1000.0, 30000.0])

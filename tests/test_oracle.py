@@ -30,6 +30,16 @@ def test_fixture_numerical():
     assert np.max(np.abs(got.imag - exp.imag)) < 1e-10
 
 
+def test_sincos_form_gives_the_same_bits():
+    """The sincos() form of the loop (timed beside sin()/cos() for the CPU
+    baseline) gives the same bits on the fixture and the mpmath cases."""
+    cases = [fixture_case()[:2]] + [(r, p) for _, r, p, _ in mp_cases()]
+    for rays, pix in cases:
+        a = O.diffraction_f64(rays, *pix)
+        b = O.diffraction_f64(rays, *pix, trig="sincos")
+        assert np.array_equal(a, b)
+
+
 def test_fixture_finite_nonzero():
     """golden_diffraction_finite_results (golden_diffraction.rs:18-77)."""
     rays, pix, _ = fixture_case()

@@ -150,7 +150,8 @@ static_assert(XRTK_VAR_CHECK == INST_F_CHECK && XRTK_VAR_TILE == INST_F_TILE &&
                   XRTK_VAR_PATHUP == INST_F_PATHUP && XRTK_VAR_UNIFORM_N == INST_F_UNIFORM_N &&
                   XRTK_VAR_NOFRACT == INST_F_NOFRACT && XRTK_VAR_LIN2 == INST_F_LIN2 &&
                   XRTK_VAR_PAX == INST_F_PAX && XRTK_VAR_SEPAMP == INST_F_SEPAMP &&
-                  XRTK_VAR_LINE == INST_F_LINE && XRTK_VAR_LINEDEV == INST_F_LINEDEV,
+                  XRTK_VAR_LINE == INST_F_LINE && XRTK_VAR_LINEDEV == INST_F_LINEDEV &&
+                  XRTK_VAR_EPZERO == INST_F_EPZERO,
               "variant flags of the ABI match the device planner's");
 
 void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {

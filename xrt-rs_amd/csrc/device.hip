@@ -639,7 +639,8 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 // force_flags: 0x100 forces TILE to (force & TILE), 0x200 forces CHECK,
 // 0x400 forbids LIN2, 0x800 forbids PAX, 0x1000 forbids SEPAMP, 0x2000
 // forbids LINE, 0x4000 forbids LINEDEV, 0x8000 forbids UNIFORM_N (every ray
-// then takes its own normal, as rays off a curved optic do).
+// then takes its own normal, as rays off a curved optic do), 0x10000 forbids
+// EPZERO.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags, uint32_t P) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
@@ -659,6 +660,8 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     v2 = kr * (2.0 * (ox * dx + oy * dy + oz * dz) + om * om * (om / den)) / den;
   }
   if (mode == MODE_F32R) {
+    // no ray has Ep != 0: the Ep sums are exact zeros
+    if (scr->ep_nonzero == 0u && !(force_flags & 0x10000u)) f |= INST_F_EPZERO;
     bool tile = v0 > 2.0;
     if (force_flags & 0x100u) tile = (force_flags & INST_F_TILE) != 0u;
     if (tile) f |= INST_F_TILE;
@@ -841,7 +844,10 @@ struct SlotRegs {
 //          applies to the summed partials: no per-pair amplitude.
 // Per pair (TILE, UNIN, NOFRACT, LIN2, PAX): 21 FP32 add/mul/FMA + sin, cos;
 // with SEP: 14 + sin, cos; without PAX: 20 + rsq, rcp, sin, cos.
-template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP>
+// EP: some ray has Ep != 0; without it the Ep sums (a[i][2], a[i][3]) stay
+// exact zeros and are not formed (4 of the 6 sum operations per slot).
+template <int P, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
+          bool EP = true>
 __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, float nt,
                                          const SlotRegs<P>& s, float ikr, float h,
                                          float (&a)[P][4]) {
@@ -893,10 +899,12 @@ __device__ __forceinline__ void pair_ray(const RayRec32& r, float B, float off, 
     }
     a[i][0] = fmaf(r.cs, wr, a[i][0]);
     a[i][1] = fmaf(r.cs, wi, a[i][1]);
-    a[i][2] = fmaf(r.pr, wr, a[i][2]);
-    a[i][2] = fmaf(-r.pi, wi, a[i][2]);
-    a[i][3] = fmaf(r.pr, wi, a[i][3]);
-    a[i][3] = fmaf(r.pi, wr, a[i][3]);
+    if constexpr (EP) {
+      a[i][2] = fmaf(r.pr, wr, a[i][2]);
+      a[i][2] = fmaf(-r.pi, wi, a[i][2]);
+      a[i][3] = fmaf(r.pr, wi, a[i][3]);
+      a[i][3] = fmaf(r.pi, wr, a[i][3]);
+    }
   }
 }
 
@@ -923,7 +931,7 @@ struct LineRegs {
 // DEV: slot i lies off the line by its own offset; its phase gains
 // o.w_i radians (w_i from the item prologue), applied to the slot's
 // e^{i phase} to first order: 5 more operations per slot (4 with LIN2).
-template <int P, bool NOFRACT, bool LIN2, bool DEV = false>
+template <int P, bool NOFRACT, bool LIN2, bool DEV = false, bool EP = true>
 __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
                                           const LineRegs& L, float h, float (&a)[P][4],
                                           const float (&dw)[P][3]) {
@@ -967,17 +975,19 @@ __device__ __forceinline__ void pair_line(const RayRec32& r, float B, float off,
     }
     a[i][0] = fmaf(r.cs, cc, a[i][0]);
     a[i][1] = fmaf(r.cs, ss, a[i][1]);
-    a[i][2] = fmaf(r.pr, cc, a[i][2]);
-    a[i][2] = fmaf(-r.pi, ss, a[i][2]);
-    a[i][3] = fmaf(r.pr, ss, a[i][3]);
-    a[i][3] = fmaf(r.pi, cc, a[i][3]);
+    if constexpr (EP) {
+      a[i][2] = fmaf(r.pr, cc, a[i][2]);
+      a[i][2] = fmaf(-r.pi, ss, a[i][2]);
+      a[i][3] = fmaf(r.pr, ss, a[i][3]);
+      a[i][3] = fmaf(r.pi, cc, a[i][3]);
+    }
   }
 }
 
 // Slots first .. first + PH - 1 of each lane of one work item (the wave's
 // tile holds P slots per lane).
 template <int P, int PH, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX,
-          bool SEP, bool LINE, bool DEV>
+          bool SEP, bool LINE, bool DEV, bool EP>
 __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave, uint32_t first) {
   const uint32_t npad = I.npad;
@@ -1103,7 +1113,7 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
           const TileRay tr = my[jj];
-          pair_line<PH, NOFRACT, LIN2, DEV>(r, tr.B, tr.off, L, h, a, dw);
+          pair_line<PH, NOFRACT, LIN2, DEV, EP>(r, tr.B, tr.off, L, h, a, dw);
         }
       } else {
         asm volatile(".p2align 6");  // (as above)
@@ -1111,7 +1121,8 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
         for (uint32_t jj = 0; jj < nb; ++jj) {
           const RayRec32 r = rays[b0 + jj];
           const TileRay tr = my[jj];
-          pair_ray<PH, true, UNIN, NOFRACT, LIN2, PAX, SEP>(r, tr.B, tr.off, tr.n, s, ikr, h, a);
+          pair_ray<PH, true, UNIN, NOFRACT, LIN2, PAX, SEP, EP>(r, tr.B, tr.off, tr.n, s, ikr, h,
+                                                                a);
         }
       }
     }
@@ -1119,7 +1130,7 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
 #pragma unroll 2
     for (uint32_t j = r0; j < r1; ++j) {
       const RayRec32 r = rays[j];
-      pair_ray<PH, false, UNIN, NOFRACT, LIN2, false, false>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
+      pair_ray<PH, false, UNIN, NOFRACT, LIN2, false, false, EP>(r, 0.f, 0.f, 0.f, s, ikr, h, a);
     }
   }
   // (SEP: k_merge applies the slot factor of the amplitude)
@@ -1127,7 +1138,7 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
 #pragma unroll
   for (int i = 0; i < PH; ++i) {
     const uint32_t p = sbase + i * 64u + lane;
-    part[p] = make_float4(a[i][0], a[i][1], a[i][2], a[i][3]);
+    part[p] = make_float4(a[i][0], a[i][1], EP ? a[i][2] : 0.f, EP ? a[i][3] : 0.f);
   }
 }
 
@@ -1137,15 +1148,15 @@ __device__ __forceinline__ void f32r_pass(const KInst& I, const WorkItem& w, Til
 // reads the ray chunk again).  (The slot line with per-slot offsets in one
 // pass of 16 would need ~256 VGPRs.)
 template <int P, int W, bool TILE, bool UNIN, bool NOFRACT, bool LIN2, bool PAX, bool SEP,
-          bool LINE = false, bool DEV = false>
+          bool LINE = false, bool DEV = false, bool EP = true>
 __device__ __forceinline__ void f32r_item(const KInst& I, const WorkItem& w, TileRay* lds,
                                           uint32_t lane, uint32_t wave) {
   constexpr int PH = ((LINE && !DEV) || P <= 8) ? P : 8;
   static_assert(P % PH == 0, "passes must tile the lane's slots");
 #pragma unroll 1
   for (uint32_t first = 0; first < (uint32_t)P; first += PH)
-    f32r_pass<P, PH, W, TILE, UNIN, NOFRACT, LIN2, PAX, SEP, LINE, DEV>(I, w, lds, lane, wave,
-                                                                         first);
+    f32r_pass<P, PH, W, TILE, UNIN, NOFRACT, LIN2, PAX, SEP, LINE, DEV, EP>(I, w, lds, lane, wave,
+                                                                             first);
 }
 
 // Whether an instance runs the slot-line form at this P.
@@ -1172,24 +1183,32 @@ __device__ __forceinline__ bool linedev_variant(uint32_t f) {
 template <int P, int W, bool LINE>
 __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const WorkItem& w,
                                               TileRay* lds, uint32_t lane, uint32_t wave) {
+  // bit 4 of the slot-line variant index: Ep sums present
+  const uint32_t ep = (f & INST_F_EPZERO) ? 0u : 4u;
   if constexpr (LINE) {
-    const uint32_t v = ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+    const uint32_t v = ep | ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
     switch (v) {
-      case 0: f32r_item<P, W, true, true, false, false, true, true, true>(I, w, lds, lane, wave); break;
-      case 1: f32r_item<P, W, true, true, false, true, true, true, true>(I, w, lds, lane, wave); break;
-      case 2: f32r_item<P, W, true, true, true, false, true, true, true>(I, w, lds, lane, wave); break;
-      case 3: f32r_item<P, W, true, true, true, true, true, true, true>(I, w, lds, lane, wave); break;
+#define XRTK_L(V_)                                                                               \
+  case V_:                                                                                      \
+    f32r_item<P, W, true, true, (V_ & 2) != 0, (V_ & 1) != 0, true, true, true, false,           \
+              (V_ & 4) != 0>(I, w, lds, lane, wave);                                             \
+    break;
+      XRTK_L(0) XRTK_L(1) XRTK_L(2) XRTK_L(3) XRTK_L(4) XRTK_L(5) XRTK_L(6) XRTK_L(7)
+#undef XRTK_L
     }
     return;
   }
   if constexpr (P >= 4) {
     if (linedev_variant<P>(f)) {
-      const uint32_t v = ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
+      const uint32_t v = ep | ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u);
       switch (v) {
-        case 0: f32r_item<P, W, true, true, false, false, true, true, true, true>(I, w, lds, lane, wave); break;
-        case 1: f32r_item<P, W, true, true, false, true, true, true, true, true>(I, w, lds, lane, wave); break;
-        case 2: f32r_item<P, W, true, true, true, false, true, true, true, true>(I, w, lds, lane, wave); break;
-        case 3: f32r_item<P, W, true, true, true, true, true, true, true, true>(I, w, lds, lane, wave); break;
+#define XRTK_D(V_)                                                                               \
+  case V_:                                                                                      \
+    f32r_item<P, W, true, true, (V_ & 2) != 0, (V_ & 1) != 0, true, true, true, true,            \
+              (V_ & 4) != 0>(I, w, lds, lane, wave);                                             \
+    break;
+        XRTK_D(0) XRTK_D(1) XRTK_D(2) XRTK_D(3) XRTK_D(4) XRTK_D(5) XRTK_D(6) XRTK_D(7)
+#undef XRTK_D
       }
       return;
     }
@@ -1200,16 +1219,23 @@ __device__ __forceinline__ void f32r_dispatch(uint32_t f, const KInst& I, const 
   const uint32_t v = (tile ? 8u : 0u) | ((f & INST_F_UNIFORM_N) ? 4u : 0u) |
                      ((f & INST_F_NOFRACT) ? 2u : 0u) | ((f & INST_F_LIN2) ? 1u : 0u) |
                      (pax ? 16u : 0u) | (sep ? 32u : 0u);
-  switch (v) {
+  // The tile-referenced paraxial forms (per-ray normals: 24-27; separable
+  // amplitude: 60-63) also come without Ep sums (bit 64); the rest always
+  // form them.
+  const uint32_t vx = v | ((ep == 0u && (v & 24u) == 24u) ? 64u : 0u);
+  switch (vx) {
 #define XRTK_V(V_)                                                                      \
   case V_:                                                                             \
     f32r_item<P, W, (V_ & 8) != 0, (V_ & 4) != 0, (V_ & 2) != 0, (V_ & 1) != 0,         \
-              (V_ & 16) != 0, (V_ & 32) != 0>(I, w, lds, lane, wave);                  \
+              (V_ & 16) != 0, (V_ & 32) != 0, false, false, (V_ & 64) == 0>(I, w, lds,  \
+                                                                            lane, wave); \
     break;
     XRTK_V(0) XRTK_V(1) XRTK_V(2) XRTK_V(3) XRTK_V(4) XRTK_V(5) XRTK_V(6) XRTK_V(7)
     XRTK_V(8) XRTK_V(9) XRTK_V(10) XRTK_V(11) XRTK_V(12) XRTK_V(13) XRTK_V(14) XRTK_V(15)
     XRTK_V(24) XRTK_V(25) XRTK_V(26) XRTK_V(27) XRTK_V(28) XRTK_V(29) XRTK_V(30) XRTK_V(31)
     XRTK_V(60) XRTK_V(61) XRTK_V(62) XRTK_V(63)
+    XRTK_V(88) XRTK_V(89) XRTK_V(90) XRTK_V(91) XRTK_V(92) XRTK_V(93) XRTK_V(94) XRTK_V(95)
+    XRTK_V(124) XRTK_V(125) XRTK_V(126) XRTK_V(127)
 #undef XRTK_V
   }
 }

@@ -150,6 +150,8 @@ enum : uint32_t {
                           // (bound <= 1e-7 rev)
   INST_F_LINEDEV = 1024u, // LINE with each slot's own offset from the line applied
                           // to first order (pixel lists off a line by rounding)
+  INST_F_EPZERO = 2048u,  // F32R: every ray has Ep == 0, so the Ep sums are exact
+                          // zeros and the pair kernel skips them
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };
