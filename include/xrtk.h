@@ -276,6 +276,23 @@ typedef uint64_t xrtk_rayset;
 int xrtk_rayset_upload(xrtk_ctx *ctx, const XrtkRaysF64 *rays,
                        const double *path_up, size_t n_rays,
                        xrtk_rayset *out);
+/* beam_to_diffraction_rays (src/waves/prepare_wave.rs:78-111) on the
+ * device, straight into a resident ray set: the Beam's SoA columns (x, y, z,
+ * direction a, b, c, energy e, optional amplitudes es / ep: all four or
+ * NULL, then Es = 1, Ep = 0 as prepare_wave.rs:89-95), its normals (3 per
+ * beam ray) and the good indices go up once, and a gather kernel writes ray
+ * k from beam ray good[k] with nl = n . (a, b, c): the same records as
+ * xrtk_beam_to_rays followed by xrtk_rayset_upload.  path (may be NULL) is
+ * the beam's optical path per beam ray [mm], folded into the phase as
+ * xrtk_rayset_upload's path_up (the term the reference leaves to the
+ * caller). */
+int xrtk_rayset_from_beam(xrtk_ctx *ctx, const double *bx, const double *by,
+                          const double *bz, const double *ba, const double *bb,
+                          const double *bc, const double *be, const double *es_re,
+                          const double *es_im, const double *ep_re,
+                          const double *ep_im, const double *path, size_t n_beam,
+                          const double *normals3, const size_t *good,
+                          size_t n_good, xrtk_rayset *out);
 int xrtk_rayset_release(xrtk_ctx *ctx, xrtk_rayset rs);
 /* Bytes of HBM the ray set holds. */
 int xrtk_rayset_bytes(xrtk_ctx *ctx, xrtk_rayset rs, size_t *bytes);

@@ -245,3 +245,61 @@ def test_no_graph_switch_gives_the_same_bits(monkeypatch):
         for rs in sets:
             rs.release()
         c.close()
+
+
+# ---- beam_to_diffraction_rays on the device ---------------------------------------------
+def _beam(n, seed, amps=True):
+    """A Beam's SoA columns after a curved optic: positions on a cap,
+    directions fanning out from a point 10 m upstream, energies at 10 keV,
+    per-ray normals, complex amplitudes (or none)."""
+    rng = np.random.default_rng(seed)
+    x, z = rng.uniform(-0.1, 0.1, n), rng.uniform(-0.1, 0.1, n)
+    y = (x * x + z * z) / 4000.0
+    d = np.sqrt(x * x + (y + 1e4) ** 2 + z * z)
+    beam = {"x": x, "y": y, "z": z, "a": x / d, "b": (y + 1e4) / d, "c": z / d,
+            "e": np.full(n, 10000.0)}
+    if amps:
+        beam["es"] = np.exp(1j * rng.uniform(0, 6.28, n)) * rng.uniform(0.5, 1.0, n)
+        beam["ep"] = 0.3 * np.exp(1j * rng.uniform(0, 6.28, n))
+    t = rng.uniform(-0.05, 0.05, (n, 2))
+    nrm = np.stack([t[:, 0], np.ones(n), t[:, 1]], axis=1)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    return beam, nrm, d - 1e4
+
+
+@pytest.mark.parametrize("amps,with_path", [(True, True), (False, False), (True, False)])
+def test_rayset_from_beam_equals_host_gather(ctx, amps, with_path):
+    """The device gather (xrtk_rayset_from_beam) writes the records the host
+    gather (beam_to_diffraction_rays, prepare_wave.rs:78-111) and upload
+    write: the same bits through the FP64 and the FP32-reduced solves, with
+    the good indices in any order; with a path column its phase is folded in
+    as the upload's path_up would, matching the oracle."""
+    n = 20000
+    beam, nrm, path = _beam(n, 7, amps)
+    good = np.random.default_rng(8).permutation(n)[: n - 1234]
+    pth = path if with_path else None
+    dev = xrtk.RaySet.from_beam(ctx, beam, nrm, good, path=pth)
+    rays = xrtk.beam_to_diffraction_rays(beam, nrm, good)
+    host = xrtk.RaySet(ctx, rays, path_up=path[good] if with_path else None)
+    g = xrtk.WaveParams((0.0, 1000.0, 0.0), 0.02, 0.02, 48, 48)
+    try:
+        assert dev.n == host.n == good.shape[0]
+        for mode in ("f64", "f32r"):
+            (a,) = xrtk.solve_jobs(ctx, [xrtk.Job(dev, 10000.0, g)], mode=mode)
+            (b,) = xrtk.solve_jobs(ctx, [xrtk.Job(host, 10000.0, g)], mode=mode)
+            assert np.array_equal(a, b), mode
+        px, py, pz = xrtk.prepare_wave(g)
+        ref = O.diffraction_f64(rays, px, py, pz, path_up=path[good] if with_path else None,
+                                stride=37)
+        (a,) = xrtk.solve_jobs(ctx, [xrtk.Job(dev, 10000.0, g)], mode="f64")
+        assert rel_l2(a[::37], ref) < 1e-9
+    finally:
+        dev.release()
+        host.release()
+
+
+def test_rayset_from_beam_rejects_bad_index(ctx):
+    beam, nrm, _ = _beam(100, 9)
+    with pytest.raises(xrtk.XrtkError) as ei:
+        xrtk.RaySet.from_beam(ctx, beam, nrm, [0, 5, 100])
+    assert ei.value.code == _lib.EINVAL

@@ -680,3 +680,35 @@ def test_batch_equals_standalone_bitwise(ctx):
     finally:
         for rs in sets:
             rs.release()
+
+
+def test_batch_config3_full_size(ctx):
+    """The 64 uneven instances at their stated sizes (16k-2M rays, 64^2-1024^2
+    screens, 6-25 keV, 200-20,000 mm) in one persistent launch: every
+    instance against the long-double oracle on a strided pixel sample (the
+    plain FP64 sum itself rounds phases of up to ~1e12 rad here), and its
+    first instances bit for bit against their standalone solves."""
+    specs = W.batch_instances()
+    assert len(specs) == 64
+    sets = [xrtk.RaySet(ctx, W.instance_rays(s)) for s in specs]
+    try:
+        jobs = [xrtk.Job(rs, s["energy"], grid(W.instance_screen(s))) for rs, s in zip(sets, specs)]
+        outs = xrtk.solve_jobs(ctx, jobs)
+        plan = ctx.last_plan()
+        assert plan["n_fp64"] == 0, plan
+        errs = []
+        for i, s in enumerate(specs):
+            px, py, pz = xrtk.prepare_wave(jobs[i].screen)
+            stride = max(1, px.shape[0] // 12)
+            ld = O.diffraction_ld(W.instance_rays(s), px, py, pz, stride=stride)
+            errs.append(rel_l2(outs[i][::stride], ld))
+        worst = int(np.argmax(errs))
+        print(f"config 3: {len(specs)} instances, max rel-L2 vs long double {max(errs):.2e} "
+              f"(instance {worst}: {specs[worst]}), median {np.median(errs):.2e}")
+        assert max(errs) < STRESS_BAR
+        for j in range(3):
+            (alone,) = xrtk.solve_jobs(ctx, [jobs[j]])
+            assert np.array_equal(alone, outs[j])
+    finally:
+        for rs in sets:
+            rs.release()

@@ -274,3 +274,50 @@ def test_non_square_screens_equal_batched(ctx, sets):
         assert np.array_equal(g, b), k
         # (the FP64 kernel's own phase rounding reaches ~5e-5 at 20 m)
         assert W.rel_l2(b, d) < 1e-4, k
+
+
+def test_poisson_energy_scan_stream(ctx, sets):
+    """A served stream in the form of the energy-scan workload, shortened:
+    Poisson arrivals (100 req/s for 3 s, seeded) of energy scans (1-50
+    energies, 5 eV apart) over the resident ray sets, admitted under a
+    budget and batched continuously.  Every result equals the batched solve
+    of the same (ray set, energy, screen) to FP32 rounding (the 10 ms
+    quantum cuts the ray chunks), requests overlap on the device, and the
+    timestamps are ordered submit <= admit <= first launch <= done."""
+    import time
+    specs, rsets = sets
+    rng = np.random.default_rng(17)
+    reqs = []
+    while len(reqs) < 300:
+        i = int(rng.integers(len(specs)))
+        m = int(rng.integers(1, 51))
+        reqs += [(i, specs[i]["energy"] + 5.0 * (k - m // 2)) for k in range(m)]
+    arrivals = np.cumsum(rng.exponential(1.0 / 100.0, len(reqs)))
+    n = int(np.searchsorted(arrivals, 3.0))
+    reqs, arrivals = reqs[:n], arrivals[:n]
+    screens = [grid(W.instance_screen(s)) for s in specs]
+    srv = xrtk.Server(ctx, hbm_budget_bytes=4 << 30, max_active=64, quantum_ms=10.0)
+    try:
+        t0 = time.perf_counter()
+        tickets = []
+        for (i, e), ta in zip(reqs, arrivals):
+            d = t0 + ta - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(xrtk.Job(rsets[i], e, screens[i])))
+        got = [srv.wait(t, timeout_ms=120000) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = np.array([ts[3] - ts[0] for _, ts in got]) * 1e3
+    print(f"{n} requests in {arrivals[-1]:.2f} s: p50 {np.percentile(lat, 50):.1f} ms, "
+          f"p99 {np.percentile(lat, 99):.1f} ms, max concurrent {st['max_concurrent']}")
+    assert st["completed"] == n and st["max_concurrent"] > 1
+    for (out, ts), (i, e) in zip(got, reqs):
+        assert ts[0] <= ts[1] <= ts[2] <= ts[3]
+    # every distinct (set, energy) against the batched solve
+    seen = {}
+    for (out, _), (i, e) in zip(got, reqs):
+        if (i, e) not in seen:
+            (seen[(i, e)],) = xrtk.solve_jobs(ctx, [xrtk.Job(rsets[i], e, screens[i])])
+        assert W.rel_l2(out, seen[(i, e)]) < 2e-6

@@ -712,6 +712,80 @@ int xrtk_rayset_upload(xrtk_ctx* ctx, const XrtkRaysF64* rays, const double* pat
   }
 }
 
+int xrtk_rayset_from_beam(xrtk_ctx* ctx, const double* bx, const double* by, const double* bz,
+                          const double* ba, const double* bb, const double* bc, const double* be,
+                          const double* es_re, const double* es_im, const double* ep_re,
+                          const double* ep_im, const double* path, size_t n_beam,
+                          const double* normals3, const size_t* good, size_t n_good,
+                          xrtk_rayset* out) {
+  static_assert(sizeof(size_t) == sizeof(uint64_t), "good indices go up as uint64");
+  if (!ctx) return XRTK_ENODEV;
+  if (!out) return XRTK_EINVAL;
+  if (n_good && (!bx || !by || !bz || !ba || !bb || !bc || !be || !normals3 || !good))
+    return XRTK_EINVAL;
+  if (n_beam > 0xffffffffull || n_good > 0xffffffffull) return XRTK_EINVAL;
+  for (size_t k = 0; k < n_good; ++k)
+    if (good[k] >= n_beam) return XRTK_EINVAL;
+  const bool amps = es_re && es_im && ep_re && ep_im;
+  try {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    RaySet rs;
+    rs.nr = (uint32_t)n_good;
+    rs.has_path = path != nullptr;
+    if (n_good) {
+      rs.energy = be[good[0]];
+      for (size_t k = 0; k < n_good; ++k) {
+        const size_t i = good[k];
+        if (be[i] != rs.energy) rs.mono = false;
+        rs.box.add(bx[i], by[i], bz[i]);
+      }
+    }
+    // the beam's columns, its normals and the good indices go up once; the
+    // gather runs on the device
+    std::vector<const double*> cols = {bx, by, bz, ba, bb, bc, be};
+    if (amps) cols.insert(cols.end(), {es_re, es_im, ep_re, ep_im});
+    if (path) cols.push_back(path);
+    const size_t nb = std::max<size_t>(n_beam, 1), cb = sizeof(double) * nb;
+    const size_t off_n = align256(cols.size() * cb), off_g = off_n + align256(3 * cb);
+    if (ctx->ws.stage.ensure(off_g + sizeof(uint64_t) * std::max<size_t>(n_good, 1) + 256) !=
+            hipSuccess ||
+        rs.rays.ensure(sizeof(RayRec64) * std::max<size_t>(n_good, 1)) != hipSuccess) {
+      rs.rays.release();
+      return XRTK_ENOMEM;
+    }
+    char* S = ctx->ws.stage.as<char>();
+    hipError_t e = hipSuccess;
+    if (n_good) {
+      for (size_t c = 0; c < cols.size() && e == hipSuccess; ++c)
+        e = hipMemcpyAsync(S + c * cb, cols[c], sizeof(double) * n_beam, hipMemcpyHostToDevice,
+                           ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(S + off_n, normals3, 3 * sizeof(double) * n_beam,
+                           hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(S + off_g, good, sizeof(uint64_t) * n_good, hipMemcpyHostToDevice,
+                           ctx->stream);
+      if (e == hipSuccess)
+        e = launch_rays_from_beam(reinterpret_cast<const double*>(S), (uint32_t)n_beam,
+                                  reinterpret_cast<const double*>(S + off_n),
+                                  reinterpret_cast<const uint64_t*>(S + off_g), (uint32_t)n_good,
+                                  amps, path != nullptr, rs.rays.as<RayRec64>(), ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // the caller's buffers are free after
+    if (e != hipSuccess) {
+      rs.rays.release();
+      return map_err(e);
+    }
+    const uint64_t h = ctx->next_rs++;
+    ctx->raysets.emplace(h, rs);
+    *out = h;
+    return XRTK_OK;
+  } catch (...) {
+    return XRTK_ENOMEM;
+  }
+}
+
 int xrtk_rayset_release(xrtk_ctx* ctx, xrtk_rayset rs) {
   if (!ctx) return XRTK_ENODEV;
   std::lock_guard<std::mutex> lk(ctx->mu);

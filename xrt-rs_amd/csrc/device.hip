@@ -207,6 +207,38 @@ __global__ void k_rays_from_cols(const double* __restrict__ cols,
   dst[i] = r;
 }
 
+// beam_to_diffraction_rays (src/waves/prepare_wave.rs:78-111) as a gather
+// on the device: ray k is beam ray good[k], its normal normals[3 good[k]],
+// nl = n . (a, b, c) in the host helper's operation order (no contraction,
+// so the records hold the host gather's bits), Es / Ep from the beam's
+// amplitude columns or (1, 0) without them (prepare_wave.rs:89-95), and the
+// upstream path of the beam ray when a path column is given.
+// cols: x y z a b c e [es_re es_im ep_re ep_im] [path], n_beam doubles each.
+__global__ void k_rays_from_beam(const double* __restrict__ cols, uint32_t n_beam,
+                                 const double* __restrict__ normals,
+                                 const uint64_t* __restrict__ good, uint32_t n_good,
+                                 uint32_t has_amps, uint32_t has_path,
+                                 RayRec64* __restrict__ dst) {
+#pragma clang fp contract(off)
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_good) return;
+  const size_t N = n_beam;
+  const size_t i = good[k];  // checked against n_beam on the host
+  const double* n = normals + 3 * i;
+  RayRec64 r;
+  r.x = cols[0 * N + i]; r.y = cols[1 * N + i]; r.z = cols[2 * N + i];
+  r.nx = n[0]; r.ny = n[1]; r.nz = n[2];
+  r.nl = n[0] * cols[3 * N + i] + n[1] * cols[4 * N + i] + n[2] * cols[5 * N + i];
+  r.energy = cols[6 * N + i];
+  r.es_re = has_amps ? cols[7 * N + i] : 1.0;
+  r.es_im = has_amps ? cols[8 * N + i] : 0.0;
+  r.ep_re = has_amps ? cols[9 * N + i] : 0.0;
+  r.ep_im = has_amps ? cols[10 * N + i] : 0.0;
+  r.path_up = has_path ? cols[(has_amps ? 11 : 7) * N + i] : 0.0;
+  r.pad0 = r.pad1 = r.pad2 = 0.0;
+  dst[k] = r;
+}
+
 // Explicit pixel lists: within each tile of 64 P slots, lane l's P slots
 // (slot t*64P + i*64 + l) take the list entries t*64P + l*P + i, so that a
 // lane holds P consecutive entries (neighbours along a row of a grid-ordered
@@ -1465,6 +1497,14 @@ hipError_t launch_rays_from_f32aos(const float* src, RayRec64* dst, uint32_t n, 
 hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, hipStream_t st) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_rays_from_cols, dim3(cdiv(n, 256)), dim3(256), 0, st, cols, dst, n);
+  return hipGetLastError();
+}
+hipError_t launch_rays_from_beam(const double* cols, uint32_t n_beam, const double* normals,
+                                 const uint64_t* good, uint32_t n_good, bool has_amps,
+                                 bool has_path, RayRec64* dst, hipStream_t st) {
+  if (!n_good) return hipSuccess;
+  hipLaunchKernelGGL(k_rays_from_beam, dim3(cdiv(n_good, 256)), dim3(256), 0, st, cols, n_beam,
+                     normals, good, n_good, has_amps ? 1u : 0u, has_path ? 1u : 0u, dst);
   return hipGetLastError();
 }
 hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t* perm, uint32_t n,

@@ -21,6 +21,11 @@ namespace xrtk {
 // ---- device launchers (device.hip) ----
 hipError_t launch_rays_from_f32aos(const float* src, RayRec64* dst, uint32_t n, hipStream_t st);
 hipError_t launch_rays_from_cols(const double* cols, RayRec64* dst, uint32_t n, hipStream_t st);
+// cols: x y z a b c e [es_re es_im ep_re ep_im] [path] of n_beam beam rays;
+// writes the n_good rays good[0..n_good) (beam_to_diffraction_rays).
+hipError_t launch_rays_from_beam(const double* cols, uint32_t n_beam, const double* normals,
+                                 const uint64_t* good, uint32_t n_good, bool has_amps,
+                                 bool has_path, RayRec64* dst, hipStream_t st);
 // Explicit pixel lists: slots in tiles of 64 P, each lane holding P
 // consecutive list entries; perm[slot] = entry or ~0u (padding).
 hipError_t launch_pix_from_f32aos(const void* src, PixF64* dst, uint32_t* perm, uint32_t n,

@@ -354,6 +354,42 @@ class RaySet:
         self.handle = h.value
         self.n = n
 
+    @classmethod
+    def from_beam(cls, ctx: GpuContext, beam: dict, normals, good, path=None) -> "RaySet":
+        """beam_to_diffraction_rays (src/waves/prepare_wave.rs:78-111) on the
+        device into a resident ray set: beam holds x, y, z, a, b, c, e and
+        optionally es / ep (complex); normals is (n, 3); path (optional) the
+        beam's optical path per beam ray, folded into the phase."""
+        cols = {k: np.ascontiguousarray(np.asarray(beam[k], dtype=np.float64)) for k in
+                ("x", "y", "z", "a", "b", "c", "e")}
+        n_beam = cols["x"].shape[0]
+        normals = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(n_beam, 3))
+        good = np.ascontiguousarray(np.asarray(good, dtype=np.uint64).reshape(-1))
+        nul = C.POINTER(C.c_double)()
+        amps = [nul] * 4
+        keep = []
+        if beam.get("es") is not None and beam.get("ep") is not None:
+            es = np.asarray(beam["es"], dtype=np.complex128)
+            ep = np.asarray(beam["ep"], dtype=np.complex128)
+            keep = [np.ascontiguousarray(a) for a in (es.real, es.imag, ep.real, ep.imag)]
+            amps = [dptr(a) for a in keep]
+        pth = None
+        if path is not None:
+            pth = np.ascontiguousarray(np.asarray(path, dtype=np.float64).reshape(-1))
+            if pth.shape[0] != n_beam:
+                raise ValueError("path length mismatch")
+        h = C.c_uint64()
+        check(lib.xrtk_rayset_from_beam(ctx.handle, *[dptr(cols[k]) for k in
+                                                     ("x", "y", "z", "a", "b", "c", "e")],
+                                        *amps, dptr(pth) if pth is not None else nul, n_beam,
+                                        dptr(normals), good.ctypes.data_as(C.POINTER(C.c_size_t)),
+                                        good.shape[0], C.byref(h)), "xrtk_rayset_from_beam")
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.handle = h.value
+        self.n = int(good.shape[0])
+        return self
+
     def nbytes(self) -> int:
         b = C.c_size_t()
         check(lib.xrtk_rayset_bytes(self.ctx.handle, self.handle, C.byref(b)), "xrtk_rayset_bytes")

@@ -138,6 +138,8 @@ PROTOTYPES = {
     "xrtk_prepare_wave": (_i, [C.POINTER(XrtkWaveParams), _dp, _dp, _dp]),
     "xrtk_beam_to_rays": (_i, [_dp] * 11 + [_sz, _dp, C.POINTER(_sz), _sz, C.POINTER(_dp)]),
     "xrtk_rayset_upload": (_i, [_p, C.POINTER(XrtkRaysF64), _dp, _sz, C.POINTER(C.c_uint64)]),
+    "xrtk_rayset_from_beam": (_i, [_p] + [_dp] * 12 + [_sz, _dp, C.POINTER(_sz), _sz,
+                                                   C.POINTER(C.c_uint64)]),
     "xrtk_rayset_release": (_i, [_p, C.c_uint64]),
     "xrtk_rayset_bytes": (_i, [_p, C.c_uint64, C.POINTER(_sz)]),
     "xrtk_solve_jobs": (_i, [_p, C.POINTER(XrtkJob), _sz, C.POINTER(XrtkOpts)]),
