@@ -6,21 +6,25 @@
 // The kernel factorises exactly, exp(i K (dx^2 + dz^2)) = exp(i K dx^2)
 // exp(i K dz^2), and the screen is a grid, so
 //
-//   U = X Z'^T,  X[ix][j] = exp(i K (x'_ix - x_j)^2),
-//                Z'[iz][j] = Es_j exp(i K (z'_iz - z_j)^2):
+//   U[iz][ix] = sum_j Z_j[iz] X_j[ix],  Z_j[iz] = Es_j exp(i K (z'_iz - z_j)^2),
+//                                       X_j[ix] = exp(i K (x'_ix - x_j)^2):
 //
-// a complex matrix product over the rays.  The two factor matrices
-// ((nx + nz) n_rays entries, phases reduced and evaluated in FP64) are built
-// by k_fresnel_factor*; the product, nx nz n_rays complex multiply-adds, runs
-// on the matrix cores through rocBLAS, chunk by chunk over the rays:
-//   FP64  zgemm on the complex factors;
-//   FP32  one real sgemm on the block forms
-//           [Xr -Xi; Xi Xr] (2 nx x 2 nc) times [Zr Zi]^T (2 nc x nz)
-//         = [Ur; Ui] (2 nx x nz), the same 8 flops per pair, which gfx950's
-//         FP32 matrix path runs at about twice the rate of cgemm;
-//         k_accum_blk adds each chunk product into the FP64 result.
-// Output index iz * nx + ix: the column-major nx x nz result.
-#include <dlfcn.h>
+// a complex matrix product over the rays, which runs on the matrix cores.
+// The factors are never stored in HBM: each workgroup owns a tile of the
+// screen and one slice of the rays, and generates the factors of its tile's
+// rows and columns block by block into LDS (each phase K d^2 evaluated in
+// FP64 and reduced to a fractional revolution first), then multiplies them
+// there:
+//   FP32 (XRTK_MODE_F32R): v_mfma_f32_32x32x2f32, 128 x 128 pixels per
+//        workgroup of 4 waves (64 x 64 each), sin/cos of the reduced phase
+//        by v_sin/v_cos; each workgroup's FP32 sums cover at most 16,384
+//        rays (a ray slice) and go to HBM as partial sums;
+//   FP64 (XRTK_MODE_F64): v_mfma_f64_16x16x4f64, 64 x 64 pixels per
+//        workgroup (32 x 32 per wave), FP64 sincospi of the reduced phase.
+// The complex product takes four real products per pair of factors:
+//   Re U += Zr Xr + Zi (-Xi),  Im U += Zr Xi + Zi Xr   (8 flops per pair).
+// k_fresnel_merge* sums the ray slices' partials in FP64, in slice order.
+// Output index iz * nx + ix (fresnel.rs:78-99).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,156 +34,325 @@
 
 #include "runtime.h"
 
+#if defined(__HIP_DEVICE_COMPILE__)
+#define XRTK_FCONST __attribute__((address_space(4)))
+#else
+#define XRTK_FCONST
+#endif
+
 namespace xrtk {
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-// out[i + j * n] = (amp_j or 1) * exp(i 2 pi krev (p_i - r_j)^2), i < n, j < nc.
+constexpr uint32_t kT32 = 128;     // FP32: tile side in pixels
+constexpr uint32_t kKB32 = 16;     // FP32: rays per LDS block
+constexpr uint32_t kSlice32 = 16384;  // FP32: most rays in one FP32 sum
+constexpr uint32_t kT64 = 64;      // FP64: tile side
+constexpr uint32_t kKB64 = 8;      // FP64: rays per LDS block
+constexpr uint32_t kTab = 256;     // FP64: entries of the e^{2 pi i j / 256} table
+
+// Factor planes of one block of rays: Z (rows) and X (columns), real and
+// imaginary parts, [ray][pixel], rows LD elements apart.
+template <class T, uint32_t KB, uint32_t N, uint32_t LD = N>
+struct Planes {
+  T zr[KB][LD], zi[KB][LD], xr[KB][LD], xi[KB][LD];
+};
+
+
+// ---- FP32 -------------------------------------------------------------------------------
+// Generates the factors of rays kb .. kb + KB - 1 for column / row g (thread
+// t: g = t % 128, rays (t / 128) * 8 .. + 7 of the block): the phase
+// K d^2 / (2 pi) in FP64, its fractional revolution, v_sin / v_cos.
+__device__ __forceinline__ void gen32(Planes<float, kKB32, kT32>& P, uint32_t kb, uint32_t g,
+                                      uint32_t h, double xg, double zg, double krev,
+                                      const XRTK_FCONST double* crx,
+                                      const XRTK_FCONST double* crz,
+                                      const XRTK_FCONST double2* ces) {
+#pragma unroll
+  for (uint32_t i = 0; i < kKB32 / 2; ++i) {
+    const uint32_t kl = h * (kKB32 / 2) + i, k = kb + kl;
+    const double xk = crx[k], zk = crz[k];
+    const double2 e = ces[k];
+    float fx, fz;
+    {
+#pragma clang fp contract(off)
+      // revolutions, reduced to [-1/2, 1/2] (FP32 resolves the phase twice
+      // as finely there as on [0, 1))
+      const double dx = xg - xk, dz = zg - zk;
+      const double px = krev * (dx * dx), pz = krev * (dz * dz);
+      fx = (float)(px - rint(px));
+      fz = (float)(pz - rint(pz));
+    }
+    P.xr[kl][g] = __builtin_amdgcn_cosf(fx);
+    P.xi[kl][g] = __builtin_amdgcn_sinf(fx);
+    const float c = __builtin_amdgcn_cosf(fz), sn = __builtin_amdgcn_sinf(fz);
+    const float er = (float)e.x, ei = (float)e.y;
+    P.zr[kl][g] = er * c - ei * sn;
+    P.zi[kl][g] = er * sn + ei * c;
+  }
+}
+
+// The products of one block: each wave's 2 x 2 blocks of 32 x 32, four real
+// products per pair of complex factors (v_mfma_f32_32x32x2f32: lane l
+// supplies A[l % 32][l / 32] and B[l / 32][l % 32]).
+// A wave's 2 x NB blocks of 32 x 32 (z rows zb.., x columns xb..).
+template <int NB>
+__device__ __forceinline__ void mma32(const Planes<float, kKB32, kT32>& P, uint32_t lane,
+                                      uint32_t zb, uint32_t xb, floatx16 (&ur)[2][NB],
+                                      floatx16 (&ui)[2][NB]) {
+#pragma unroll
+  for (uint32_t kk = 0; kk < kKB32 / 2; ++kk) {
+    const uint32_t kr = 2u * kk + (lane >> 5), c = lane & 31u;
+    float ar[2], ai[2], br[NB], bi[NB], nbi[NB];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      ar[a] = P.zr[kr][zb + 32u * a + c];
+      ai[a] = P.zi[kr][zb + 32u * a + c];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      br[b] = P.xr[kr][xb + 32u * b + c];
+      bi[b] = P.xi[kr][xb + 32u * b + c];
+      nbi[b] = -bi[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        ur[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[a], br[b], ur[a][b], 0, 0, 0);
+        ui[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[a], bi[b], ui[a][b], 0, 0, 0);
+      }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        ur[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[a], nbi[b], ur[a][b], 0, 0, 0);
+        ui[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[a], br[b], ui[a][b], 0, 0, 0);
+      }
+  }
+}
+
+// FP32 kernel.  grid = (tiles, ray slices); block = 256 (4 waves), two
+// workgroups per CU.  Wave w owns z rows (w & 1) * 64 .. + 63 and x columns
+// (w >> 1) * 64 .. + 63 of the tile (D element r of lane l is row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of a 32 x 32 block).  Two
+// LDS buffers: the waves generate the next block of rays' factors into one
+// while they multiply the block in the other (one barrier per block).  A
+// slice is a whole number of block pairs; rays are padded with zero
+// amplitudes to whole slices plus one block (the block generated after a
+// slice's last is never used).
+__global__ __launch_bounds__(256, 2) void k_fresnel_mfma32(
+    const double* __restrict__ xp, uint32_t nx, const double* __restrict__ zp, uint32_t nz,
+    const double* __restrict__ rx, const double* __restrict__ rz, const double2* __restrict__ es,
+    uint32_t slice, double krev, uint32_t n_tx, float2* __restrict__ partial) {
+  __shared__ Planes<float, kKB32, kT32> P0, P1;
+  const uint32_t t = threadIdx.x, lane = t & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const uint32_t tx = blockIdx.x % n_tx, tz = blockIdx.x / n_tx;
+  const uint32_t x0 = tx * kT32, z0 = tz * kT32;
+  // factor generation: column / row g of the tile, half h of each ray block
+  const uint32_t g = t & 127u, h = __builtin_amdgcn_readfirstlane(t >> 7);
+  const double xg = xp[min(x0 + g, nx - 1u)];
+  const double zg = zp[min(z0 + g, nz - 1u)];
+  const uint32_t zb = (wave & 1u) * 64u, xb = (wave >> 1) * 64u;
+  floatx16 ur[2][2], ui[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) ur[a][b][r] = ui[a][b][r] = 0.f;
+  const XRTK_FCONST double* crx = (const XRTK_FCONST double*)rx;
+  const XRTK_FCONST double* crz = (const XRTK_FCONST double*)rz;
+  const XRTK_FCONST double2* ces = (const XRTK_FCONST double2*)es;
+  const uint32_t k0 = blockIdx.y * slice;
+  gen32(P0, k0, g, h, xg, zg, krev, crx, crz, ces);
+  __syncthreads();
+  for (uint32_t kb = k0; kb < k0 + slice; kb += 2u * kKB32) {
+    gen32(P1, kb + kKB32, g, h, xg, zg, krev, crx, crz, ces);
+    mma32<2>(P0, lane, zb, xb, ur, ui);
+    __syncthreads();
+    gen32(P0, kb + 2u * kKB32, g, h, xg, zg, krev, crx, crz, ces);
+    mma32<2>(P1, lane, zb, xb, ur, ui);
+    __syncthreads();
+  }
+  float2* Pp = partial + (size_t)blockIdx.y * nz * nx;
+  const uint32_t xl = x0 + xb + (lane & 31u);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const uint32_t x = xl + 32u * b;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t z = z0 + zb + 32u * a + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
+        if (z < nz && x < nx) Pp[(size_t)z * nx + x] = make_float2(ur[a][b][r], ui[a][b][r]);
+      }
+    }
+}
+
+// ---- FP64 -------------------------------------------------------------------------------
+// e^{2 pi i f} for f in [0, 1), to FP64 rounding: the nearest table entry
+// e^{2 pi i j / 256} (tab, in LDS, from sincospi) times e^{i x},
+// x = 2 pi (f - j / 256), |x| <= pi / 256, by its Taylor series through x^6
+// (cos) and x^7 (sin): the first omitted terms are below 6e-21.
+__device__ __forceinline__ void cis_rev_tab(double f, const double2* tab, double* c, double* s) {
+  const double jd = rint(f * (double)kTab);
+  const double x = fma(-jd, 1.0 / kTab, f) * (2.0 * 3.14159265358979323846);
+  const double x2 = x * x;
+  const double cx = fma(x2, fma(x2, fma(x2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  const double sx = x * fma(x2, fma(x2, fma(x2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
+  const double2 t = tab[(uint32_t)jd & (kTab - 1u)];
+  *c = fma(t.x, cx, -(t.y * sx));
+  *s = fma(t.y, cx, t.x * sx);
+}
+
+// thread t: column / row g = t % 64, rays (t / 64) * 2 .. + 1 of the block
+__device__ __forceinline__ void gen64(Planes<double, kKB64, kT64>& P, const double2* tab,
+                                      uint32_t kb, uint32_t g, uint32_t q, double xg, double zg,
+                                      double krev, const XRTK_FCONST double* crx,
+                                      const XRTK_FCONST double* crz,
+                                      const XRTK_FCONST double2* ces) {
+#pragma unroll
+  for (uint32_t i = 0; i < kKB64 / 4; ++i) {
+    const uint32_t kl = q * (kKB64 / 4) + i, k = kb + kl;
+    const double xk = crx[k], zk = crz[k];
+    const double2 e = ces[k];
+    double fx, fz;
+    {
+#pragma clang fp contract(off)
+      const double dx = xg - xk, dz = zg - zk;
+      fx = __builtin_amdgcn_fract(krev * (dx * dx));  // revolutions, [0, 1)
+      fz = __builtin_amdgcn_fract(krev * (dz * dz));
+    }
+    double cx, sx, cz, sz;
+    cis_rev_tab(fx, tab, &cx, &sx);
+    cis_rev_tab(fz, tab, &cz, &sz);
+    P.xr[kl][g] = cx;
+    P.xi[kl][g] = sx;
+    P.zr[kl][g] = e.x * cz - e.y * sz;
+    P.zi[kl][g] = e.x * sz + e.y * cz;
+  }
+}
+
+// v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; D element r of lane l is row 4 r + l / 16, column l % 16.
+__device__ __forceinline__ void mma64(const Planes<double, kKB64, kT64>& P, uint32_t lane,
+                                      uint32_t zb, uint32_t xb, doublex4 (&ur)[2][2],
+                                      doublex4 (&ui)[2][2]) {
+#pragma unroll
+  for (uint32_t kk = 0; kk < kKB64 / 4; ++kk) {
+    const uint32_t kr = 4u * kk + (lane >> 4), c = lane & 15u;
+    double ar[2], ai[2], br[2], bi[2], nbi[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      ar[b] = P.zr[kr][zb + 16u * b + c];
+      ai[b] = P.zi[kr][zb + 16u * b + c];
+      br[b] = P.xr[kr][xb + 16u * b + c];
+      bi[b] = P.xi[kr][xb + 16u * b + c];
+      nbi[b] = -bi[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        ur[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], ur[a][b], 0, 0, 0);
+        ui[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ui[a][b], 0, 0, 0);
+      }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        ur[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], nbi[b], ur[a][b], 0, 0, 0);
+        ui[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ui[a][b], 0, 0, 0);
+      }
+  }
+}
+
+// FP64 kernel.  grid = (tiles, ray slices); block = 256.  Wave w owns z rows
+// (w & 1) * 32 .. + 31 and x columns (w >> 1) * 32 .. + 31 of the 64 x 64
+// tile, as 2 x 2 blocks of 16 x 16; double-buffered like the FP32 kernel.
+__global__ __launch_bounds__(256, 2) void k_fresnel_mfma64(
+    const double* __restrict__ xp, uint32_t nx, const double* __restrict__ zp, uint32_t nz,
+    const double* __restrict__ rx, const double* __restrict__ rz, const double2* __restrict__ es,
+    uint32_t slice, double krev, uint32_t n_tx, double2* __restrict__ partial) {
+  __shared__ Planes<double, kKB64, kT64> P0, P1;
+  __shared__ double2 tab[kTab];
+  const uint32_t t = threadIdx.x, lane = t & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  {
+    double sj, cj;
+    sincospi((double)t / (kTab / 2), &sj, &cj);  // e^{2 pi i t / 256}
+    tab[t] = make_double2(cj, sj);
+  }
+  const uint32_t tx = blockIdx.x % n_tx, tz = blockIdx.x / n_tx;
+  const uint32_t x0 = tx * kT64, z0 = tz * kT64;
+  const uint32_t g = t & 63u, q = wave;
+  const double xg = xp[min(x0 + g, nx - 1u)];
+  const double zg = zp[min(z0 + g, nz - 1u)];
+  const uint32_t zb = (wave & 1u) * 32u, xb = (wave >> 1) * 32u;
+  doublex4 ur[2][2], ui[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ur[a][b][r] = ui[a][b][r] = 0.0;
+  const XRTK_FCONST double* crx = (const XRTK_FCONST double*)rx;
+  const XRTK_FCONST double* crz = (const XRTK_FCONST double*)rz;
+  const XRTK_FCONST double2* ces = (const XRTK_FCONST double2*)es;
+  const uint32_t k0 = blockIdx.y * slice;
+  __syncthreads();  // the table
+  gen64(P0, tab, k0, g, q, xg, zg, krev, crx, crz, ces);
+  __syncthreads();
+  for (uint32_t kb = k0; kb < k0 + slice; kb += 2u * kKB64) {
+    gen64(P1, tab, kb + kKB64, g, q, xg, zg, krev, crx, crz, ces);
+    mma64(P0, lane, zb, xb, ur, ui);
+    __syncthreads();
+    gen64(P0, tab, kb + 2u * kKB64, g, q, xg, zg, krev, crx, crz, ces);
+    mma64(P1, lane, zb, xb, ur, ui);
+    __syncthreads();
+  }
+  double2* Pp = partial + (size_t)blockIdx.y * nz * nx;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const uint32_t x = x0 + xb + 16u * b + (lane & 15u);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t z = z0 + zb + 16u * a + 4u * r + (lane >> 4);
+        if (z < nz && x < nx) Pp[(size_t)z * nx + x] = make_double2(ur[a][b][r], ui[a][b][r]);
+      }
+    }
+}
+
+// U[p] = sum over the ray slices of partial[s][p], in FP64 and slice order.
 template <class T>
-__global__ __launch_bounds__(256) void k_fresnel_factor(const double* __restrict__ p, uint32_t n,
-                                                        const double* __restrict__ r,
-                                                        const double2* __restrict__ amp,
-                                                        uint32_t nc, double krev,
-                                                        T* __restrict__ out) {
-  const size_t total = (size_t)n * nc;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t i = (uint32_t)(e % n), j = (uint32_t)(e / n);
-    double s, c;
-    {
-#pragma clang fp contract(off)
-      const double d = p[i] - r[j];
-      const double ph = krev * (d * d);  // revolutions
-      sincospi(2.0 * (ph - rint(ph)), &s, &c);
+__global__ __launch_bounds__(256) void k_fresnel_merge(const T* __restrict__ partial,
+                                                       uint32_t n_slices, size_t np,
+                                                       double2* __restrict__ out) {
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (size_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (uint32_t s = 0; s < n_slices; ++s) {
+      const T v = partial[(size_t)s * np + p];
+      re += (double)v.x;
+      im += (double)v.y;
     }
-    if (amp) {
-      const double2 a = amp[j];
-      const double re = a.x * c - a.y * s, im = a.x * s + a.y * c;
-      c = re;
-      s = im;
-    }
-    T v;
-    v.x = c;
-    v.y = s;
-    out[e] = v;
+    out[p] = make_double2(re, im);
   }
-}
-
-// Block factors for the real product, FP32 (column-major):
-//   kind 0 (X):  A is 2n x 2nc, A[i][j] = A[n+i][nc+j] = Re, A[n+i][j] = Im,
-//                A[i][nc+j] = -Im;
-//   kind 1 (Z'): B is n x 2nc,  B[i][j] = Re, B[i][nc+j] = Im;
-// with Re + i Im = (amp_j or 1) exp(i 2 pi krev (p_i - r_j)^2).
-__global__ __launch_bounds__(256) void k_fresnel_factor_blk(const double* __restrict__ p,
-                                                            uint32_t n,
-                                                            const double* __restrict__ r,
-                                                            const double2* __restrict__ amp,
-                                                            uint32_t nc, double krev, int kind,
-                                                            float* __restrict__ out) {
-  const size_t total = (size_t)n * nc;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const size_t i = e % n, j = e / n;
-    double s, c;
-    {
-#pragma clang fp contract(off)
-      const double d = p[i] - r[j];
-      const double ph = krev * (d * d);  // revolutions
-      sincospi(2.0 * (ph - rint(ph)), &s, &c);
-    }
-    if (amp) {
-      const double2 a = amp[j];
-      const double re = a.x * c - a.y * s, im = a.x * s + a.y * c;
-      c = re;
-      s = im;
-    }
-    if (kind == 0) {
-      const size_t ld = 2 * (size_t)n;
-      out[i + j * ld] = (float)c;
-      out[i + (nc + j) * ld] = (float)-s;
-      out[n + i + j * ld] = (float)s;
-      out[n + i + (nc + j) * ld] = (float)c;
-    } else {
-      out[i + j * (size_t)n] = (float)c;
-      out[i + (nc + j) * (size_t)n] = (float)s;
-    }
-  }
-}
-
-// acc[ix + iz nx] += (C[ix + iz 2nx], C[nx + ix + iz 2nx]): the FP32 chunk
-// product [Ur; Ui] into the FP64 result.
-__global__ __launch_bounds__(256) void k_accum_blk(const float* __restrict__ c, uint32_t nx,
-                                                   size_t np, double2* __restrict__ acc) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < np;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const size_t ix = e % nx, iz = e / nx;
-    const float* col = c + iz * 2 * (size_t)nx;
-    double2 a = acc[e];
-    a.x += (double)col[ix];
-    a.y += (double)col[nx + ix];
-    acc[e] = a;
-  }
-}
-
-__global__ void k_zero_c64(double2* __restrict__ p, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    p[i] = make_double2(0.0, 0.0);
 }
 
 unsigned grid_for(size_t n) {
   return (unsigned)std::min<size_t>((n + 255) / 256, 256u * 64u);
 }
 
-// rocBLAS, loaded on first use (the Kirchhoff path never needs it).
-struct Blas {
-  using create_t = int (*)(void**);
-  using destroy_t = int (*)(void*);
-  using stream_t = int (*)(void*, hipStream_t);
-  using sgemm_t = int (*)(void*, int, int, int, int, int, const float*, const float*, int,
-                          const float*, int, const float*, float*, int);
-  using zgemm_t = int (*)(void*, int, int, int, int, int, const double2*, const double2*, int,
-                          const double2*, int, const double2*, double2*, int);
-  void* lib = nullptr;
-  create_t create = nullptr;
-  destroy_t destroy = nullptr;
-  stream_t set_stream = nullptr;
-  sgemm_t sgemm = nullptr;
-  zgemm_t zgemm = nullptr;
-  bool ok = false;
-};
-constexpr int kOpNone = 111, kOpTrans = 112;  // rocblas_operation
-
-Blas& blas() {
-  static Blas b;
-  static std::once_flag once;
-  std::call_once(once, [] {
-    for (const char* name : {"librocblas.so.5", "librocblas.so", "/opt/rocm/lib/librocblas.so"}) {
-      b.lib = dlopen(name, RTLD_NOW | RTLD_LOCAL);
-      if (b.lib) break;
-    }
-    if (!b.lib) return;
-    b.create = reinterpret_cast<Blas::create_t>(dlsym(b.lib, "rocblas_create_handle"));
-    b.destroy = reinterpret_cast<Blas::destroy_t>(dlsym(b.lib, "rocblas_destroy_handle"));
-    b.set_stream = reinterpret_cast<Blas::stream_t>(dlsym(b.lib, "rocblas_set_stream"));
-    b.sgemm = reinterpret_cast<Blas::sgemm_t>(dlsym(b.lib, "rocblas_sgemm"));
-    b.zgemm = reinterpret_cast<Blas::zgemm_t>(dlsym(b.lib, "rocblas_zgemm"));
-    b.ok = b.create && b.destroy && b.set_stream && b.sgemm && b.zgemm;
-  });
-  return b;
-}
-
 }  // namespace
 
-void fresnel_release(xrtk_ctx* ctx) {
-  if (ctx->blas) {
-    Blas& b = blas();
-    if (b.ok) (void)b.destroy(ctx->blas);
-    ctx->blas = nullptr;
-  }
-  ctx->fresnel.release();
-}
+void fresnel_release(xrtk_ctx* ctx) { ctx->fresnel.release(); }
 
 }  // namespace xrtk
 
@@ -217,10 +390,7 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
   if (!screen || !out2 || screen->nx == 0 || screen->nz == 0) return XRTK_EINVAL;
   if (n_rays && (!ray_x || !ray_z || !es_re || !es_im)) return XRTK_EINVAL;
   const uint32_t nx = screen->nx, nz = screen->nz;
-  // rocBLAS takes int dimensions; the FP32 product has 2 nx rows and 2 nc columns
-  if ((uint64_t)nx * nz > (1ull << 31) || nx > (1u << 29) || nz > (1u << 30) ||
-      n_rays > 0x7fffffffull)
-    return XRTK_EINVAL;
+  if ((uint64_t)nx * nz > 0x7fff0000ull || n_rays > 0x7fffffffull) return XRTK_EINVAL;
   const bool f64 = opts && opts->mode == XRTK_MODE_F64;
   try {
     std::vector<double> xp(nx), zp(nz);
@@ -233,85 +403,90 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
       std::memset(out2, 0, sizeof(double) * 2 * np);
       return XRTK_OK;
     }
-    Blas& b = blas();
-    if (!b.ok) return XRTK_ELAUNCH;
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceScope scope(ctx->device);
     hipStream_t st = ctx->stream;
-    if (!ctx->blas) {
-      if (b.create(&ctx->blas) != 0) { ctx->blas = nullptr; return XRTK_ELAUNCH; }
-    }
-    if (b.set_stream(ctx->blas, st) != 0) return XRTK_ELAUNCH;
-
-    // ray chunk: each factor block at most ~256 MiB (an FP32 X entry takes
-    // four floats in the block form)
-    const size_t esz = f64 ? sizeof(double2) : 4 * sizeof(float);
-    const size_t cap = (256ull << 20) / (esz * std::max(nx, nz));
-    size_t nc = std::min<size_t>(n_rays, std::max<size_t>(256, cap / 256 * 256));
-    // device buffers: pixel axes, rays (x, z, Es), factors, chunk product, result
+    // launch shape: tiles of the screen x ray slices; slices of at most
+    // kSlice32 rays for the FP32 sums, and enough of them for ~4 workgroups
+    // per CU over the launch
+    const uint32_t T = f64 ? kT64 : kT32, KB = f64 ? kKB64 : kKB32;
+    const uint32_t n_tx = (nx + T - 1) / T, n_tz = (nz + T - 1) / T;
+    const uint64_t tiles = (uint64_t)n_tx * n_tz;
+    const uint64_t want = std::max<uint64_t>(1, (4ull * ctx->n_cu + tiles - 1) / tiles);
+    uint64_t n_sl = std::max<uint64_t>(want, f64 ? 1 : (n_rays + kSlice32 - 1) / kSlice32);
+    n_sl = std::min<uint64_t>(n_sl, (n_rays + 2 * KB - 1) / (2 * KB));
+    // a slice is a whole number of block pairs (the kernels alternate two LDS
+    // buffers); the rays are padded with zeros to whole slices plus the one
+    // block the kernels generate past a slice's end
+    uint64_t slice = (n_rays + n_sl - 1) / n_sl;
+    slice = (slice + 2 * KB - 1) / (2 * KB) * (2 * KB);
+    n_sl = (n_rays + slice - 1) / slice;
+    const uint64_t nr_pad = n_sl * slice + KB;
+    if (n_sl > 65535 || tiles > 0x7fffffffull) return XRTK_EINVAL;
+    // device buffers: pixel axes, rays (x, z, Es; zero-padded to whole
+    // slices), partial sums per slice, result
+    const size_t psz = f64 ? sizeof(double2) : sizeof(float2);
     DBuf& W = ctx->fresnel;
-    const size_t o_xp = 0, o_zp = o_xp + 8 * nx, o_rx = (o_zp + 8 * nz + 255) / 256 * 256;
-    const size_t o_rz = o_rx + 8 * n_rays, o_es = (o_rz + 8 * n_rays + 255) / 256 * 256;
-    const size_t o_fx = (o_es + 16 * n_rays + 255) / 256 * 256;
-    const size_t o_fz = (o_fx + esz * nx * nc + 255) / 256 * 256;
-    const size_t o_c = (o_fz + esz * nz * nc + 255) / 256 * 256;
-    const size_t o_u = (o_c + (f64 ? 0 : 2 * sizeof(float) * np) + 255) / 256 * 256;
+    const size_t o_xp = 0, o_zp = (8 * (size_t)nx + 255) / 256 * 256;
+    const size_t o_rx = o_zp + (8 * (size_t)nz + 255) / 256 * 256;
+    const size_t o_rz = o_rx + (8 * nr_pad + 255) / 256 * 256;
+    const size_t o_es = o_rz + (8 * nr_pad + 255) / 256 * 256;
+    const size_t o_p = o_es + (16 * nr_pad + 255) / 256 * 256;
+    const size_t o_u = o_p + (psz * n_sl * np + 255) / 256 * 256;
     const size_t total = o_u + sizeof(double2) * np;
     if (W.ensure(total) != hipSuccess) return XRTK_ENOMEM;
     char* base = W.as<char>();
-    std::vector<double> es(2 * n_rays);
+    std::vector<double> rxp(nr_pad, 0.0), rzp(nr_pad, 0.0), es(2 * nr_pad, 0.0);
+    std::memcpy(rxp.data(), ray_x, 8 * n_rays);
+    std::memcpy(rzp.data(), ray_z, 8 * n_rays);
     for (size_t j = 0; j < n_rays; ++j) { es[2 * j] = es_re[j]; es[2 * j + 1] = es_im[j]; }
     hipError_t e = hipMemcpyAsync(base + o_xp, xp.data(), 8 * nx, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_zp, zp.data(), 8 * nz, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rx, ray_x, 8 * n_rays, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rz, ray_z, 8 * n_rays, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_es, es.data(), 16 * n_rays, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rx, rxp.data(), 8 * nr_pad, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rz, rzp.data(), 8 * nr_pad, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_es, es.data(), 16 * nr_pad, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return map_err(e);
     const double k = energy / kCHBAR_H * 1e7;
-    const double krev = k / (2.0 * dist) / (2.0 * kPi);
+    const double krev = k / (2.0 * dist) / (2.0 * M_PI);
     const double* dxp = reinterpret_cast<const double*>(base + o_xp);
     const double* dzp = reinterpret_cast<const double*>(base + o_zp);
     const double* drx = reinterpret_cast<const double*>(base + o_rx);
     const double* drz = reinterpret_cast<const double*>(base + o_rz);
     const double2* des = reinterpret_cast<const double2*>(base + o_es);
     double2* U = reinterpret_cast<double2*>(base + o_u);
-    hipLaunchKernelGGL(k_zero_c64, dim3(grid_for(np)), dim3(256), 0, st, U, np);
     (void)hipEventRecord(ctx->ev[0], st);
-    for (size_t j0 = 0; j0 < n_rays; j0 += nc) {
-      const uint32_t c = (uint32_t)std::min<size_t>(nc, n_rays - j0);
-      if (f64) {
-        double2* FX = reinterpret_cast<double2*>(base + o_fx);
-        double2* FZ = reinterpret_cast<double2*>(base + o_fz);
-        hipLaunchKernelGGL(k_fresnel_factor<double2>, dim3(grid_for((size_t)nx * c)), dim3(256),
-                           0, st, dxp, nx, drx + j0, nullptr, c, krev, FX);
-        hipLaunchKernelGGL(k_fresnel_factor<double2>, dim3(grid_for((size_t)nz * c)), dim3(256),
-                           0, st, dzp, nz, drz + j0, des + j0, c, krev, FZ);
-        const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-        if (b.zgemm(ctx->blas, kOpNone, kOpTrans, (int)nx, (int)nz, (int)c, &one, FX, (int)nx,
-                    FZ, (int)nz, j0 ? &one : &zero, U, (int)nx) != 0)
-          return XRTK_ELAUNCH;
-      } else {
-        float* FX = reinterpret_cast<float*>(base + o_fx);
-        float* FZ = reinterpret_cast<float*>(base + o_fz);
-        float* Cc = reinterpret_cast<float*>(base + o_c);
-        hipLaunchKernelGGL(k_fresnel_factor_blk, dim3(grid_for((size_t)nx * c)), dim3(256), 0,
-                           st, dxp, nx, drx + j0, nullptr, c, krev, 0, FX);
-        hipLaunchKernelGGL(k_fresnel_factor_blk, dim3(grid_for((size_t)nz * c)), dim3(256), 0,
-                           st, dzp, nz, drz + j0, des + j0, c, krev, 1, FZ);
-        const float one = 1.f, zero = 0.f;
-        if (b.sgemm(ctx->blas, kOpNone, kOpTrans, 2 * (int)nx, (int)nz, 2 * (int)c, &one, FX,
-                    2 * (int)nx, FZ, (int)nz, &zero, Cc, 2 * (int)nx) != 0)
-          return XRTK_ELAUNCH;
-        hipLaunchKernelGGL(k_accum_blk, dim3(grid_for(np)), dim3(256), 0, st, Cc, nx, np, U);
-      }
+    (void)hipEventRecord(ctx->ev[1], st);
+    const dim3 grid((unsigned)tiles, (unsigned)n_sl);
+    if (f64) {
+      double2* Pd = reinterpret_cast<double2*>(base + o_p);
+      hipLaunchKernelGGL(k_fresnel_mfma64, grid, dim3(256), 0, st, dxp, nx, dzp, nz, drx, drz, des,
+                         (uint32_t)slice, krev, n_tx, Pd);
+      e = hipGetLastError();
+      (void)hipEventRecord(ctx->ev[2], st);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_fresnel_merge<double2>, dim3(grid_for(np)), dim3(256), 0, st, Pd,
+                           (uint32_t)n_sl, np, U);
+    } else {
+      float2* Pf = reinterpret_cast<float2*>(base + o_p);
+      hipLaunchKernelGGL(k_fresnel_mfma32, grid, dim3(256), 0, st, dxp, nx, dzp, nz, drx, drz, des,
+                         (uint32_t)slice, krev, n_tx, Pf);
+      e = hipGetLastError();
+      (void)hipEventRecord(ctx->ev[2], st);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_fresnel_merge<float2>, dim3(grid_for(np)), dim3(256), 0, st, Pf,
+                           (uint32_t)n_sl, np, U);
     }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return map_err(e);
     (void)hipEventRecord(ctx->ev[3], st);
     e = hipMemcpyAsync(out2, U, sizeof(double2) * np, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return map_err(e);
-    float ms = 0.f;
+    float ms = 0.f, kms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) != hipSuccess) ms = 0.f;
-    ctx->last_pair_ms = ms;
+    if (hipEventElapsedTime(&kms, ctx->ev[1], ctx->ev[2]) != hipSuccess) kms = 0.f;
+    (void)hipGetLastError();
+    ctx->last_pair_ms = kms;
     ctx->last_solve_ms = ms;
     return XRTK_OK;
   } catch (...) {

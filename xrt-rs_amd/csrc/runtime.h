@@ -178,8 +178,7 @@ struct xrtk_ctx {
     std::vector<XrtkJob> jobs;
   };
   std::vector<GraphEntry*> graphs;
-  // Fresnel propagation: rocBLAS handle (created on first use) and workspace
-  void* blas = nullptr;
+  // Fresnel propagation workspace
   xrtk::DBuf fresnel;
   // undulator maps
   xrtk::DBuf undulator;
