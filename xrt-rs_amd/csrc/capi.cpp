@@ -232,6 +232,9 @@ int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* 
 // The one-shot pipeline: group, route, upload, solve, copy back.
 int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& ps, double skip,
                   double* out4, const XrtkOpts* opts) {
+  Range whole("xrtk.solve");
+  Phases ph;
+  ph.next("xrtk.plan");
   const uint32_t want = mode_from(opts);
   const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
   // the F32R shape fixes the slot layout even when only the FP64 kernel runs
@@ -320,6 +323,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     gs.push_back(G{0, nr, MODE_F64, 0.0});
   }
   // ---- buffers
+  ph.next("xrtk.upload");
   const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
                                   : align256(13 * sizeof(double) * (size_t)nr);
   const size_t pix_stage = ps.grid ? 0 : (ps.f32 ? sizeof(XrtkGpuPixel) * np
@@ -412,6 +416,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   int rc = run_instances(ctx, c32, c64, specs, ws.work, ws.meta, true, &plan, split, force,
                          false);
   if (rc != XRTK_OK) return rc;
+  ph.next("xrtk.readback");
   e = hipMemcpyAsync(out4, ws.outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
                      ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -632,6 +637,7 @@ int xrtk_rayset_upload(xrtk_ctx* ctx, const XrtkRaysF64* rays, const double* pat
   try {
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
+    Range range("xrtk.rayset.upload");
     RaySet rs;
     rs.nr = (uint32_t)n_rays;
     rs.has_path = path_up != nullptr;
@@ -679,6 +685,7 @@ int xrtk_rayset_from_beam(xrtk_ctx* ctx, const double* bx, const double* by, con
   try {
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
+    Range range("xrtk.rayset.from_beam");
     RaySet rs;
     rs.nr = (uint32_t)n_good;
     rs.has_path = path != nullptr;

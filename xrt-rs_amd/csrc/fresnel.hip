@@ -406,6 +406,9 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceScope scope(ctx->device);
     hipStream_t st = ctx->stream;
+    Range whole("xrtk.fresnel");
+    Phases ph;
+    ph.next("xrtk.upload");
     // launch shape: tiles of the screen x ray slices; slices of at most
     // kSlice32 rays for the FP32 sums, and enough of them for ~4 workgroups
     // per CU over the launch
@@ -454,6 +457,7 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     const double* drz = reinterpret_cast<const double*>(base + o_rz);
     const double2* des = reinterpret_cast<const double2*>(base + o_es);
     double2* U = reinterpret_cast<double2*>(base + o_u);
+    ph.next("xrtk.pair");
     (void)hipEventRecord(ctx->ev[0], st);
     (void)hipEventRecord(ctx->ev[1], st);
     const dim3 grid((unsigned)tiles, (unsigned)n_sl);
@@ -479,6 +483,7 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return map_err(e);
     (void)hipEventRecord(ctx->ev[3], st);
+    ph.next("xrtk.readback");
     e = hipMemcpyAsync(out2, U, sizeof(double2) * np, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return map_err(e);

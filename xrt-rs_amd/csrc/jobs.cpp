@@ -179,7 +179,9 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
     for (auto* g2 : ctx->graphs)
       if (g2->key == key) { ge = g2; break; }
     hipError_t e;
+    Phases ph;
     if (!ge) {
+      ph.next("xrtk.jobs.build");  // layout, grids and the first (eager) run
       auto* ne = new xrtk_ctx::GraphEntry();
       ne->key = key;
       auto drop = [&](xrtk_ctx::GraphEntry* x) {
@@ -203,6 +205,7 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
       if (rc != XRTK_OK) { drop(ne); return rc; }
       if (use_graph) {
+        ph.next("xrtk.jobs.capture");
         hipGraph_t graph = nullptr;
         e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
@@ -234,6 +237,7 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
     } else if (!ge->exec) {
       // no graph: the same kernels again, eagerly (the metadata the first
       // run uploaded is unchanged)
+      ph.next("xrtk.jobs.replay");
       int rc = enqueue_job_grids(ctx, c32, ge->jobs.data(), ge->specs);
       if (rc == XRTK_OK)
         rc = run_instances(ctx, c32, c64, ge->specs, ge->work, ge->meta, true, nullptr, 0, 0,
@@ -247,6 +251,7 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       ctx->last_pair_ms = a;
       ctx->last_solve_ms = b;
     } else {
+      ph.next("xrtk.jobs.replay");
       (void)hipEventRecord(ctx->ev[0], ctx->stream);
       e = hipGraphLaunch(ge->exec, ctx->stream);
       if (e != hipSuccess) return map_err(e);
@@ -272,6 +277,7 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       q.reserved = 0;
     }
     // results -> host
+    ph.next("xrtk.readback");
     for (size_t i = 0; i < n_jobs; ++i) {
       const size_t np = ge->outs[i].second;
       if (!np) continue;

@@ -227,6 +227,8 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
                   Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
                   bool upload) {
+  Phases ph;
+  ph.next("xrtk.layout");  // work items and instance metadata
   hipStream_t st = ctx->stream;
   if (!pair_cfg_supported(MODE_F32R, cfg32) || !pair_cfg_supported(MODE_F64, cfg64))
     return XRTK_EINVAL;
@@ -344,6 +346,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     if (e != hipSuccess) return map_err(e);
   }
 
+  ph.next("xrtk.prep");
   if (timed) (void)hipEventRecord(ctx->ev[0], st);
   // ---- prep of every instance
   for (size_t i = 0; i < n; ++i) {
@@ -377,6 +380,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     }
     return g;
   };
+  ph.next("xrtk.pair");
   if (timed) (void)hipEventRecord(ctx->ev[1], st);
   if (!items32.empty()) {
     const uint32_t remap = (!cfg32.persistent && n == 1) ? 1u : 0u;
@@ -392,6 +396,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   }
   if (timed) (void)hipEventRecord(ctx->ev[2], st);
   // ---- merge
+  ph.next("xrtk.merge");
   if (!first.empty()) {
     e = launch_merge(d_insts, d_lists, (uint32_t)first.size(), first_np, st);
     if (e != hipSuccess) return map_err(e);

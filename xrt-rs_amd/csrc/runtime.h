@@ -2,6 +2,7 @@
 // the solve pipeline shared by the one-shot, batched and serving entry points.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <atomic>
 #include <cstddef>
@@ -246,7 +247,33 @@ int kirchhoff_f32_ws(xrtk_ctx* ctx, Workspace& ws, const XrtkGpuRay* rays, size_
 // looked up once per device (noalloc.hip); nullptr when the lookup failed.
 void* noalloc_scratch(int device, size_t* bytes);
 double now_s();
-// Releases the Fresnel workspace and rocBLAS handle of a context.
+// Releases the Fresnel workspace of a context.
 void fresnel_release(xrtk_ctx* ctx);
+
+// A roctx range over one host phase of a solve ("xrtk.upload",
+// "xrtk.pair", ...), shown by rocprofv3 --marker-trace beside the kernel
+// trace.  Ranges mark host time: a range around an enqueue covers the
+// launch, the range around a readback includes the wait for the kernels
+// queued before it.  Without a profiler attached the push and pop return at
+// once.
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
+// Consecutive phases of one function: next() closes the open range and
+// opens the next one; the last closes with the object.
+struct Phases {
+  bool open = false;
+  void next(const char* name) {
+    if (open) roctxRangePop();
+    roctxRangePushA(name);
+    open = true;
+  }
+  ~Phases() {
+    if (open) roctxRangePop();
+  }
+};
 
 }  // namespace xrtk

@@ -239,6 +239,7 @@ struct xrtk_server {
 
   // Carves the block, fills the KInst, enqueues grid + prep.
   int start_request(Req* r) {
+    Range range("xrtk.server.start");  // slots, upload and prep of one request
     char* p = static_cast<char*>(r->dblk);
     InstSpec& s = r->spec;
     const bool f32 = s.mode == MODE_F32R;
@@ -331,6 +332,7 @@ struct xrtk_server {
   // A request whose launch failed (rc set) only records its event: its
   // partial sums were never all computed, so nothing is merged or copied.
   void enqueue_merge(Req* r) {
+    Range range("xrtk.server.merge");  // merge and readback of one request
     r->state = MERGING;
     hipError_t e = hipSuccess;
     if (r->rc == XRTK_OK && r->spec.nr)
@@ -349,6 +351,7 @@ struct xrtk_server {
 
   // Fills and enqueues one quantum.  Returns false when there was no work.
   bool launch_quantum() {
+    Range range("xrtk.server.quantum");  // work items and pair launches of one quantum
     Launch& L = launches[cur];  // the caller has retired it
     double budget = std::max(1.0, cfg.quantum_ms) * 1e-3 * rate_pairs_per_s;
     if (c32.tile > 0) {
