@@ -66,7 +66,7 @@ FLOP_PER_WAVE_INST = 128
 # Issue rates of gfx950 measured by a microbenchmark (16 independent chains
 # per lane, 8 waves per SIMD), wave-instructions per CU per clock: v_fmac_f32
 # 1.417, v_sin_f32 / v_cos_f32 ~0.435 (packed FP32 excluded).
-ISSUE_FMA, ISSUE_TRANS = 1.417, 0.435
+ISSUE_FMA = 1.417  # v_fmac_f32 wave-instructions per CU per clock, measured
 COUNTER_FILE = os.path.join(ROOT, "profiles", "pair_kernel_counters.json")
 
 
@@ -162,10 +162,10 @@ def valu_roofline(rec, kernel_ms, pairs):
         clk = rec.get("sclk_ghz", 2.4) * 1e9
         rate = vi / (kernel_ms * 1e-3) / rec["n_cu"] / clk
         roof["valu_issue_per_cu_per_clk"] = rate
-        if ti is not None:
-            # share of the issue capacity measured for these instruction kinds
-            t_rate = ti / (kernel_ms * 1e-3) / rec["n_cu"] / clk
-            roof["issue_frac_measured_rates"] = (rate - t_rate) / ISSUE_FMA + t_rate / ISSUE_TRANS
+        roof["sclk_ghz_assumed"] = clk / 1e9
+        # share of the back-to-back v_fmac_f32 issue rate measured on this
+        # part (every VALU instruction taken as one issue slot)
+        roof["issue_frac_fmac"] = rate / ISSUE_FMA
     roof["counter_source"] = rec.get("source")
     return roof
 

@@ -101,7 +101,8 @@ def test_config0_aperture_to_256sq(ctx):
 
 
 def test_ray_chunks(ctx):
-    """A wide screen makes the ray chunk short (several GEMMs accumulate)."""
+    """A wide, short screen has few tiles, so the rays split into many slices
+    whose partial sums are merged in FP64."""
     rx, rz, es = _aperture(20000, 0.05, 8000.0, 11)
     scr = xrtk.FresnelScreen(-0.2, 0.2, 8192, -0.01, 0.01, 3)
     stride = 211
@@ -110,3 +111,30 @@ def test_ray_chunks(ctx):
     got = run_both(ctx, rx, rz, es, 8000.0, 500.0, scr)
     assert rel_l2(got["f64"].amplitude[::stride], ref) < REL_F64
     assert rel_l2(got["f32r"].amplitude[::stride], ref) < REL_F32
+
+
+def test_ragged_rays_and_tiles(ctx):
+    """A ray count that is not a whole number of LDS blocks (zero-padded) on
+    a screen one to three pixels past whole 128- and 64-pixel tiles."""
+    rx, rz, es = _aperture(4099, 0.08, 12000.0, 17)
+    scr = xrtk.FresnelScreen(-0.05, 0.05, 130, -0.05, 0.04, 131)
+    ref, _, _ = O.fresnel_f64(rx, rz, es, 12000.0, 800.0, scr.x_min, scr.x_max, 130,
+                              scr.z_min, scr.z_max, 131)
+    got = run_both(ctx, rx, rz, es, 12000.0, 800.0, scr)
+    assert rel_l2(got["f64"].amplitude, ref) < REL_F64
+    assert rel_l2(got["f32r"].amplitude, ref) < REL_F32
+
+
+def test_slices_set_by_the_fp32_cap(ctx):
+    """A screen of 4096^2 pixels fills the device with tiles alone, so only
+    the 16,384-ray cap on an FP32 sum splits the rays: 40,000 rays take
+    three slices of 13,344, the last one partly zero-padded."""
+    rx, rz, es = _aperture(40000, 0.1, 10000.0, 19)
+    scr = xrtk.FresnelScreen(-0.1, 0.1, 4096, -0.1, 0.1, 4096)
+    stride = 40009
+    ref, _, _ = O.fresnel_f64(rx, rz, es, 10000.0, 1000.0, -0.1, 0.1, 4096, -0.1, 0.1, 4096,
+                              stride=stride)
+    got = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx)
+    e32 = rel_l2(got.amplitude[::stride], ref)
+    print(f"fresnel 40000 x 4096^2 (three FP32 slices): {e32:.2e}")
+    assert e32 < REL_F32

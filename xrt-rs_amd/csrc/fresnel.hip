@@ -20,7 +20,8 @@
 //        by v_sin/v_cos; each workgroup's FP32 sums cover at most 16,384
 //        rays (a ray slice) and go to HBM as partial sums;
 //   FP64 (XRTK_MODE_F64): v_mfma_f64_16x16x4f64, 64 x 64 pixels per
-//        workgroup (32 x 32 per wave), FP64 sincospi of the reduced phase.
+//        workgroup (32 x 32 per wave); e^{2 pi i f} of the reduced phase f
+//        from a 256-entry table and a degree-7 Taylor series (cis_rev_tab).
 // The complex product takes four real products per pair of factors:
 //   Re U += Zr Xr + Zi (-Xi),  Im U += Zr Xi + Zi Xr   (8 flops per pair).
 // k_fresnel_merge* sums the ray slices' partials in FP64, in slice order.
