@@ -102,8 +102,16 @@ typedef struct XrtkOpts {
     int32_t ray_src;          /* XRTK_RAYSRC_*                              */
     int32_t persistent;       /* -1 = autotune, 0 = one group per item,
                                  1 = persistent groups pulling a queue       */
-    int32_t reserved;
+    int32_t engine;           /* XRTK_ENGINE_*                              */
 } XrtkOpts;
+
+/* Which pair kernels an FP32-reduced solve may use.  AUTO: an instance on a
+ * prepare_wave grid whose phase is linear over 64 x 64-pixel blocks to
+ * within 1e-7 rev (bounded by the device planner) runs on the matrix cores
+ * (FP16-split MFMA products, FP32 block sums); every other instance, and
+ * VALU always, on the FP32 VALU pair kernels. */
+#define XRTK_ENGINE_AUTO 0
+#define XRTK_ENGINE_VALU 1
 
 /* Fills *o with the defaults (mode F32R, everything autotuned). */
 void xrtk_opts_default(XrtkOpts *o);
@@ -341,6 +349,7 @@ typedef struct XrtkPlan {
 #define XRTK_VAR_LINE      256 /* phase along a lane's pixel line by rotation    */
 #define XRTK_VAR_LINEDEV   1024 /* LINE with per-pixel offsets from the line     */
 #define XRTK_VAR_EPZERO    2048 /* every ray has Ep = 0: no Ep sums are formed    */
+#define XRTK_VAR_MFMA      4096 /* 64 x 64-pixel blocks on the matrix cores       */
 int xrtk_last_plan(xrtk_ctx *ctx, XrtkPlan *plan);
 /* Backend used by the last xrtk_kirchhoff_auto_f32 call of this process:
  * 1 = device, 0 = CPU fallback, -1 = none yet. */

@@ -278,9 +278,10 @@ def test_non_square_screens_equal_batched(ctx, sets):
 
 def test_poisson_energy_scan_stream(ctx, sets):
     """A served stream in the form of the energy-scan workload, shortened:
-    Poisson arrivals (100 req/s for 3 s, seeded) of energy scans (1-50
-    energies, 5 eV apart) over the resident ray sets, admitted under a
-    budget and batched continuously.  Every result equals the batched solve
+    Poisson arrivals (1000 req/s for 0.3 s, seeded: dense enough that these
+    small requests share launches) of energy scans (1-50 energies, 5 eV
+    apart) over the resident ray sets, admitted under a budget and batched
+    continuously.  Every result equals the batched solve
     of the same (ray set, energy, screen) to FP32 rounding (the 10 ms
     quantum cuts the ray chunks), requests overlap on the device, and the
     timestamps are ordered submit <= admit <= first launch <= done."""
@@ -292,8 +293,8 @@ def test_poisson_energy_scan_stream(ctx, sets):
         i = int(rng.integers(len(specs)))
         m = int(rng.integers(1, 51))
         reqs += [(i, specs[i]["energy"] + 5.0 * (k - m // 2)) for k in range(m)]
-    arrivals = np.cumsum(rng.exponential(1.0 / 100.0, len(reqs)))
-    n = int(np.searchsorted(arrivals, 3.0))
+    arrivals = np.cumsum(rng.exponential(1.0 / 1000.0, len(reqs)))
+    n = int(np.searchsorted(arrivals, 0.3))
     reqs, arrivals = reqs[:n], arrivals[:n]
     screens = [grid(W.instance_screen(s)) for s in specs]
     srv = xrtk.Server(ctx, hbm_budget_bytes=4 << 30, max_active=64, quantum_ms=10.0)

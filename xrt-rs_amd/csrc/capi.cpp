@@ -165,7 +165,7 @@ void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
   q.split = c.split;
   q.ray_src = c.ray_src;
   q.persistent = c.persistent;
-  q.n_items = (int32_t)(p.n_items32 + p.n_items64);
+  q.n_items = (int32_t)(p.n_items32 + p.n_items64 + p.n_items_mfma);
   q.n_fp64 = (int32_t)p.n_f64;
   q.backend = backend;
   q.inst_flags = 0;
@@ -398,6 +398,11 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     s.skip = skip;
     s.out = ws.outbuf.as<double>();
     s.accumulate = gi > 0;
+    if (ps.grid && np) {
+      uint32_t th, tw;
+      grid_tile(c32.P, ps.grid->nx, ps.grid->nz, &th, &tw);
+      spec_set_grid(s, *ps.grid, th, tw);
+    }
     specs.push_back(s);
   }
   if (specs.empty()) {  // no rays: zeros
@@ -413,6 +418,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   const int split = opts && opts->split > 0 ? opts->split : 0;
   uint32_t force = 0;
   if (const char* f = std::getenv("XRTK_FORCE_FLAGS")) force = (uint32_t)std::strtoul(f, nullptr, 0);
+  if (opts && opts->engine == XRTK_ENGINE_VALU) force |= 0x20000u;
   int rc = run_instances(ctx, c32, c64, specs, ws.work, ws.meta, true, &plan, split, force,
                          false);
   if (rc != XRTK_OK) return rc;

@@ -351,7 +351,8 @@ __global__ __launch_bounds__(256) void k_centroid_final(
   if (threadIdx.x == 0) {
     const double inv = n ? 1.0 / (double)n : 0.0;
     scr->cx = sx[0] * inv; scr->cy = sy[0] * inv; scr->cz = sz[0] * inv;
-    scr->pad0 = 0.0;
+    scr->blk_rev = 0.0;
+    scr->amax_bits = 0u;
     for (int k = 0; k < 4; ++k) scr->omax_bits[k] = 0u;
     for (int k = 0; k < 3; ++k) scr->qmax_bits[k] = scr->dqmax_bits[k] = 0u;
     for (int k = 0; k < 3; ++k) { scr->qlo_key[k] = 0xffffffffu; scr->qhi_key[k] = 0u; }
@@ -674,7 +675,8 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 // then takes its own normal, as rays off a curved optic do), 0x10000 forbids
 // EPZERO.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
-                       uint32_t force_flags, uint32_t static_flags, uint32_t P) {
+                       uint32_t force_flags, uint32_t static_flags, uint32_t P, double gsx,
+                       double gsz) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
   const double oz = __uint_as_float(scr->omax_bits[2]), om = __uint_as_float(scr->omax_bits[3]);
   const double qx = __uint_as_float(scr->qmax_bits[0]), qy = __uint_as_float(scr->qmax_bits[1]);
@@ -744,6 +746,41 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         }
       }
     }
+    // MFMA (a grid screen in the x-z plane, steps gsx, gsz): over a block of
+    // 64 x 64 pixels the reduced phase f(q) = kr (|q - o| - |q|) is taken
+    // linear about the block centre a, f(a) + grad f(a).(q - a).  The
+    // remainder is at most 1/2 max|f_ee| |q - a|^2, |q - a| <= h = 32 |(gsx,
+    // gsz)|, with e a unit vector in the screen plane, d = q - o, and
+    //   f_ee = kr [(1/|d| - 1/|q|) - ((d.e)^2/|d|^3 - (q.e)^2/|q|^3)],
+    //   |1/|d| - 1/|q|| <= dR / Rl^2,  dR = (|o|^2 + 2 sum |o_i| Q_i) / (2 Rl)
+    //     (a bound of ||d| - |q||),
+    //   (d.e)^2/|d|^3 - (q.e)^2/|q|^3 = (q.e)^2 (1/|d|^3 - 1/|q|^3)
+    //     + ((o.e)^2 - 2 (q.e)(o.e)) / |d|^3,
+    // where Rl = min|q| - |o|max - h bounds |q| and |d| from below over the
+    // blocks, Q_i = max|q_i| (+ h in the plane), and |q.e| <= |(Q_x, Q_z)|,
+    // |o.e| <= |(o_x, o_z)|.  Allowed when the bound is <= 1e-7 rev and the
+    // phase steps stay small enough (|grad f . step| <= 2 kr |step| |o| / Rl
+    // per pixel, times 16 from a tile centre, plus the half revolution of the
+    // anchor: <= 8 rev) that FP32 holds them to ~5e-7 rev.
+    const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
+                            INST_F_EPZERO;
+    if ((static_flags & INST_F_MFMA_OK) && (f & need_m) == need_m && !(force_flags & 0x20000u)) {
+      const double ax = fabs(gsx), az = fabs(gsz);
+      const double h = 32.0 * sqrt(ax * ax + az * az);
+      const double Rl = rmin - om - h;
+      if (Rl > 0.0) {
+        const double Qx = qx + h, Qy = qy, Qz = qz + h;
+        const double dR = (om * om + 2.0 * (ox * Qx + oy * Qy + oz * Qz)) / (2.0 * Rl);
+        const double R2 = Rl * Rl, R3 = R2 * Rl;
+        const double Qp2 = Qx * Qx + Qz * Qz, op = sqrt(ox * ox + oz * oz);
+        const double t1 = dR / R2;
+        const double t2 = Qp2 * 3.0 * dR / (R2 * R2) + (2.0 * sqrt(Qp2) * op + op * op) / R3;
+        const double E = 0.5 * kr * (t1 + t2) * h * h;
+        const double smax = 2.0 * kr * fmax(ax, az) * om / Rl;
+        scr->blk_rev = E;
+        if (E <= 1e-7 && 16.0 * smax + 0.5 <= 8.0) f |= INST_F_MFMA;
+      }
+    }
   }
   if (force_flags & 0x200u) f |= INST_F_CHECK;
   scr->flags = f;
@@ -760,13 +797,12 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
 // from delta kept in the record; the centroid-referenced variant, used only
 // when the whole phase is small, takes kr delta.u once per ray, u from the
 // centroid to the centre of the screen's bounding box.
-__global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__ rays,
-                                                    uint32_t n,
-                                                    const InstScratch* __restrict__ scr,
-                                                    double kr, double kr_lo, uint32_t have_pix,
-                                                    RayRec32* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Returns |cs| of the record (the MFMA kernel scales its FP16 factors by the
+// largest).
+__device__ __forceinline__ float ray_prep_one(const RayRec64* __restrict__ rays, uint32_t i,
+                                              const InstScratch* __restrict__ scr, double kr,
+                                              double kr_lo, uint32_t have_pix,
+                                              RayRec32* __restrict__ out) {
   const double cx = scr->cx, cy = scr->cy, cz = scr->cz;
   const uint32_t flags = scr->flags;
   int ax[3];
@@ -842,6 +878,21 @@ __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__
   o.am = (float)am;
   o.off = (float)off;
   out[i] = o;
+  return fabsf(o.cs);
+}
+
+__global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__ rays,
+                                                    uint32_t n, InstScratch* __restrict__ scr,
+                                                    double kr, double kr_lo, uint32_t have_pix,
+                                                    RayRec32* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  float a = 0.f;
+  if (i < n) a = ray_prep_one(rays, i, scr, kr, kr_lo, have_pix, out);
+  if (scr->flags & INST_F_MFMA) {
+    // every lane of the wave takes part in the reduction (lanes past n hold 0)
+    a = wave_max(a);
+    if ((threadIdx.x & 63u) == 0u) atomicMax(&scr->amax_bits, __float_as_uint(a));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1297,8 +1348,230 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
     const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
-    if (line_variant<P>(f) == LINE) f32r_dispatch<P, W, LINE>(f, I, w, lds, lane, wave);
+    // (instances on the matrix cores are k_pair_mfma's)
+    if (!(f & INST_F_MFMA) && line_variant<P>(f) == LINE)
+      f32r_dispatch<P, W, LINE>(f, I, w, lds, lane, wave);
     if constexpr (!PERSIST) break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block pair kernel on the matrix cores (MFMA variant of the FP32 kernel)
+// ---------------------------------------------------------------------------
+// Over a block of 64 x 64 grid pixels with centre a, a ray's reduced phase
+// is linear in the pixel offsets (u, v) from a, up to a remainder that
+// k_plan bounds (<= 1e-7 rev):
+//   f(u, v) = ph + u sx + v sz,
+// so its term factors into a row factor and a column factor,
+//   A e^{2 pi i f} = [A e^{2 pi i (ph + u sx)}] [e^{2 pi i v sz}],
+// and the block's field is a complex product of a 64 x K row-factor matrix
+// and a K x 64 column-factor matrix over the K rays of the item's chunk.
+// ph (with the ray's phase offset and the FP64 rounding correction of o),
+// sx and sz are formed per (ray, block) in FP64, by one lane per ray; the
+// factors are then generated in FP32 in the layout of the MFMA operands and
+// split into FP16 pairs x = hi + lo (both rounded to nearest, so x - hi - lo
+// is ~2^-22 |x|); the three products hi.hi + hi.lo + lo.hi run on
+// v_mfma_f32_32x32x16_f16 with FP32 accumulation, leaving ~2^-21 per term.
+// Amplitudes are scaled by a power of two so the largest is in [1/2, 1).
+// Each wave owns one block (2 x 2 tiles of 32 x 32 pixels) and one ray
+// chunk, and writes the chunk's partial sums of the block's real pixels to
+// the slots the F32R kernel would write (k_merge is shared).
+typedef _Float16 xh8 __attribute__((ext_vector_type(8)));
+typedef _Float16 xh2 __attribute__((ext_vector_type(2)));
+typedef float xf2 __attribute__((ext_vector_type(2)));
+typedef float xf16 __attribute__((ext_vector_type(16)));
+
+static constexpr uint32_t kMfmaBlk = 64;  // block side in pixels
+
+// Per-(ray, block) values, one array each, 64 rays per LDS block.
+enum : int {
+  MR_PH = 0,  // phase at the centre of the first row tile, revolutions in [-1/2, 1/2]
+  MR_SX,      // phase step per row (grid x)
+  MR_GZ,      // phase at the centre of the first column tile relative to a
+  MR_SZ,      // phase step per column (grid z)
+  MR_AMP,     // scaled amplitude
+  MR_RXC, MR_RXS,  // e^{2 pi i 32 sx}: first row tile to the second
+  MR_RZC, MR_RZS,  // e^{2 pi i 32 sz}
+  MR_N
+};
+
+// hi = RNE(x), lo = RNE(x - hi) of 8 values, packed as an MFMA operand.
+// m1 is -1 held in a register: fma(hi, m1, x) is then one v_fma_mix_f32.
+__device__ __forceinline__ void split8(const float (&x)[8], float m1, xh8& hi, xh8& lo) {
+  xh2 h[4], l[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const xf2 p = {x[2 * j], x[2 * j + 1]};
+    h[j] = __builtin_convertvector(p, xh2);
+    const xf2 r = {__builtin_fmaf((float)h[j].x, m1, p.x), __builtin_fmaf((float)h[j].y, m1, p.y)};
+    l[j] = __builtin_convertvector(r, xh2);
+  }
+  hi = (xh8){h[0].x, h[0].y, h[1].x, h[1].y, h[2].x, h[2].y, h[3].x, h[3].y};
+  lo = (xh8){l[0].x, l[0].y, l[1].x, l[1].y, l[2].x, l[2].y, l[3].x, l[3].y};
+}
+
+__device__ __forceinline__ xf16 mfma3(xh8 ah, xh8 al, xh8 bh, xh8 bl, xf16 c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+__device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
+                                          float (*lds)[kBlockRays], uint32_t lane, float m1) {
+  const InstScratch* sc = (const InstScratch*)I.scr;
+  const uint32_t nbz = (I.gnz + kMfmaBlk - 1u) / kMfmaBlk;
+  const uint32_t bx = w.pix0 / nbz, bz = w.pix0 - bx * nbz;
+  const uint32_t ix0 = bx * kMfmaBlk, iz0 = bz * kMfmaBlk;
+  const double kr = I.kr_hi;
+  // block centre relative to the centroid (world axes)
+  const double qa[3] = {(I.gx0 + (ix0 + 31.5) * I.gsx) - sc->cx, I.gy - sc->cy,
+                        (I.gz0 + (iz0 + 31.5) * I.gsz) - sc->cz};
+  const double Ra = sqrt(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2]);
+  const double iRa = 1.0 / Ra;
+  int ax[3];
+  kernel_axes(sc, ax);
+  const float amax = __uint_as_float(sc->amax_bits);
+  int ex = 0;
+  if (amax > 0.f) (void)frexpf(amax, &ex);
+  const float scale = ldexpf(1.f, -ex), unscale = ldexpf(1.f, ex);
+
+  const uint32_t half = lane >> 5;
+  const float u0 = (float)(lane & 31u) - 15.5f;  // offset from a tile's centre
+  xf16 acc[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[a][b][c][i] = 0.f;
+
+  const RayRec32* rays = I.rays32;  // one ray per lane: vector loads
+  const uint32_t r0 = w.chunk * I.chunk_len;
+  const uint32_t r1 = min(r0 + I.chunk_len, I.nr);
+  for (uint32_t b0 = r0; b0 < r1; b0 += kBlockRays) {
+    const uint32_t nb = min(kBlockRays, r1 - b0);
+    // per-(ray, block) values, one ray per lane, in FP64
+    {
+      float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f};
+      if (lane < nb) {
+        const RayRec32 rr = rays[b0 + lane];
+        const float ok[3] = {rr.ox, rr.oy, rr.oz};
+        const float dk[3] = {rr.dx, rr.dy, rr.oo};  // TILE: oo holds the third delta
+        double o[3], dl[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { o[ax[k]] = ok[k]; dl[ax[k]] = dk[k]; }
+        const double Bq = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) -
+                          2.0 * (o[0] * qa[0] + o[1] * qa[1] + o[2] * qa[2]);
+        const double path = sqrt(Ra * Ra + Bq);
+        const double ip = 1.0 / path, iden = 1.0 / (path + Ra);
+        // kr (|a - o| - |a|), the phase of the FP32 rounding delta of o and
+        // the ray's own offset
+        const double corr = kr * (dl[0] * (qa[0] - o[0]) + dl[1] * (qa[1] - o[1]) +
+                                  dl[2] * (qa[2] - o[2])) * ip;
+        const double g = -Bq * iden * ip * iRa;  // 1/|a - o| - 1/|a|
+        // d/dt of |a + t s - o| - |a + t s| = (a.s) g - (o.s) / |a - o|
+        const double sx = kr * I.gsx * (qa[0] * g - o[0] * ip);
+        const double sz = kr * I.gsz * (qa[2] * g - o[2] * ip);
+        double ph = kr * Bq * iden + ((double)rr.off + corr) - 16.0 * sx;
+        ph -= rint(ph);
+        double gz = -16.0 * sz;
+        gz -= rint(gz);
+        double tx = 32.0 * sx, tz = 32.0 * sz;
+        tx -= rint(tx);
+        tz -= rint(tz);
+        v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
+        v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
+        v[MR_AMP] = rr.cs * scale;
+        v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
+        v[MR_RZC] = __builtin_amdgcn_cosf((float)tz); v[MR_RZS] = __builtin_amdgcn_sinf((float)tz);
+      }
+      __syncthreads();  // the previous block's reads are done
+#pragma unroll
+      for (int k = 0; k < MR_N; ++k) lds[k][lane] = v[k];
+      __syncthreads();
+    }
+    const uint32_t nks = (nb + 15u) / 16u;
+    for (uint32_t ks = 0; ks < nks; ++ks) {
+      const uint32_t kb = ks * 16u + 8u * half;  // this lane's 8 rays of the step
+      float fr0[8], fi0[8], fr1[8], fi1[8], gr0[8], gi0[8], gr1[8], gi1[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float ph = lds[MR_PH][kb + j], sx = lds[MR_SX][kb + j];
+        const float gz = lds[MR_GZ][kb + j], sz = lds[MR_SZ][kb + j];
+        const float amp = lds[MR_AMP][kb + j];
+        const float rxc = lds[MR_RXC][kb + j], rxs = lds[MR_RXS][kb + j];
+        const float rzc = lds[MR_RZC][kb + j], rzs = lds[MR_RZS][kb + j];
+        const float a = fmaf(u0, sx, ph);
+        const float c = __builtin_amdgcn_cosf(a) * amp, s = __builtin_amdgcn_sinf(a) * amp;
+        fr0[j] = c; fi0[j] = s;
+        fr1[j] = fmaf(c, rxc, -(s * rxs));
+        fi1[j] = fmaf(c, rxs, s * rxc);
+        const float b = fmaf(u0, sz, gz);
+        const float gc = __builtin_amdgcn_cosf(b), gs = __builtin_amdgcn_sinf(b);
+        gr0[j] = gc; gi0[j] = gs;
+        gr1[j] = fmaf(gc, rzc, -(gs * rzs));
+        gi1[j] = fmaf(gc, rzs, gs * rzc);
+      }
+      xh8 arh[2], arl[2], aih[2], ail[2];
+      xh8 brh[2], brl[2], bih[2], bil[2], bnh[2], bnl[2];
+      split8(fr0, m1, arh[0], arl[0]); split8(fi0, m1, aih[0], ail[0]);
+      split8(fr1, m1, arh[1], arl[1]); split8(fi1, m1, aih[1], ail[1]);
+      split8(gr0, m1, brh[0], brl[0]); split8(gi0, m1, bih[0], bil[0]);
+      split8(gr1, m1, brh[1], brl[1]); split8(gi1, m1, bih[1], bil[1]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) { bnh[t] = -bih[t]; bnl[t] = -bil[t]; }
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) {
+          // re += Fr Gr - Fi Gi, im += Fr Gi + Fi Gr
+          acc[tr][tc][0] = mfma3(arh[tr], arl[tr], brh[tc], brl[tc], acc[tr][tc][0]);
+          acc[tr][tc][0] = mfma3(aih[tr], ail[tr], bnh[tc], bnl[tc], acc[tr][tc][0]);
+          acc[tr][tc][1] = mfma3(arh[tr], arl[tr], bih[tc], bil[tc], acc[tr][tc][1]);
+          acc[tr][tc][1] = mfma3(aih[tr], ail[tr], brh[tc], brl[tc], acc[tr][tc][1]);
+        }
+    }
+  }
+  // partial sums of the block's real pixels, in the grid's slot order
+  const uint32_t th = I.gth, tw = I.gtw, tsz = th * tw;
+  const uint32_t ntz = (I.gnz + tw - 1u) / tw;
+  float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * I.npad;
+  const uint32_t n = lane & 31u;
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t m = 8u * (i >> 2) + 4u * half + (i & 3);
+        const uint32_t ix = ix0 + 32u * tr + m, iz = iz0 + 32u * tc + n;
+        if (ix < I.gnx && iz < I.gnz) {
+          const uint32_t slot = ((ix / th) * ntz + iz / tw) * tsz + (ix % th) * tw + iz % tw;
+          part[slot] = make_float4(acc[tr][tc][0][i] * unscale, acc[tr][tc][1][i] * unscale, 0.f,
+                                   0.f);
+        }
+      }
+}
+
+// Persistent: one wave per group takes items from the queue until it runs
+// dry; items of instances without the MFMA flag are skipped.
+__global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ insts,
+                                                  const WorkItem* __restrict__ items,
+                                                  uint32_t n_items, uint32_t* __restrict__ queue,
+                                                  float m1) {
+  __shared__ float lds[MR_N][kBlockRays];
+  const uint32_t lane = threadIdx.x;
+  for (;;) {
+    const uint32_t v = lane == 0u ? atomicAdd(queue, 1u) : 0u;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(v);
+    if (it >= n_items) break;
+    const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
+    const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
+    const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
+    if (f & INST_F_MFMA) mfma_item(I, w, lds, lane, m1);
   }
 }
 
@@ -1538,8 +1811,9 @@ hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx,
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
                        const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
                        uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
-                       uint32_t static_flags, int P, RayRec32* rays32, PixRec32* pix32,
-                       TileRec* tiles, double* R64, hipStream_t st) {
+                       uint32_t static_flags, int P, double gsx, double gsz,
+                       RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
+                       hipStream_t st) {
   const unsigned nb = nr ? std::min<unsigned>(kRedBlocks, cdiv(nr, 256)) : 1u;
   hipLaunchKernelGGL(k_centroid_partial, dim3(nb), dim3(256), 0, st, rays, nr, part);
   hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(256), 0, st, part, nb, nr, rays, scr);
@@ -1566,7 +1840,7 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
     }
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, st, scr, skip, kr, mode, force_flags,
-                     static_flags, (uint32_t)P);
+                     static_flags, (uint32_t)P, gsx, gsz);
   if (mode == MODE_F32R && nr)
     hipLaunchKernelGGL(k_ray_prep32, dim3(cdiv(nr, 256)), dim3(256), 0, st, rays, nr, scr, kr,
                        kr_lo, npad ? 1u : 0u, rays32);
@@ -1625,6 +1899,17 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 #undef XRTK_F32
 #undef XRTK_F64
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
+                       uint32_t* queue, uint32_t grid, hipStream_t st) {
+  if (!n_items) return hipSuccess;
+  for (uint32_t f = g_fail_launches.load(); f > 0;)
+    if (g_fail_launches.compare_exchange_weak(f, f - 1)) return hipErrorLaunchFailure;
+  hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 1u);
+  hipLaunchKernelGGL(k_pair_mfma, dim3(std::max<uint32_t>(1u, grid)), dim3(64), 0, st, insts,
+                     items, n_items, queue, -1.0f);
+  return hipGetLastError();
 }
 
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c) {

@@ -110,7 +110,7 @@ inline size_t tile_table_bytes(uint32_t npad, uint32_t P) {
 // Per-instance values computed on the device (centroid, bounds, flags).
 struct InstScratch {
   double cx, cy, cz;       // FP64 ray centroid
-  double pad0;
+  double blk_rev;          // MFMA: bound of the block phase model's remainder [rev]
   uint32_t omax_bits[4];   // max |o_x|, |o_y|, |o_z|, |o| as float bits
   uint32_t qmax_bits[3];   // max |q_x|, |q_y|, |q_z|
   uint32_t dqmax_bits[3];  // max |dq_x|, |dq_y|, |dq_z| (slot to tile reference)
@@ -119,7 +119,7 @@ struct InstScratch {
   uint32_t flags;          // INST_F_* decided by the plan kernel
   uint32_t ep_nonzero;     // 1 if any ray has Ep != 0
   uint32_t n_differs;      // 1 if some ray's normal differs from ray 0's
-  uint32_t pad1;
+  uint32_t amax_bits;      // max |cs| of the FP32 ray records (float bits)
   double n0x, n0y, n0z;    // normal of ray 0
   double v0_rev, v2_rev;   // bounds of the reduced phase of the two variants
   uint32_t qlo_key[3];     // min q_x, q_y, q_z (order-preserving float keys)
@@ -152,6 +152,11 @@ enum : uint32_t {
                           // to first order (pixel lists off a line by rounding)
   INST_F_EPZERO = 2048u,  // F32R: every ray has Ep == 0, so the Ep sums are exact
                           // zeros and the pair kernel skips them
+  INST_F_MFMA = 4096u,    // F32R grid screen: blocks of 64 x 64 pixels run on the
+                          // matrix cores (phase linear over a block, bound <= 1e-7
+                          // rev); the VALU pair kernels skip the instance
+  INST_F_MFMA_OK = 8192u, // static flag from the host: this launch runs the MFMA
+                          // items of the instance (a grid screen)
 };
 
 enum : uint32_t { MODE_F32R = 0, MODE_F64 = 1 };
@@ -178,6 +183,10 @@ struct KInst {
   double kr_hi, kr_lo;      // k / (2 pi) as a double-double [rev / mm]
   double skip;              // skip pairs with path < skip [mm]
   double pad3;
+  // prepare_wave grid of the screen (MFMA items): pixel (ix, iz) at
+  // (gx0 + ix gsx, gy, gz0 + iz gsz); slots in tiles of gth x gtw pixels
+  double gx0, gy, gz0, gsx, gsz, pad4;
+  uint32_t gnx, gnz, gth, gtw;
 };
 static_assert(sizeof(KInst) % 16 == 0, "KInst size");
 

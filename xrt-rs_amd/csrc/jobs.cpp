@@ -120,6 +120,7 @@ int build_job_specs(xrtk_ctx* ctx, uint32_t want, const LaunchCfg& c32, const La
     s.skip = 1e-30;
     s.static_flags = rs.has_path ? INST_F_PATHUP : 0u;
     s.out = outbuf.as<double>() + 4 * out_off[i];
+    if (s.np) spec_set_grid(s, j.screen, th, tw);
     specs.push_back(s);
   }
   return XRTK_OK;
@@ -157,10 +158,13 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       return XRTK_EINVAL;
     const char* ng = std::getenv("XRTK_NO_GRAPH");
     const bool use_graph = !(ng && *ng == '1');
+    // XRTK_ENGINE_VALU keeps every instance off the matrix cores
+    const uint32_t force = opts && opts->engine == XRTK_ENGINE_VALU ? 0x20000u : 0u;
     // graph key: everything the captured nodes bake in
     std::vector<uint64_t> key;
     key.push_back(n_jobs);
     key.push_back(want);
+    key.push_back(force);
     for (const LaunchCfg* c : {&c32, &c64})
       key.push_back(((uint64_t)c->P << 32) | ((uint64_t)c->W << 16) |
                     ((uint64_t)c->ray_src << 8) | (uint64_t)c->persistent | ((uint64_t)c->tile << 40));
@@ -201,7 +205,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       // first run: eager, with the metadata upload; then capture the kernels
       rc = enqueue_job_grids(ctx, c32, jobs, specs);
       if (rc == XRTK_OK)
-        rc = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, true, &ne->plan, 0, 0, true);
+        rc = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, true, &ne->plan, 0, force,
+                           true);
       if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
       if (rc != XRTK_OK) { drop(ne); return rc; }
       if (use_graph) {
@@ -211,8 +216,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
         if (e == hipSuccess) {
           int rc2 = enqueue_job_grids(ctx, c32, jobs, specs);
           if (rc2 == XRTK_OK)
-            rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, false, nullptr, 0, 0,
-                                true, /*upload=*/false);
+            rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, false, nullptr, 0,
+                                force, true, /*upload=*/false);
           e = hipStreamEndCapture(ctx->stream, &graph);
           if (rc2 == XRTK_OK && e == hipSuccess && graph) {
             e = hipGraphInstantiate(&ne->exec, graph, nullptr, nullptr, 0);
@@ -240,8 +245,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       ph.next("xrtk.jobs.replay");
       int rc = enqueue_job_grids(ctx, c32, ge->jobs.data(), ge->specs);
       if (rc == XRTK_OK)
-        rc = run_instances(ctx, c32, c64, ge->specs, ge->work, ge->meta, true, nullptr, 0, 0,
-                           true, /*upload=*/false);
+        rc = run_instances(ctx, c32, c64, ge->specs, ge->work, ge->meta, true, nullptr, 0,
+                           force, true, /*upload=*/false);
       if (rc != XRTK_OK) return rc;
       e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return map_err(e);
@@ -271,7 +276,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       q.mode = all64 ? XRTK_MODE_F64 : XRTK_MODE_F32R;
       q.pixels_per_lane = c.P; q.waves_per_group = c.W; q.tile_rays = c.tile; q.split = 0;
       q.ray_src = c.ray_src; q.persistent = c.persistent;
-      q.n_items = (int32_t)(p.n_items32 + p.n_items64); q.n_fp64 = (int32_t)p.n_f64;
+      q.n_items = (int32_t)(p.n_items32 + p.n_items64 + p.n_items_mfma);
+      q.n_fp64 = (int32_t)p.n_f64;
       q.backend = 1;
       q.inst_flags = 0;
       q.reserved = 0;
@@ -286,6 +292,12 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       if (e != hipSuccess) return map_err(e);
     }
     e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && ge->plan.scr0) {
+      // the variant flags the device planner chose for the first job
+      uint32_t fl = 0;
+      if (hipMemcpy(&fl, &ge->plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) == hipSuccess)
+        ctx->last_plan.inst_flags = (int32_t)fl;
+    }
     return map_err(e);
   } catch (...) {
     return XRTK_ENOMEM;

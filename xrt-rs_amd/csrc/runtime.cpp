@@ -223,6 +223,23 @@ static void kr_dd(double k, double* hi, double* lo) {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+void spec_set_grid(InstSpec& s, const XrtkWaveParams& w, uint32_t th, uint32_t tw) {
+  s.gnx = w.nx; s.gnz = w.nz; s.gth = th; s.gtw = tw;
+  s.gx0 = w.nx > 1 ? w.center[0] - w.dx : w.center[0];
+  s.gz0 = w.nz > 1 ? w.center[2] - w.dz : w.center[2];
+  s.gsx = w.nx > 1 ? 2.0 * w.dx / (double)(w.nx - 1) : 0.0;
+  s.gsz = w.nz > 1 ? 2.0 * w.dz / (double)(w.nz - 1) : 0.0;
+  s.gy = w.center[1];
+}
+
+// Whether an instance may run on the matrix cores: a reduced-kernel instance
+// with rays on a grid screen, unless force flag 0x20000 (XRTK_ENGINE_VALU)
+// forbids it.  The planner kernel has the last word.
+bool mfma_allowed(const InstSpec& s, uint32_t force_flags) {
+  return !(force_flags & 0x20000u) && s.mode == MODE_F32R && s.nr > 0 && s.gnx > 0 &&
+         s.gnz > 0 && s.gth > 0 && s.gtw > 0;
+}
+
 int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
                   Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
@@ -266,7 +283,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
 
   // ---- instance table and work items
   std::vector<KInst> insts(n);
-  std::vector<WorkItem> items32, items64;
+  std::vector<WorkItem> items32, items64, items_mfma;
   uint32_t n_f64 = 0;
   for (size_t i = 0; i < n; ++i) {
     const InstSpec& s = specs[i];
@@ -292,11 +309,20 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     I.k = s.energy / kCHBAR_H * 1e7;
     kr_dd(I.k, &I.kr_hi, &I.kr_lo);
     I.skip = s.skip;
+    I.gx0 = s.gx0; I.gy = s.gy; I.gz0 = s.gz0; I.gsx = s.gsx; I.gsz = s.gsz;
+    I.gnx = s.gnx; I.gnz = s.gnz; I.gth = s.gth; I.gtw = s.gtw;
     if (!f32) ++n_f64;
     const uint32_t B = f32 ? 64u * cfg32.W * cfg32.P : 64u * cfg64.W * cfg64.P;
     std::vector<WorkItem>& items = f32 ? items32 : items64;
     for (uint32_t c = 0; c < L.n_chunks; ++c)
       for (uint32_t p0 = 0; p0 < s.npad; p0 += B) items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
+    // a grid screen may run on the matrix cores: its block items (the
+    // planner decides on the device; the other kernels skip what it takes)
+    if (mfma_allowed(s, force_flags)) {
+      const uint32_t nb = ((s.gnx + 63u) / 64u) * ((s.gnz + 63u) / 64u);
+      for (uint32_t c = 0; c < L.n_chunks; ++c)
+        for (uint32_t b = 0; b < nb; ++b) items_mfma.push_back(WorkItem{(uint32_t)i, b, c, 0});
+    }
   }
   // Batched launches: longest items first so the queue drains evenly.
   if (sorted && n > 1) {
@@ -309,6 +335,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     };
     std::stable_sort(items32.begin(), items32.end(), by_len);
     std::stable_sort(items64.begin(), items64.end(), by_len);
+    std::stable_sort(items_mfma.begin(), items_mfma.end(), by_len);
   }
   // merge lists: instances writing their own output merge in one launch;
   // instances that accumulate into an earlier output follow one by one.
@@ -323,15 +350,17 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   const size_t a_ins = align_up(sizeof(KInst) * std::max<size_t>(n, 1), 256);
   const size_t a_i32 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items32.size()), 256);
   const size_t a_i64 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items64.size()), 256);
+  const size_t a_imf = align_up(sizeof(WorkItem) * std::max<size_t>(1, items_mfma.size()), 256);
   const size_t a_lst = align_up(sizeof(uint32_t) * (lists.size() + 1), 256);
-  e = meta.ensure(a_ins + a_i32 + a_i64 + a_lst + 256);
+  e = meta.ensure(a_ins + a_i32 + a_i64 + a_imf + a_lst + 256);
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* M = meta.as<char>();
   KInst* d_insts = reinterpret_cast<KInst*>(M);
   WorkItem* d_items32 = reinterpret_cast<WorkItem*>(M + a_ins);
   WorkItem* d_items64 = reinterpret_cast<WorkItem*>(M + a_ins + a_i32);
-  uint32_t* d_lists = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64);
-  uint32_t* d_queue = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_lst);
+  WorkItem* d_items_mfma = reinterpret_cast<WorkItem*>(M + a_ins + a_i32 + a_i64);
+  uint32_t* d_lists = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_imf);
+  uint32_t* d_queue = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_imf + a_lst);
   if (upload) {
     if (n) e = hipMemcpyAsync(d_insts, insts.data(), sizeof(KInst) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !items32.empty())
@@ -339,6 +368,9 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                          hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !items64.empty())
       e = hipMemcpyAsync(d_items64, items64.data(), sizeof(WorkItem) * items64.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items_mfma.empty())
+      e = hipMemcpyAsync(d_items_mfma, items_mfma.data(), sizeof(WorkItem) * items_mfma.size(),
                          hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !lists.empty())
       e = hipMemcpyAsync(d_lists, lists.data(), sizeof(uint32_t) * lists.size(),
@@ -364,7 +396,8 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     e = launch_prep(s.rays, s.nr, s.pix, s.npad, s.perm, s.np,
                     reinterpret_cast<InstScratch*>(W + L.scr),
                     reinterpret_cast<double*>(W + L.part), s.mode, insts[i].kr_hi,
-                    insts[i].kr_lo, s.skip, force_flags, s.static_flags, cfg32.P,
+                    insts[i].kr_lo, s.skip, force_flags,
+                    s.static_flags | (mfma_allowed(s, force_flags) ? INST_F_MFMA_OK : 0u), cfg32.P, s.gsx, s.gsz,
                     reinterpret_cast<RayRec32*>(W + L.rays32),
                     reinterpret_cast<PixRec32*>(W + L.pix32),
                     reinterpret_cast<TileRec*>(W + L.tiles), reinterpret_cast<double*>(W + L.R64),
@@ -386,6 +419,12 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     const uint32_t remap = (!cfg32.persistent && n == 1) ? 1u : 0u;
     e = launch_pair(MODE_F32R, cfg32, d_insts, d_items32, (uint32_t)items32.size(), d_queue,
                     grid_of(cfg32, MODE_F32R, items32.size()), remap, st);
+    if (e != hipSuccess) return map_err(e);
+  }
+  if (!items_mfma.empty()) {
+    // one wave per group, up to two groups per SIMD resident
+    const uint32_t g = std::min<uint32_t>((uint32_t)items_mfma.size(), (uint32_t)ctx->n_cu * 8u);
+    e = launch_mfma(d_insts, d_items_mfma, (uint32_t)items_mfma.size(), d_queue + 32, g, st);
     if (e != hipSuccess) return map_err(e);
   }
   if (!items64.empty()) {
@@ -411,6 +450,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     plan_out->cfg64 = cfg64;
     plan_out->n_items32 = (uint32_t)items32.size();
     plan_out->n_items64 = (uint32_t)items64.size();
+    plan_out->n_items_mfma = (uint32_t)items_mfma.size();
     plan_out->n_f64 = n_f64;
     plan_out->n_inst = (uint32_t)n;
     plan_out->scr0 = n ? reinterpret_cast<const InstScratch*>(W + lay[0].scr) : nullptr;

@@ -41,12 +41,17 @@ hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx,
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
                        const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
                        uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
-                       uint32_t static_flags, int P, RayRec32* rays32, PixRec32* pix32,
-                       TileRec* tiles, double* R64, hipStream_t st);
+                       uint32_t static_flags, int P, double gsx, double gsz, RayRec32* rays32,
+                       PixRec32* pix32, TileRec* tiles, double* R64, hipStream_t st);
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
                        uint32_t remap, hipStream_t st);
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c);
+// Block items (inst, block, chunk) of grid instances on the matrix cores,
+// persistent groups of one wave on queue[0]; items of instances the planner
+// did not give the MFMA flag are skipped.
+hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
+                       uint32_t* queue, uint32_t grid, hipStream_t st);
 // Pair-kernel launches still to be failed on purpose (xrtk_debug_fail_launches):
 // launch_pair takes one and returns hipErrorLaunchFailure before enqueuing
 // anything.
@@ -133,11 +138,23 @@ struct InstSpec {
   uint32_t static_flags = 0;
   double* out = nullptr;   // np * 4 doubles
   uint32_t accumulate = 0; // add into out (energy groups after the first)
+  // a prepare_wave grid screen laid out in tiles of gth x gtw pixels (gnx > 0):
+  // such an instance may run on the matrix cores
+  double gx0 = 0, gy = 0, gz0 = 0, gsx = 0, gsz = 0;
+  uint32_t gnx = 0, gnz = 0, gth = 0, gtw = 0;
 };
+// Records the grid of a prepare_wave screen (src/waves/prepare_wave.rs:40-68:
+// x = cx + (ix/(nx-1) - 1/2) 2 dx, likewise z; a dimension of 1 sits at the
+// centre) in an instance spec.
+void spec_set_grid(InstSpec& s, const XrtkWaveParams& w, uint32_t th, uint32_t tw);
+// Whether an instance may run on the matrix cores: a reduced-kernel instance
+// with rays on a grid screen, unless force flag 0x20000 (XRTK_ENGINE_VALU)
+// forbids it.  The planner kernel decides from the instance's bounds.
+bool mfma_allowed(const InstSpec& s, uint32_t force_flags);
 
 struct Plan {
   LaunchCfg cfg32, cfg64;
-  uint32_t n_items32 = 0, n_items64 = 0;
+  uint32_t n_items32 = 0, n_items64 = 0, n_items_mfma = 0;
   uint32_t n_f64 = 0;     // instances on the FP64 kernel
   uint32_t n_inst = 0;
   const InstScratch* scr0 = nullptr;  // device scratch of the first instance

@@ -18,6 +18,10 @@
 //      whole number of rounds of full-size items over the resident
 //      workgroups, and its items run longest first, so the workgroups end
 //      the launch together,
+//      A request on a grid screen learns after its prep whether the device
+//      planner put it on the matrix cores (an async copy of its flags); its
+//      items are the MFMA kernel's 64 x 64-pixel blocks then, and it joins
+//      the quanta once the flags have landed,
 //   3. enqueues, for each request whose last items went into that launch,
 //      its FP64 merge and an async copy of the result into pinned memory,
 //   4. retires requests whose copy has completed (result handed to the
@@ -126,6 +130,13 @@ struct Req {
   KInst* d_kinst = nullptr;
   uint32_t* d_list = nullptr;
   hipEvent_t done_ev = nullptr;
+  // routing: a request that may run on the matrix cores waits for its
+  // planner flags (copied into hblk, recorded by prep_ev) before its items
+  // are fixed
+  hipEvent_t prep_ev = nullptr;
+  bool routed = true;
+  uint32_t mblocks = 0;  // 64 x 64-pixel blocks of the screen (MFMA items)
+  bool mfma = false;
   std::shared_ptr<std::atomic<int>> users;  // pending requests on the ray set
 };
 
@@ -138,7 +149,7 @@ struct Launch {
   void* hmeta = nullptr; // pinned host staging of the same bytes
   size_t meta_size = 0;
   std::vector<KInst> h_insts;
-  std::vector<WorkItem> h_items32, h_items64;
+  std::vector<WorkItem> h_items32, h_items64, h_items_mf;
 };
 
 }  // namespace
@@ -158,6 +169,10 @@ struct xrtk_server {
   // the configuration and quantum_ms, never on the other requests.
   uint32_t cap32 = 0, cap64 = 0;
   hipStream_t stream = nullptr;
+  // grid and prep of requests that may run on the matrix cores: kept off the
+  // quantum stream so their planner flags land while a quantum runs (the
+  // quanta that take their items are enqueued after the flags are seen)
+  hipStream_t pstream = nullptr;
   std::thread th;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -222,6 +237,9 @@ struct xrtk_server {
     r->pblocks = s.npad / B;
     r->n_items = (uint64_t)r->n_chunks * r->pblocks;
     r->item_pairs = (double)B * r->chunk_len;
+    if (np) spec_set_grid(s, j.screen, th, tw);
+    r->mblocks = ((j.screen.nx + 63u) / 64u) * ((j.screen.nz + 63u) / 64u);
+    r->routed = !mfma_allowed(s, 0u);
     // device block: pix64, perm, rays32, pix32, tiles, R64, partial, scr,
     // part, out, KInst, list
     const bool f32 = s.mode == MODE_F32R;
@@ -240,6 +258,7 @@ struct xrtk_server {
   // Carves the block, fills the KInst, enqueues grid + prep.
   int start_request(Req* r) {
     Range range("xrtk.server.start");  // slots, upload and prep of one request
+    hipStream_t st = r->routed ? stream : pstream;
     char* p = static_cast<char*>(r->dblk);
     InstSpec& s = r->spec;
     const bool f32 = s.mode == MODE_F32R;
@@ -268,6 +287,8 @@ struct xrtk_server {
     I.tile_slots = 64u * c32.P;
     I.energy = s.energy;
     I.k = s.energy / kCHBAR_H * 1e7;
+    I.gx0 = s.gx0; I.gy = s.gy; I.gz0 = s.gz0; I.gsx = s.gsx; I.gsz = s.gsz;
+    I.gnx = s.gnx; I.gnz = s.gnz; I.gth = s.gth; I.gtw = s.gtw;
     {
       const long double two_pi = 6.283185307179586476925286766559005768L;
       const long double kr = (long double)I.k / two_pi;
@@ -283,16 +304,22 @@ struct xrtk_server {
     char* hs = static_cast<char*>(r->hblk);
     std::memcpy(hs, &I, sizeof(KInst));
     std::memset(hs + sizeof(KInst), 0, 64);
-    hipError_t e = hipMemcpyAsync(r->d_kinst, hs, sizeof(KInst), hipMemcpyHostToDevice, stream);
+    hipError_t e = hipMemcpyAsync(r->d_kinst, hs, sizeof(KInst), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(r->d_list, hs + sizeof(KInst), sizeof(uint32_t), hipMemcpyHostToDevice,
-                         stream);
+                         st);
     if (e == hipSuccess && s.npad)
-      e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, s.npad, pix, perm, stream);
+      e = launch_pix_grid(w.center, w.dx, w.dz, w.nx, w.nz, th, tw, s.npad, pix, perm, st);
     if (e == hipSuccess && s.nr)
       e = launch_prep(s.rays, s.nr, pix, s.npad, perm, s.np, scr, part, s.mode, I.kr_hi, I.kr_lo,
-                      s.skip, 0, s.static_flags, c32.P, rays32, pix32, tiles, R64, stream);
-    if (e == hipSuccess && !s.nr && s.np) e = launch_zero_f64(out, (size_t)s.np * 4, stream);
+                      s.skip, 0, s.static_flags | (r->routed ? 0u : INST_F_MFMA_OK), c32.P,
+                      s.gsx, s.gsz, rays32, pix32, tiles, R64, st);
+    if (e == hipSuccess && !r->routed) {
+      // the planner's flags, for the routing (hblk takes the result later)
+      e = hipMemcpyAsync(r->hblk, &scr->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipEventRecord(r->prep_ev, st);
+    }
+    if (e == hipSuccess && !s.nr && s.np) e = launch_zero_f64(out, (size_t)s.np * 4, st);
     return map_err(e);
   }
 
@@ -319,6 +346,7 @@ struct xrtk_server {
       r->rc = start_request(r);
       if (r->rc != XRTK_OK) {
         (void)hipStreamSynchronize(stream);
+        (void)hipStreamSynchronize(pstream);
         r->state = FAILED;
         finish(r);
         continue;
@@ -349,6 +377,30 @@ struct xrtk_server {
     return std::max<uint32_t>(1, (uint32_t)ctx->n_cu * (mode == MODE_F32R ? 32u : 16u) / c.W);
   }
 
+  // Fixes the items of a request once its planner flags have landed: the
+  // MFMA kernel's blocks when the planner put it on the matrix cores, else
+  // the VALU kernel's tiles as planned.
+  void route(Req* r) {
+    const hipError_t q = hipEventQuery(r->prep_ev);
+    if (q == hipErrorNotReady) return;
+    r->routed = true;
+    if (q != hipSuccess) {
+      // the prep or the copy failed: the request fails without items
+      (void)hipGetLastError();
+      r->rc = XRTK_ELAUNCH;
+      r->next_item = r->n_items;
+      enqueue_merge(r);
+      return;
+    }
+    uint32_t fl = 0;
+    std::memcpy(&fl, r->hblk, sizeof(fl));
+    if (fl & INST_F_MFMA) {
+      r->mfma = true;
+      r->n_items = (uint64_t)r->n_chunks * r->mblocks;
+      r->item_pairs = 64.0 * 64.0 * r->chunk_len;
+    }
+  }
+
   // Fills and enqueues one quantum.  Returns false when there was no work.
   bool launch_quantum() {
     Range range("xrtk.server.quantum");  // work items and pair launches of one quantum
@@ -368,8 +420,10 @@ struct xrtk_server {
     std::vector<Req*> finishing;
     // requests with work left, fewest remaining pairs first
     std::vector<Req*> order;
-    for (Req* r : active)
-      if (r->state == ACTIVE && r->next_item < r->n_items) order.push_back(r);
+    for (Req* r : active) {
+      if (r->state == ACTIVE && !r->routed) route(r);
+      if (r->state == ACTIVE && r->routed && r->next_item < r->n_items) order.push_back(r);
+    }
     auto left = [](const Req* r) { return (double)(r->n_items - r->next_item) * r->item_pairs; };
     if (cfg.policy == XRTK_SCHED_SHORTEST)
       std::stable_sort(order.begin(), order.end(),
@@ -378,23 +432,25 @@ struct xrtk_server {
     struct Undo { Req* r; uint64_t next_item; double t2; };
     std::vector<Undo> undo;
     // items with their pair counts, ordered longest first below
-    std::vector<std::pair<double, WorkItem>> it32, it64;
+    std::vector<std::pair<double, WorkItem>> it32, it64, itmf;
     // take up to `max_items` items of r (0: until `limit` pairs are in)
     auto take = [&](Req* r, uint64_t max_items, double limit) {
       undo.push_back(Undo{r, r->next_item, r->t[2]});
       const uint32_t li = (uint32_t)L.h_insts.size();
       L.h_insts.push_back(r->kinst);
-      const uint32_t B = 64u * r->cfg.W * r->cfg.P;
-      auto& items = r->spec.mode == MODE_F32R ? it32 : it64;
+      const uint32_t B = r->mfma ? 64u * 64u : 64u * r->cfg.W * r->cfg.P;
+      const uint32_t nblk = r->mfma ? r->mblocks : r->pblocks;
+      auto& items = r->mfma ? itmf : (r->spec.mode == MODE_F32R ? it32 : it64);
       uint64_t taken = 0;
       while (r->next_item < r->n_items &&
              (max_items ? taken < max_items : (pairs < limit || taken == 0))) {
         const uint64_t it = r->next_item++;
-        const uint32_t chunk = (uint32_t)(it / r->pblocks);
-        const uint32_t pb = (uint32_t)(it % r->pblocks);
+        const uint32_t chunk = (uint32_t)(it / nblk);
+        const uint32_t pb = (uint32_t)(it % nblk);
         const double len =
             std::min<double>(r->chunk_len, (double)r->spec.nr - (double)chunk * r->chunk_len);
-        items.emplace_back((double)B * len, WorkItem{li, pb * B, chunk, 0});
+        // VALU items start at a slot, MFMA items name a block
+        items.emplace_back((double)B * len, WorkItem{li, r->mfma ? pb : pb * B, chunk, 0});
         pairs += (double)B * len;
         ++taken;
       }
@@ -421,13 +477,16 @@ struct xrtk_server {
       out.reserve(v.size());
       for (const auto& x : v) out.push_back(x.second);
     };
+    L.h_items_mf.clear();
     lpt(it32, L.h_items32);
     lpt(it64, L.h_items64);
+    lpt(itmf, L.h_items_mf);
     // metadata
     const size_t a_ins = au(sizeof(KInst) * L.h_insts.size());
     const size_t a_32 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items32.size()));
     const size_t a_64 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items64.size()));
-    const size_t need = a_ins + a_32 + a_64 + 256;
+    const size_t a_mf = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items_mf.size()));
+    const size_t need = a_ins + a_32 + a_64 + a_mf + 256;
     if (need > L.meta_size) {
       if (L.meta) (void)hipFree(L.meta);
       if (L.hmeta) (void)hipHostFree(L.hmeta);
@@ -450,11 +509,15 @@ struct xrtk_server {
     std::memcpy(H, L.h_insts.data(), sizeof(KInst) * L.h_insts.size());
     std::memcpy(H + a_ins, L.h_items32.data(), sizeof(WorkItem) * L.h_items32.size());
     std::memcpy(H + a_ins + a_32, L.h_items64.data(), sizeof(WorkItem) * L.h_items64.size());
+    std::memcpy(H + a_ins + a_32 + a_64, L.h_items_mf.data(),
+                sizeof(WorkItem) * L.h_items_mf.size());
     KInst* d_ins = reinterpret_cast<KInst*>(M);
     WorkItem* d_32 = reinterpret_cast<WorkItem*>(M + a_ins);
     WorkItem* d_64 = reinterpret_cast<WorkItem*>(M + a_ins + a_32);
-    uint32_t* d_q = reinterpret_cast<uint32_t*>(M + a_ins + a_32 + a_64);
-    hipError_t e = hipMemcpyAsync(M, H, a_ins + a_32 + a_64, hipMemcpyHostToDevice, stream);
+    WorkItem* d_mf = reinterpret_cast<WorkItem*>(M + a_ins + a_32 + a_64);
+    uint32_t* d_q = reinterpret_cast<uint32_t*>(M + a_ins + a_32 + a_64 + a_mf);
+    hipError_t e =
+        hipMemcpyAsync(M, H, a_ins + a_32 + a_64 + a_mf, hipMemcpyHostToDevice, stream);
     (void)hipEventRecord(L.start, stream);
     // persistent groups pull items from the queue; otherwise one group per item
     auto grid_of = [&](const LaunchCfg& c, uint32_t mode, size_t n) {
@@ -464,6 +527,10 @@ struct xrtk_server {
     if (e == hipSuccess && !L.h_items32.empty())
       e = launch_pair(MODE_F32R, c32, d_ins, d_32, (uint32_t)L.h_items32.size(), d_q,
                       grid_of(c32, MODE_F32R, L.h_items32.size()), 0, stream);
+    if (e == hipSuccess && !L.h_items_mf.empty())
+      e = launch_mfma(d_ins, d_mf, (uint32_t)L.h_items_mf.size(), d_q + 32,
+                      std::min<uint32_t>((uint32_t)L.h_items_mf.size(), (uint32_t)ctx->n_cu * 8u),
+                      stream);
     if (e == hipSuccess && !L.h_items64.empty())
       e = launch_pair(MODE_F64, c64, d_ins, d_64, (uint32_t)L.h_items64.size(), d_q + 16,
                       grid_of(c64, MODE_F64, L.h_items64.size()), 0, stream);
@@ -565,7 +632,7 @@ struct xrtk_server {
       if (stop && queue.empty() && active.empty()) break;
       if (!launched) {
         bool merging = false;
-        for (Req* r : active) merging |= r->state == MERGING;
+        for (Req* r : active) merging |= r->state == MERGING || !r->routed;
         if (merging || !queue.empty()) {
           lk.unlock();
           std::this_thread::sleep_for(std::chrono::microseconds(100));
@@ -620,6 +687,10 @@ int xrtk_server_create(xrtk_ctx* ctx, const XrtkServerCfg* cfg, xrtk_server** ou
     if (hipSetDevice(ctx->device) != hipSuccess) return XRTK_ENODEV;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
       return XRTK_ELAUNCH;
+    if (hipStreamCreateWithFlags(&s->pstream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipStreamDestroy(s->stream);
+      return XRTK_ELAUNCH;
+    }
     for (Launch& L : s->launches) {
       if (hipEventCreate(&L.start) != hipSuccess || hipEventCreate(&L.end) != hipSuccess)
         return XRTK_ELAUNCH;
@@ -666,6 +737,11 @@ int xrtk_server_submit(xrtk_server* s, const XrtkJob* job, xrtk_ticket* ticket) 
     }
     if (hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming) != hipSuccess)
       return XRTK_ELAUNCH;
+    if (hipEventCreateWithFlags(&r->prep_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipEventDestroy(r->done_ev);
+      r->done_ev = nullptr;
+      return XRTK_ELAUNCH;
+    }
     hold.r = nullptr;  // kept by the request until it is collected
     r->id = s->next_id++;
     *ticket = r->id;
@@ -710,6 +786,7 @@ int xrtk_server_wait(xrtk_server* s, xrtk_ticket t, double timeout_ms, double t_
   const int rc = r->state == DONE ? XRTK_OK : (r->rc ? r->rc : XRTK_ELAUNCH);
   if (r->users) r->users->fetch_sub(1);  // the ray set is held until collected
   (void)hipEventDestroy(r->done_ev);
+  (void)hipEventDestroy(r->prep_ev);
   s->all.erase(it);
   delete r;
   return rc;
@@ -734,9 +811,11 @@ void xrtk_server_destroy(xrtk_server* s) {
   if (s->th.joinable()) s->th.join();
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->pstream);
   for (auto& kv : s->all) {
     if (kv.second->users) kv.second->users->fetch_sub(1);
     if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
+    if (kv.second->prep_ev) (void)hipEventDestroy(kv.second->prep_ev);
     delete kv.second;
   }
   s->dpool.release_all();
@@ -748,6 +827,7 @@ void xrtk_server_destroy(xrtk_server* s) {
     if (L.end) (void)hipEventDestroy(L.end);
   }
   (void)hipStreamDestroy(s->stream);
+  (void)hipStreamDestroy(s->pstream);
   delete s;
 }
 

@@ -67,7 +67,7 @@ class XrtkWaveParams(C.Structure):
 class XrtkOpts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "mode", "pixels_per_lane", "waves_per_group", "tile_rays", "split", "ray_src",
-        "persistent", "reserved")]
+        "persistent", "engine")]
 
 
 class XrtkJob(C.Structure):
