@@ -420,7 +420,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   if (const char* f = std::getenv("XRTK_FORCE_FLAGS")) force = (uint32_t)std::strtoul(f, nullptr, 0);
   if (opts && opts->engine == XRTK_ENGINE_VALU) force |= 0x20000u;
   int rc = run_instances(ctx, c32, c64, specs, ws.work, ws.meta, true, &plan, split, force,
-                         false);
+                         false, true, nullptr, /*route_sync=*/true);
   if (rc != XRTK_OK) return rc;
   ph.next("xrtk.readback");
   e = hipMemcpyAsync(out4, ws.outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,

@@ -1395,19 +1395,68 @@ enum : int {
   MR_N
 };
 
-// hi = RNE(x), lo = RNE(x - hi) of 8 values, packed as an MFMA operand.
-// m1 is -1 held in a register: fma(hi, m1, x) is then one v_fma_mix_f32.
-__device__ __forceinline__ void split8(const float (&x)[8], float m1, xh8& hi, xh8& lo) {
-  xh2 h[4], l[4];
+// hi = RNE(x), lo = RNE(x - hi) of two values, packed in one register each;
+// m1 is -1 held in a register, so fma(hi, m1, x) rounded to FP16 is one
+// v_fma_mixlo/hi_f16.
+__device__ __forceinline__ void split2(float a, float b, float m1, uint32_t& hi, uint32_t& lo) {
+  const xf2 p = {a, b};
+  const xh2 h = __builtin_convertvector(p, xh2);
+  const xh2 l = {(_Float16)__builtin_fmaf((float)h.x, m1, a),
+                 (_Float16)__builtin_fmaf((float)h.y, m1, b)};
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// The MFMA operands of one 16-ray step: row factors (A, 2 row tiles) as
+// re, im and -im, column factors (B, 2 column tiles) as re and im, each as
+// an FP16 hi and lo part; 4 registers (8 rays) per part and tile.
+struct MFrag {
+  uint32_t a[2][6][4];  // [tile][re hi, re lo, im hi, im lo, -im hi, -im lo][pair]
+  uint32_t b[2][4][4];  // [tile][re hi, re lo, im hi, im lo][pair]
+};
+
+__device__ __forceinline__ xh8 as_h8(const uint32_t (&w)[4]) {
+  const uint32_t v[4] __attribute__((aligned(16))) = {w[0], w[1], w[2], w[3]};
+  return __builtin_bit_cast(xh8, v);
+}
+
+// Factors of this lane for the 8 rays kb .. kb + 7 of the LDS block: row
+// (grid x) u0 of row tile 0 and, by one rotation, of row tile 1; column u0
+// likewise.
+__device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_t kb, float u0,
+                                         float m1, MFrag& f) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const xf2 p = {x[2 * j], x[2 * j + 1]};
-    h[j] = __builtin_convertvector(p, xh2);
-    const xf2 r = {__builtin_fmaf((float)h[j].x, m1, p.x), __builtin_fmaf((float)h[j].y, m1, p.y)};
-    l[j] = __builtin_convertvector(r, xh2);
+  for (int jp = 0; jp < 4; ++jp) {
+    float fr[2][2], fi[2][2], gr[2][2], gi[2][2];  // [tile][ray of the pair]
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const uint32_t j = kb + 2 * jp + q;
+      const float ph = lds[MR_PH][j], sx = lds[MR_SX][j];
+      const float gz = lds[MR_GZ][j], sz = lds[MR_SZ][j];
+      const float amp = lds[MR_AMP][j];
+      const float rxc = lds[MR_RXC][j], rxs = lds[MR_RXS][j];
+      const float rzc = lds[MR_RZC][j], rzs = lds[MR_RZS][j];
+      const float a = fmaf(u0, sx, ph);
+      const float c = __builtin_amdgcn_cosf(a) * amp, s = __builtin_amdgcn_sinf(a) * amp;
+      fr[0][q] = c; fi[0][q] = s;
+      fr[1][q] = fmaf(c, rxc, -(s * rxs));
+      fi[1][q] = fmaf(c, rxs, s * rxc);
+      const float b = fmaf(u0, sz, gz);
+      const float gc = __builtin_amdgcn_cosf(b), gs = __builtin_amdgcn_sinf(b);
+      gr[0][q] = gc; gi[0][q] = gs;
+      gr[1][q] = fmaf(gc, rzc, -(gs * rzs));
+      gi[1][q] = fmaf(gc, rzs, gs * rzc);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      split2(fr[t][0], fr[t][1], m1, f.a[t][0][jp], f.a[t][1][jp]);
+      split2(fi[t][0], fi[t][1], m1, f.a[t][2][jp], f.a[t][3][jp]);
+      f.a[t][4][jp] = f.a[t][2][jp] ^ 0x80008000u;
+      f.a[t][5][jp] = f.a[t][3][jp] ^ 0x80008000u;
+      split2(gr[t][0], gr[t][1], m1, f.b[t][0][jp], f.b[t][1][jp]);
+      split2(gi[t][0], gi[t][1], m1, f.b[t][2][jp], f.b[t][3][jp]);
+    }
   }
-  hi = (xh8){h[0].x, h[0].y, h[1].x, h[1].y, h[2].x, h[2].y, h[3].x, h[3].y};
-  lo = (xh8){l[0].x, l[0].y, l[1].x, l[1].y, l[2].x, l[2].y, l[3].x, l[3].y};
 }
 
 __device__ __forceinline__ xf16 mfma3(xh8 ah, xh8 al, xh8 bh, xh8 bl, xf16 c) {
@@ -1417,8 +1466,29 @@ __device__ __forceinline__ xf16 mfma3(xh8 ah, xh8 al, xh8 bh, xh8 bl, xf16 c) {
   return c;
 }
 
+// The 48 products of one step: re += Fr Gr - Fi Gi, im += Fr Gi + Fi Gr
+// per pair of tiles, each real product as hi.hi + hi.lo + lo.hi.
+__device__ __forceinline__ void mfma_step(const MFrag& f, xf16 (&acc)[2][2][2]) {
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr) {
+    const xh8 rh = as_h8(f.a[tr][0]), rl = as_h8(f.a[tr][1]);
+    const xh8 ih = as_h8(f.a[tr][2]), il = as_h8(f.a[tr][3]);
+    const xh8 nh = as_h8(f.a[tr][4]), nl = as_h8(f.a[tr][5]);
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) {
+      const xh8 grh = as_h8(f.b[tc][0]), grl = as_h8(f.b[tc][1]);
+      const xh8 gih = as_h8(f.b[tc][2]), gil = as_h8(f.b[tc][3]);
+      acc[tr][tc][0] = mfma3(rh, rl, grh, grl, acc[tr][tc][0]);
+      acc[tr][tc][0] = mfma3(nh, nl, gih, gil, acc[tr][tc][0]);
+      acc[tr][tc][1] = mfma3(rh, rl, gih, gil, acc[tr][tc][1]);
+      acc[tr][tc][1] = mfma3(ih, il, grh, grl, acc[tr][tc][1]);
+    }
+  }
+}
+
 __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
-                                          float (*lds)[kBlockRays], uint32_t lane, float m1) {
+                                          float (*lds)[MR_N][kBlockRays], uint32_t lane,
+                                          float m1) {
   const InstScratch* sc = (const InstScratch*)I.scr;
   const uint32_t nbz = (I.gnz + kMfmaBlk - 1u) / kMfmaBlk;
   const uint32_t bx = w.pix0 / nbz, bz = w.pix0 - bx * nbz;
@@ -1451,88 +1521,83 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   const RayRec32* rays = I.rays32;  // one ray per lane: vector loads
   const uint32_t r0 = w.chunk * I.chunk_len;
   const uint32_t r1 = min(r0 + I.chunk_len, I.nr);
-  for (uint32_t b0 = r0; b0 < r1; b0 += kBlockRays) {
-    const uint32_t nb = min(kBlockRays, r1 - b0);
-    // per-(ray, block) values, one ray per lane, in FP64
-    {
-      float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f};
-      if (lane < nb) {
-        const RayRec32 rr = rays[b0 + lane];
-        const float ok[3] = {rr.ox, rr.oy, rr.oz};
-        const float dk[3] = {rr.dx, rr.dy, rr.oo};  // TILE: oo holds the third delta
-        double o[3], dl[3];
+  // per-(ray, block) values of the 64 rays from b0, one ray per lane, in
+  // FP64, into LDS buffer `buf` (zero amplitude past r1)
+  auto prologue = [&](uint32_t b0, uint32_t buf) {
+    float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f};
+    if (b0 + lane < r1) {
+      const RayRec32 rr = rays[b0 + lane];
+      const float ok[3] = {rr.ox, rr.oy, rr.oz};
+      const float dk[3] = {rr.dx, rr.dy, rr.oo};  // TILE: oo holds the third delta
+      double o[3], dl[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { o[ax[k]] = ok[k]; dl[ax[k]] = dk[k]; }
-        const double Bq = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) -
-                          2.0 * (o[0] * qa[0] + o[1] * qa[1] + o[2] * qa[2]);
-        const double path = sqrt(Ra * Ra + Bq);
-        const double ip = 1.0 / path, iden = 1.0 / (path + Ra);
-        // kr (|a - o| - |a|), the phase of the FP32 rounding delta of o and
-        // the ray's own offset
-        const double corr = kr * (dl[0] * (qa[0] - o[0]) + dl[1] * (qa[1] - o[1]) +
-                                  dl[2] * (qa[2] - o[2])) * ip;
-        const double g = -Bq * iden * ip * iRa;  // 1/|a - o| - 1/|a|
-        // d/dt of |a + t s - o| - |a + t s| = (a.s) g - (o.s) / |a - o|
-        const double sx = kr * I.gsx * (qa[0] * g - o[0] * ip);
-        const double sz = kr * I.gsz * (qa[2] * g - o[2] * ip);
-        double ph = kr * Bq * iden + ((double)rr.off + corr) - 16.0 * sx;
-        ph -= rint(ph);
-        double gz = -16.0 * sz;
-        gz -= rint(gz);
-        double tx = 32.0 * sx, tz = 32.0 * sz;
-        tx -= rint(tx);
-        tz -= rint(tz);
-        v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
-        v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
-        v[MR_AMP] = rr.cs * scale;
-        v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
-        v[MR_RZC] = __builtin_amdgcn_cosf((float)tz); v[MR_RZS] = __builtin_amdgcn_sinf((float)tz);
-      }
-      __syncthreads();  // the previous block's reads are done
+      for (int k = 0; k < 3; ++k) { o[ax[k]] = ok[k]; dl[ax[k]] = dk[k]; }
+      const double Bq = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) -
+                        2.0 * (o[0] * qa[0] + o[1] * qa[1] + o[2] * qa[2]);
+      const double path = sqrt(Ra * Ra + Bq);
+      const double ip = 1.0 / path, iden = 1.0 / (path + Ra);
+      // kr (|a - o| - |a|), the phase of the FP32 rounding delta of o and
+      // the ray's own offset
+      const double corr = kr * (dl[0] * (qa[0] - o[0]) + dl[1] * (qa[1] - o[1]) +
+                                dl[2] * (qa[2] - o[2])) * ip;
+      const double g = -Bq * iden * ip * iRa;  // 1/|a - o| - 1/|a|
+      // d/dt of |a + t s - o| - |a + t s| = (a.s) g - (o.s) / |a - o|
+      const double sx = kr * I.gsx * (qa[0] * g - o[0] * ip);
+      const double sz = kr * I.gsz * (qa[2] * g - o[2] * ip);
+      double ph = kr * Bq * iden + ((double)rr.off + corr) - 16.0 * sx;
+      ph -= rint(ph);
+      double gz = -16.0 * sz;
+      gz -= rint(gz);
+      double tx = 32.0 * sx, tz = 32.0 * sz;
+      tx -= rint(tx);
+      tz -= rint(tz);
+      v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
+      v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
+      v[MR_AMP] = rr.cs * scale;
+      v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
+      v[MR_RZC] = __builtin_amdgcn_cosf((float)tz); v[MR_RZS] = __builtin_amdgcn_sinf((float)tz);
+    }
 #pragma unroll
-      for (int k = 0; k < MR_N; ++k) lds[k][lane] = v[k];
+    for (int k = 0; k < MR_N; ++k) lds[buf][k][lane] = v[k];
+  };
+  // steps of 16 rays over the chunk (4 per LDS block of 64; the last block
+  // rounds up to an even count, its missing rays with zero amplitude),
+  // software-pipelined in pairs: the factors of the next step are generated
+  // (VALU) while this step's products run (MFMA), one MFMA then five VALU
+  // instructions at a time.  The LDS blocks are double-buffered: block b + 1
+  // is prepared while block b's steps run.
+  const uint32_t nblk = (r1 - r0 + kBlockRays - 1u) / kBlockRays;
+  const uint32_t nlast = r1 - r0 - (nblk - 1u) * kBlockRays;
+  const uint32_t nsteps = 4u * (nblk - 1u) + (nlast + 31u) / 32u * 2u;
+  auto step_ptr = [&](uint32_t k) {
+    return (const float(*)[kBlockRays])lds[(k >> 2) & 1u];
+  };
+  auto step_kb = [&](uint32_t k) { return (k & 3u) * 16u + 8u * half; };
+  prologue(r0, 0u);
+  __syncthreads();
+  MFrag f0, f1;
+  mfma_gen(step_ptr(0u), step_kb(0u), u0, m1, f0);
+  for (uint32_t ks = 0; ks < nsteps; ks += 2u) {
+    if ((ks & 3u) == 0u && (ks >> 2) + 1u < nblk) {
+      prologue(r0 + ((ks >> 2) + 1u) * kBlockRays, ((ks >> 2) + 1u) & 1u);
       __syncthreads();
     }
-    const uint32_t nks = (nb + 15u) / 16u;
-    for (uint32_t ks = 0; ks < nks; ++ks) {
-      const uint32_t kb = ks * 16u + 8u * half;  // this lane's 8 rays of the step
-      float fr0[8], fi0[8], fr1[8], fi1[8], gr0[8], gi0[8], gr1[8], gi1[8];
+    mfma_gen(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, f1);
+    mfma_step(f0, acc);
+    __builtin_amdgcn_sched_group_barrier(0x100, 18, 0);  // the LDS reads first
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const float ph = lds[MR_PH][kb + j], sx = lds[MR_SX][kb + j];
-        const float gz = lds[MR_GZ][kb + j], sz = lds[MR_SZ][kb + j];
-        const float amp = lds[MR_AMP][kb + j];
-        const float rxc = lds[MR_RXC][kb + j], rxs = lds[MR_RXS][kb + j];
-        const float rzc = lds[MR_RZC][kb + j], rzs = lds[MR_RZS][kb + j];
-        const float a = fmaf(u0, sx, ph);
-        const float c = __builtin_amdgcn_cosf(a) * amp, s = __builtin_amdgcn_sinf(a) * amp;
-        fr0[j] = c; fi0[j] = s;
-        fr1[j] = fmaf(c, rxc, -(s * rxs));
-        fi1[j] = fmaf(c, rxs, s * rxc);
-        const float b = fmaf(u0, sz, gz);
-        const float gc = __builtin_amdgcn_cosf(b), gs = __builtin_amdgcn_sinf(b);
-        gr0[j] = gc; gi0[j] = gs;
-        gr1[j] = fmaf(gc, rzc, -(gs * rzs));
-        gi1[j] = fmaf(gc, rzs, gs * rzc);
-      }
-      xh8 arh[2], arl[2], aih[2], ail[2];
-      xh8 brh[2], brl[2], bih[2], bil[2], bnh[2], bnl[2];
-      split8(fr0, m1, arh[0], arl[0]); split8(fi0, m1, aih[0], ail[0]);
-      split8(fr1, m1, arh[1], arl[1]); split8(fi1, m1, aih[1], ail[1]);
-      split8(gr0, m1, brh[0], brl[0]); split8(gi0, m1, bih[0], bil[0]);
-      split8(gr1, m1, brh[1], brl[1]); split8(gi1, m1, bih[1], bil[1]);
+    for (int k = 0; k < 48; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+      __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // then VALU
+    }
+    const uint32_t kn = ks + 2u < nsteps ? ks + 2u : ks + 1u;  // (the last is unused)
+    mfma_gen(step_ptr(kn), step_kb(kn), u0, m1, f0);
+    mfma_step(f1, acc);
+    __builtin_amdgcn_sched_group_barrier(0x100, 18, 1);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) { bnh[t] = -bih[t]; bnl[t] = -bil[t]; }
-#pragma unroll
-      for (int tr = 0; tr < 2; ++tr)
-#pragma unroll
-        for (int tc = 0; tc < 2; ++tc) {
-          // re += Fr Gr - Fi Gi, im += Fr Gi + Fi Gr
-          acc[tr][tc][0] = mfma3(arh[tr], arl[tr], brh[tc], brl[tc], acc[tr][tc][0]);
-          acc[tr][tc][0] = mfma3(aih[tr], ail[tr], bnh[tc], bnl[tc], acc[tr][tc][0]);
-          acc[tr][tc][1] = mfma3(arh[tr], arl[tr], bih[tc], bil[tc], acc[tr][tc][1]);
-          acc[tr][tc][1] = mfma3(aih[tr], ail[tr], brh[tc], brl[tc], acc[tr][tc][1]);
-        }
+    for (int k = 0; k < 48; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+      __builtin_amdgcn_sched_group_barrier(0x002, 5, 1);
     }
   }
   // partial sums of the block's real pixels, in the grid's slot order
@@ -1562,7 +1627,7 @@ __global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ inst
                                                   const WorkItem* __restrict__ items,
                                                   uint32_t n_items, uint32_t* __restrict__ queue,
                                                   float m1) {
-  __shared__ float lds[MR_N][kBlockRays];
+  __shared__ float lds[2][MR_N][kBlockRays];
   const uint32_t lane = threadIdx.x;
   for (;;) {
     const uint32_t v = lane == 0u ? atomicAdd(queue, 1u) : 0u;

@@ -206,7 +206,7 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       rc = enqueue_job_grids(ctx, c32, jobs, specs);
       if (rc == XRTK_OK)
         rc = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, true, &ne->plan, 0, force,
-                           true);
+                           true, true, nullptr, /*route_sync=*/true);
       if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
       if (rc != XRTK_OK) { drop(ne); return rc; }
       if (use_graph) {
@@ -217,7 +217,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
           int rc2 = enqueue_job_grids(ctx, c32, jobs, specs);
           if (rc2 == XRTK_OK)
             rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, false, nullptr, 0,
-                                force, true, /*upload=*/false);
+                                force, true, /*upload=*/false,
+                                ne->plan.routed ? &ne->plan.flags : nullptr);
           e = hipStreamEndCapture(ctx->stream, &graph);
           if (rc2 == XRTK_OK && e == hipSuccess && graph) {
             e = hipGraphInstantiate(&ne->exec, graph, nullptr, nullptr, 0);
@@ -246,7 +247,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       int rc = enqueue_job_grids(ctx, c32, ge->jobs.data(), ge->specs);
       if (rc == XRTK_OK)
         rc = run_instances(ctx, c32, c64, ge->specs, ge->work, ge->meta, true, nullptr, 0,
-                           force, true, /*upload=*/false);
+                           force, true, /*upload=*/false,
+                           ge->plan.routed ? &ge->plan.flags : nullptr);
       if (rc != XRTK_OK) return rc;
       e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return map_err(e);
@@ -527,7 +529,8 @@ extern "C" int xrtk_autotune_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_j
       for (int rep = 0; rep < 3; ++rep) {
         rc = enqueue_job_grids(ctx, c32, jobs, specs);
         if (rc == XRTK_OK)
-          rc = run_instances(ctx, c32, c64, specs, b.work, b.meta, true, nullptr, 0, 0, true);
+          rc = run_instances(ctx, c32, c64, specs, b.work, b.meta, true, nullptr, 0, 0, true,
+                             true, nullptr, /*route_sync=*/true);
         if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
         if (rc != XRTK_OK) return rc;
         float ms = 0.f;

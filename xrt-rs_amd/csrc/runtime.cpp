@@ -243,13 +243,14 @@ bool mfma_allowed(const InstSpec& s, uint32_t force_flags) {
 int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
                   Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
-                  bool upload) {
+                  bool upload, const std::vector<uint32_t>* known_flags, bool route_sync) {
   Phases ph;
   ph.next("xrtk.layout");  // work items and instance metadata
   hipStream_t st = ctx->stream;
   if (!pair_cfg_supported(MODE_F32R, cfg32) || !pair_cfg_supported(MODE_F64, cfg64))
     return XRTK_EINVAL;
   const size_t n = specs.size();
+  if (known_flags && known_flags->size() != n) return XRTK_EINVAL;
   const uint32_t sblock = slot_block(cfg32, cfg64);
   // ---- layout of the per-instance buffers in `work`
   struct Lay {
@@ -281,10 +282,11 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* W = work.as<char>();
 
-  // ---- instance table and work items
+  // ---- instance table
   std::vector<KInst> insts(n);
-  std::vector<WorkItem> items32, items64, items_mfma;
   uint32_t n_f64 = 0;
+  size_t cap32 = 0, cap64 = 0, capmf = 0;  // items of each kernel if every route were taken
+  std::vector<uint8_t> mf_ok(n, 0);
   for (size_t i = 0; i < n; ++i) {
     const InstSpec& s = specs[i];
     const Lay& L = lay[i];
@@ -313,29 +315,9 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     I.gnx = s.gnx; I.gnz = s.gnz; I.gth = s.gth; I.gtw = s.gtw;
     if (!f32) ++n_f64;
     const uint32_t B = f32 ? 64u * cfg32.W * cfg32.P : 64u * cfg64.W * cfg64.P;
-    std::vector<WorkItem>& items = f32 ? items32 : items64;
-    for (uint32_t c = 0; c < L.n_chunks; ++c)
-      for (uint32_t p0 = 0; p0 < s.npad; p0 += B) items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
-    // a grid screen may run on the matrix cores: its block items (the
-    // planner decides on the device; the other kernels skip what it takes)
-    if (mfma_allowed(s, force_flags)) {
-      const uint32_t nb = ((s.gnx + 63u) / 64u) * ((s.gnz + 63u) / 64u);
-      for (uint32_t c = 0; c < L.n_chunks; ++c)
-        for (uint32_t b = 0; b < nb; ++b) items_mfma.push_back(WorkItem{(uint32_t)i, b, c, 0});
-    }
-  }
-  // Batched launches: longest items first so the queue drains evenly.
-  if (sorted && n > 1) {
-    auto by_len = [&](const WorkItem& a, const WorkItem& b) {
-      const KInst& A = insts[a.inst];
-      const KInst& Bq = insts[b.inst];
-      const uint64_t la = std::min<uint64_t>(A.chunk_len, A.nr - (uint64_t)a.chunk * A.chunk_len);
-      const uint64_t lb = std::min<uint64_t>(Bq.chunk_len, Bq.nr - (uint64_t)b.chunk * Bq.chunk_len);
-      return la > lb;
-    };
-    std::stable_sort(items32.begin(), items32.end(), by_len);
-    std::stable_sort(items64.begin(), items64.end(), by_len);
-    std::stable_sort(items_mfma.begin(), items_mfma.end(), by_len);
+    (f32 ? cap32 : cap64) += (size_t)L.n_chunks * (s.npad / B);
+    mf_ok[i] = mfma_allowed(s, force_flags) ? 1u : 0u;
+    if (mf_ok[i]) capmf += (size_t)L.n_chunks * ((s.gnx + 63u) / 64u) * ((s.gnz + 63u) / 64u);
   }
   // merge lists: instances writing their own output merge in one launch;
   // instances that accumulate into an earlier output follow one by one.
@@ -347,10 +329,11 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   uint32_t first_np = 0;
   for (uint32_t i : first) first_np = std::max(first_np, specs[i].npad);
 
+  // metadata: a fixed layout with room for the items of every route
   const size_t a_ins = align_up(sizeof(KInst) * std::max<size_t>(n, 1), 256);
-  const size_t a_i32 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items32.size()), 256);
-  const size_t a_i64 = align_up(sizeof(WorkItem) * std::max<size_t>(1, items64.size()), 256);
-  const size_t a_imf = align_up(sizeof(WorkItem) * std::max<size_t>(1, items_mfma.size()), 256);
+  const size_t a_i32 = align_up(sizeof(WorkItem) * std::max<size_t>(1, cap32), 256);
+  const size_t a_i64 = align_up(sizeof(WorkItem) * std::max<size_t>(1, cap64), 256);
+  const size_t a_imf = align_up(sizeof(WorkItem) * std::max<size_t>(1, capmf), 256);
   const size_t a_lst = align_up(sizeof(uint32_t) * (lists.size() + 1), 256);
   e = meta.ensure(a_ins + a_i32 + a_i64 + a_imf + a_lst + 256);
   if (e != hipSuccess) return XRTK_ENOMEM;
@@ -363,15 +346,6 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   uint32_t* d_queue = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_imf + a_lst);
   if (upload) {
     if (n) e = hipMemcpyAsync(d_insts, insts.data(), sizeof(KInst) * n, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !items32.empty())
-      e = hipMemcpyAsync(d_items32, items32.data(), sizeof(WorkItem) * items32.size(),
-                         hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !items64.empty())
-      e = hipMemcpyAsync(d_items64, items64.data(), sizeof(WorkItem) * items64.size(),
-                         hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !items_mfma.empty())
-      e = hipMemcpyAsync(d_items_mfma, items_mfma.data(), sizeof(WorkItem) * items_mfma.size(),
-                         hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !lists.empty())
       e = hipMemcpyAsync(d_lists, lists.data(), sizeof(uint32_t) * lists.size(),
                          hipMemcpyHostToDevice, st);
@@ -397,11 +371,76 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                     reinterpret_cast<InstScratch*>(W + L.scr),
                     reinterpret_cast<double*>(W + L.part), s.mode, insts[i].kr_hi,
                     insts[i].kr_lo, s.skip, force_flags,
-                    s.static_flags | (mfma_allowed(s, force_flags) ? INST_F_MFMA_OK : 0u), cfg32.P, s.gsx, s.gsz,
+                    s.static_flags | (mf_ok[i] ? INST_F_MFMA_OK : 0u), cfg32.P, s.gsx, s.gsz,
                     reinterpret_cast<RayRec32*>(W + L.rays32),
                     reinterpret_cast<PixRec32*>(W + L.pix32),
                     reinterpret_cast<TileRec*>(W + L.tiles), reinterpret_cast<double*>(W + L.R64),
                     st);
+    if (e != hipSuccess) return map_err(e);
+  }
+  // ---- routing: the planner's flags of the instances that may run on the
+  // matrix cores (given, or read back after the prep), so that each kernel
+  // gets only the items it runs; without them both kernels get the items
+  // and skip what the other takes
+  std::vector<uint32_t> flags(n, 0u);
+  bool have_flags = false;
+  if (known_flags) {
+    flags = *known_flags;
+    have_flags = true;
+  } else if (route_sync && capmf > 0) {
+    ph.next("xrtk.route");
+    e = hipStreamSynchronize(st);
+    for (size_t i = 0; i < n && e == hipSuccess; ++i)
+      if (mf_ok[i] && specs[i].nr)
+        e = hipMemcpy(&flags[i], &reinterpret_cast<InstScratch*>(W + lay[i].scr)->flags,
+                      sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return map_err(e);
+    have_flags = true;
+  }
+  // ---- work items
+  std::vector<WorkItem> items32, items64, items_mfma;
+  for (size_t i = 0; i < n; ++i) {
+    const InstSpec& s = specs[i];
+    const Lay& L = lay[i];
+    const bool f32 = s.mode == MODE_F32R;
+    const bool mf = mf_ok[i] && have_flags && (flags[i] & INST_F_MFMA);
+    const bool valu_items = !mf;
+    const bool mfma_items = mf_ok[i] && (!have_flags || mf);
+    const uint32_t B = f32 ? 64u * cfg32.W * cfg32.P : 64u * cfg64.W * cfg64.P;
+    std::vector<WorkItem>& items = f32 ? items32 : items64;
+    if (valu_items)
+      for (uint32_t c = 0; c < L.n_chunks; ++c)
+        for (uint32_t p0 = 0; p0 < s.npad; p0 += B)
+          items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
+    if (mfma_items) {
+      const uint32_t nb = ((s.gnx + 63u) / 64u) * ((s.gnz + 63u) / 64u);
+      for (uint32_t c = 0; c < L.n_chunks; ++c)
+        for (uint32_t b = 0; b < nb; ++b) items_mfma.push_back(WorkItem{(uint32_t)i, b, c, 0});
+    }
+  }
+  // Batched launches: longest items first so the queue drains evenly.
+  if (sorted && n > 1) {
+    auto by_len = [&](const WorkItem& a, const WorkItem& b) {
+      const KInst& A = insts[a.inst];
+      const KInst& Bq = insts[b.inst];
+      const uint64_t la = std::min<uint64_t>(A.chunk_len, A.nr - (uint64_t)a.chunk * A.chunk_len);
+      const uint64_t lb = std::min<uint64_t>(Bq.chunk_len, Bq.nr - (uint64_t)b.chunk * Bq.chunk_len);
+      return la > lb;
+    };
+    std::stable_sort(items32.begin(), items32.end(), by_len);
+    std::stable_sort(items64.begin(), items64.end(), by_len);
+    std::stable_sort(items_mfma.begin(), items_mfma.end(), by_len);
+  }
+  if (upload) {
+    if (!items32.empty())
+      e = hipMemcpyAsync(d_items32, items32.data(), sizeof(WorkItem) * items32.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items64.empty())
+      e = hipMemcpyAsync(d_items64, items64.data(), sizeof(WorkItem) * items64.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items_mfma.empty())
+      e = hipMemcpyAsync(d_items_mfma, items_mfma.data(), sizeof(WorkItem) * items_mfma.size(),
+                         hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return map_err(e);
   }
   // ---- pair kernels
@@ -454,6 +493,8 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     plan_out->n_f64 = n_f64;
     plan_out->n_inst = (uint32_t)n;
     plan_out->scr0 = n ? reinterpret_cast<const InstScratch*>(W + lay[0].scr) : nullptr;
+    plan_out->routed = have_flags;
+    plan_out->flags = flags;
   }
   return XRTK_OK;
 }

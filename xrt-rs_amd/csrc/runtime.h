@@ -158,6 +158,10 @@ struct Plan {
   uint32_t n_f64 = 0;     // instances on the FP64 kernel
   uint32_t n_inst = 0;
   const InstScratch* scr0 = nullptr;  // device scratch of the first instance
+  // routing: whether the items followed the planner's flags, and those flags
+  // (per instance; 0 where no instance could run on the matrix cores)
+  bool routed = false;
+  std::vector<uint32_t> flags;
 };
 
 }  // namespace xrtk
@@ -242,16 +246,23 @@ double residual_bound_rev(const Box& rays, const Box& pix, const double dq[3], d
 uint32_t route_mode(uint32_t mode, const Box& rays, const Box& pix, const double dq[3],
                     double energy);
 
-// Runs prep, pair and merge for `specs` on ctx->stream (no host sync, no
-// host copies).  work/meta are grown as needed.  When `timed`, HIP events
-// bracket the pair kernels and the whole device pipeline.  With upload ==
-// false the metadata copies are skipped (made by an earlier call with the
-// same specs and buffers), so the call only enqueues kernels and can be
-// captured into a hipGraph.
+// Runs prep, pair and merge for `specs` on ctx->stream (no host copies and
+// no host sync, except the routing read below).  work/meta are grown as
+// needed.  When `timed`, HIP events bracket the pair kernels and the whole
+// device pipeline.  With upload == false the metadata copies are skipped
+// (made by an earlier call with the same specs, flags and buffers), so the
+// call only enqueues kernels and can be captured into a hipGraph.
+// Routing between the VALU and MFMA pair kernels: with `known_flags` (the
+// planner flags an earlier call with the same specs returned in
+// Plan::flags), or with `route_sync` (the stream is synchronised after the
+// prep kernels and the flags read back), each kernel gets only its own
+// items; otherwise both get the items of a grid instance and the device
+// flag decides.
 int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                   const std::vector<InstSpec>& specs, DBuf& work, DBuf& meta, bool timed,
                   Plan* plan_out, int split_override, uint32_t force_flags, bool sorted,
-                  bool upload = true);
+                  bool upload = true, const std::vector<uint32_t>* known_flags = nullptr,
+                  bool route_sync = false);
 
 int map_err(hipError_t e);
 // kirchhoff_gpu's one-shot solve with its device buffers in `ws` (the
