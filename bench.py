@@ -2,14 +2,17 @@
 
 Headline workload (one step): the Gaussian-source case, 1,048,576 rays x 512^2
 pixels at 1000 mm, 10 keV (2.75e11 ray-pixel pairs), solved from a ray set
-resident in HBM: FP64 centroid + prep, FP32 phase-reduced pair kernel, FP64
-merge and base phase, result copied to the host.  Launch shape: that of the
-committed counter record (profiles/pair_kernel_counters.json) when it was
-taken on this workload, else the autotuner's (--autotune forces the tuner,
---shape fixes one).  The same solve with the plain FP64 pair kernel runs
-beside it for the speed ratio and the error (complex rel-L2 and intensity
-error of FP32-reduced vs FP64 over all pixels), and the same solve with
-Ep != 0 on every ray (no Ep = 0 specialisation) for the general rate.
+resident in HBM: FP64 centroid + prep, FP32-reduced pair products (on this
+grid screen the block kernel on the matrix cores: FP16-split factors,
+v_mfma_f32_32x32x16_f16), FP64 merge and base phase, result copied to the
+host.  Launch shape: that of the committed counter record
+(profiles/pair_kernel_counters.json) when it was taken on this workload, else
+the autotuner's (--autotune forces the tuner, --shape fixes one).  Beside it:
+the same solve on the FP32 VALU pair kernel (XRTK_ENGINE_VALU), the same
+solve with the plain FP64 pair kernel for the speed ratio and the error
+(complex rel-L2 and intensity error of FP32-reduced vs FP64 over all pixels),
+and the same solve with Ep != 0 on every ray (no Ep = 0 specialisation) for
+the general rate.
 
 Further legs, reported in the same JSON line:
   batched   64 uneven instances (rays log-uniform 16k-2M, square screens
@@ -58,6 +61,13 @@ PEAK_FP32 = 157.3           # TFLOP/s, MI355X FP32 vector datasheet
 PEAK_FP64 = 78.6            # TFLOP/s, MI355X FP64 vector
 PEAK_MFMA_FP32 = 157.3      # TFLOP/s, MI355X dense FP32 matrix
 PEAK_MFMA_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
+PEAK_MFMA_FP16 = 2500.0     # TFLOP/s, MI355X dense FP16 matrix
+# The block kernel's products: per 16 rays and 32 x 32 pixels, 4 real products
+# x 3 FP16 parts (hi.hi, hi.lo, lo.hi) = 12 v_mfma_f32_32x32x16_f16 of
+# 32 x 32 x 16 x 2 flops, i.e. 24 flops per (ray, pixel) pair.
+MFMA_FLOPS_PER_INST = 32768
+MFMA_FLOPS_PER_PAIR = 24
+VAR_MFMA = 4096             # XRTK_VAR_MFMA
 PEAK_HBM = 8000.0           # GB/s
 # One VALU wave-instruction occupies the issue slot of 64 lanes x one FMA
 # (2 flops): the FP32 datasheet peak is 2 such wave-instructions per CU per
@@ -170,6 +180,64 @@ def valu_roofline(rec, kernel_ms, pairs):
     return roof
 
 
+def mfma_roofline(rec, kernel_ms, pairs, n_mfma):
+    """The block pair kernel against the FP16 matrix-core roofline, by the
+    work it executes: n_mfma v_mfma_f32_32x32x16_f16 per launch (from the
+    launch's blocks and ray steps; the committed counter record's
+    SQ_INSTS_MFMA when it was taken on this workload), 32768 flops each, over
+    the measured launch time, against the 2.5 PFLOP/s dense FP16 peak.  The
+    reference formula's 51 flops per pair are reported apart
+    (equivalent_tflops_51), not as a utilisation."""
+    roof = {"bound": "mfma", "achieved": None, "peak": PEAK_MFMA_FP16, "unit": "TFLOP/s",
+            "frac": None, "traffic": None, "kernel_ms": kernel_ms,
+            "equivalent_tflops_51": pairs * F_PAIR / (kernel_ms * 1e-3) / 1e12,
+            "note": "achieved = executed v_mfma_f32_32x32x16_f16 per launch x 32768 flops / "
+                    "measured launch time, against the FP16 dense matrix peak (each pair "
+                    "takes 24 MFMA flops: 4 real products x 3 FP16 parts); "
+                    "equivalent_tflops_51 counts the reference formula's 51 flops per pair "
+                    "instead and is not a utilisation"}
+    mi = n_mfma
+    if rec is not None and rec.get("mfma_insts_per_dispatch"):
+        roof["mfma_insts_counted"] = rec["mfma_insts_per_dispatch"]
+        mi = rec["mfma_insts_per_dispatch"]
+    roof["mfma_insts_per_launch"] = mi
+    roof["achieved"] = mi * MFMA_FLOPS_PER_INST / (kernel_ms * 1e-3) / 1e12
+    roof["frac"] = roof["achieved"] / PEAK_MFMA_FP16
+    roof["mfma_flops_per_pair"] = mi * MFMA_FLOPS_PER_INST / pairs
+    if rec is None:
+        roof["note"] += "; no counter record matches this plan: traffic and busy share unmeasured"
+        return roof
+    if rec.get("hbm_bytes_per_launch"):
+        roof["traffic"] = rec["hbm_bytes_per_launch"]
+        roof["algorithmic_bytes_per_launch"] = rec.get("algorithmic_bytes_per_launch")
+        roof["traffic_gbps"] = rec["hbm_bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9
+    if rec.get("mfma_busy_frac") is not None:
+        roof["mfma_busy_frac"] = rec["mfma_busy_frac"]
+    if rec.get("sclk_ghz"):
+        # the clock the part held through the profiled launch (GRBM_GUI_ACTIVE
+        # over its duration): the peak scales with it
+        roof["sclk_ghz_measured"] = rec["sclk_ghz"]
+        roof["frac_at_measured_clock"] = roof["frac"] * 2.4 / rec["sclk_ghz"]
+    if rec.get("valu_insts_per_dispatch"):
+        roof["valu_insts_per_pair"] = rec["valu_insts_per_dispatch"] * 64 / pairs
+    roof["counter_source"] = rec.get("source")
+    return roof
+
+
+def mfma_insts(nr, nx, nz, chunk):
+    """v_mfma_f32_32x32x16_f16 of one block-kernel launch: per 64 x 64-pixel
+    block and ray chunk, 48 per step of 16 rays (the steps of each chunk's
+    64-ray LDS blocks, the last rounded up to an even count)."""
+    blocks = ((nx + 63) // 64) * ((nz + 63) // 64)
+    steps = 0
+    for c0 in range(0, nr, chunk):
+        n = min(chunk, nr - c0)
+        nblk = (n + 63) // 64
+        last = n - (nblk - 1) * 64
+        steps += 4 * (nblk - 1) + (last + 31) // 32 * 2
+    return blocks * steps * 48
+
+
 # ---------------------------------------------------------------------------
 def leg_headline(args, ctx, xrtk, W):
     rays = W.gaussian_source(args.rays, seed=2)
@@ -231,7 +299,36 @@ def leg_headline(args, ctx, xrtk, W):
         k_ms.append(ctx.last_timing()[0])
     kernel_ms = float(np.median(k_ms))
     plan = ctx.last_plan()
-    log(f"pair kernel: {kernel_ms:.1f} ms/launch  plan {plan}")
+    on_mfma = bool(plan["inst_flags"] & VAR_MFMA)
+    log(f"pair kernel: {kernel_ms:.2f} ms/launch ({'matrix cores' if on_mfma else 'VALU'})  "
+        f"plan {plan}")
+
+    # the same solve on the FP32 VALU pair kernel (XRTK_ENGINE_VALU): the
+    # rate the matrix-core kernel is measured against, with its own roofline
+    valu_leg = None
+    if on_mfma and not args.no_valu_leg:
+        xrtk.solve_jobs(ctx, [job], mode="f32r", engine=1, **shape)
+        device_sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            (ov,) = xrtk.solve_jobs(ctx, [job], mode="f32r", engine=1, **shape)
+        device_sync()
+        t_v = (time.perf_counter() - t) / args.steps
+        kv = []
+        for _ in range(2):
+            xrtk.diffraction_integral(rays, wave, ctx=ctx, mode="f32r", engine=1, **shape)
+            kv.append(ctx.last_timing()[0])
+        plan_v = ctx.last_plan()
+        kms_v = float(np.median(kv))
+        valu_leg = {"value": pairs / t_v / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * t_v,
+                    "kernel_ms": kms_v, "inst_flags": plan_v["inst_flags"],
+                    "roofline": valu_roofline(counter_record(nr, npx, plan_v), kms_v, pairs),
+                    "rel_l2_vs_matrix_cores": W.rel_l2(ov, out32),
+                    "what": "the headline solve on the FP32 VALU pair kernel (XRTK_ENGINE_VALU)"}
+        log(f"valu kernel: {1e3 * t_v:.1f} ms/step, {valu_leg['value']:.0f} Gpair/s, pair kernel "
+            f"{kms_v:.1f} ms, flags {plan_v['inst_flags']:#x}; matrix cores are "
+            f"{value / valu_leg['value']:.2f}x faster, rel-L2 between the two "
+            f"{valu_leg['rel_l2_vs_matrix_cores']:.2e}")
 
     # the same rays with Ep != 0 (Ep = 0.5 e^{i pi/5} Es on every ray): the
     # general sums, without the Ep = 0 specialisation.  Es is the same field
@@ -305,11 +402,22 @@ def leg_headline(args, ctx, xrtk, W):
                            "merge, result copy (PCIe-inclusive)"}
         log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call")
 
-    roof = valu_roofline(counter_record(nr, npx, plan), kernel_ms, pairs)
-    if roof["achieved"] is not None:
-        log(f"roofline: {roof['achieved']:.1f} TFLOP/s of VALU issue slots "
-            f"({roof['frac']:.3f} of {PEAK_FP32}), {roof['valu_insts_per_pair']:.2f} VALU "
-            f"instructions per pair; formula-equivalent {roof['equivalent_tflops_51']:.0f} TFLOP/s")
+    if on_mfma:
+        n_chunks = max(1, plan["n_items"] // (((args.side + 63) // 64) ** 2))
+        tr = max(1, plan["tile_rays"])
+        chunk = -(-(-(-nr // n_chunks)) // tr) * tr
+        roof = mfma_roofline(counter_record(nr, npx, plan, key="mfma"), kernel_ms, pairs,
+                             mfma_insts(nr, args.side, args.side, chunk))
+        log(f"roofline: {roof['achieved']:.0f} TFLOP/s of FP16 MFMA ({roof['frac']:.3f} of "
+            f"{PEAK_MFMA_FP16:.0f}), {roof['mfma_flops_per_pair']:.1f} MFMA flops per pair; "
+            f"formula-equivalent {roof['equivalent_tflops_51']:.0f} TFLOP/s")
+    else:
+        roof = valu_roofline(counter_record(nr, npx, plan), kernel_ms, pairs)
+        if roof["achieved"] is not None:
+            log(f"roofline: {roof['achieved']:.1f} TFLOP/s of VALU issue slots "
+                f"({roof['frac']:.3f} of {PEAK_FP32}), {roof['valu_insts_per_pair']:.2f} VALU "
+                f"instructions per pair; formula-equivalent "
+                f"{roof['equivalent_tflops_51']:.0f} TFLOP/s")
     if fp64 is not None:
         rec64 = counter_record(nr, npx, None, key="fp64")
         fp64["counters"] = None if rec64 is None else {
@@ -318,7 +426,8 @@ def leg_headline(args, ctx, xrtk, W):
                                       "trans_insts_per_pair", "source")}
     head = {"value": value, "ms_per_step": ms_step, "pairs": pairs, "nr": nr, "npx": npx,
             "plan": plan, "shape_from": shape_from, "roofline": roof, "fp64": fp64,
-            "oneshot": oneshot, "ep_nonzero": ep_leg}
+            "oneshot": oneshot, "ep_nonzero": ep_leg, "valu_kernel": valu_leg,
+            "pair_engine": "matrix cores (block kernel)" if on_mfma else "VALU"}
     return head, rays, wave, out32, rs
 
 
@@ -753,6 +862,7 @@ def main():
     ap.add_argument("--serve-compare", action="store_true",
                     help="also serve the sub-saturation stream in arrival order")
     ap.add_argument("--no-ep-leg", action="store_true")
+    ap.add_argument("--no-valu-leg", action="store_true")
     ap.add_argument("--quantum-ms", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--shape", default="", metavar="P,W,PERSISTENT",
@@ -817,12 +927,19 @@ def main():
                                       "slowdown_p99", "largest_request_latency_ms",
                                       "sustained_gpairs_per_s", "policy")}
             over = serving["stated"]["offered_gpairs_per_s"] * 1e9 / cap
-            serving["note"] = (f"the stated {args.serve_rate:.0f} req/s of requests sized as the "
-                               f"batched instances offers {over:.1f}x the pair rate one GPU "
-                               "completes: the queue grows for the whole window, so the stated "
-                               "leg's latency grows with the window (--serve-seconds) and its "
-                               "sustained rate is the figure; 'sustainable' arrives at a "
-                               "fraction of the measured batched capacity")
+            if over > 1.0:
+                serving["note"] = (f"the stated {args.serve_rate:.0f} req/s of requests sized as "
+                                   f"the batched instances offers {over:.1f}x the pair rate one "
+                                   "GPU completes: the queue grows for the whole window, so the "
+                                   "stated leg's latency grows with the window (--serve-seconds) "
+                                   "and its sustained rate is the figure; 'sustainable' arrives "
+                                   "at a fraction of the measured batched capacity")
+            else:
+                serving["note"] = (f"the stated {args.serve_rate:.0f} req/s of requests sized as "
+                                   f"the batched instances offers {over:.2f} of the pair rate "
+                                   "one GPU completes in a batch; 'sustainable' arrives at a "
+                                   "fixed fraction of that capacity")
+            serving["offered_fraction_of_batched_capacity"] = over
         for r in sets:
             r.release()
 
@@ -830,13 +947,18 @@ def main():
         "metric": "Gpair-evals/s FP32-reduced vs FP64 at relL2<=1e-4; served p50/p99 latency",
         "value": head["value"], "unit": "Gpair-evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "scaling": "weak", "vs_baseline": None,
+        # block kernel: FP16 hi + lo operands, three FP16 products per real
+        # product, FP32 accumulation (FP64 anchors and merge); VALU kernel: FP32
+        "dtype": "f16x3/f32" if head["pair_engine"].startswith("matrix") else "f32",
         "data": "synthetic (seeded Gaussian source; seeded point-source instances)",
         "config": {"workload": "gaussian-source Kirchhoff solve, 1,048,576 rays x 512^2 pixels "
                                "at 1000 mm, 10 keV, resident ray set",
                    "rays": head["nr"], "pixels": head["npx"], "pairs": head["pairs"],
                    "plan": head["plan"], "shape_from": head["shape_from"]},
         "roofline": head["roofline"],
+        "pair_engine": head["pair_engine"],
+        "valu_kernel": head["valu_kernel"],
         "fp64": head["fp64"],
         "ep_nonzero": head["ep_nonzero"],
         "oneshot_host_buffers": head["oneshot"],
