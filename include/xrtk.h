@@ -107,7 +107,7 @@ typedef struct XrtkOpts {
 
 /* Which pair kernels an FP32-reduced solve may use.  AUTO: an instance on a
  * prepare_wave grid whose phase is linear over 64 x 64-pixel blocks to
- * within 1e-7 rev (bounded by the device planner) runs on the matrix cores
+ * within 1e-6 rev (bounded by the device planner) runs on the matrix cores
  * (FP16-split MFMA products, FP32 block sums); every other instance, and
  * VALU always, on the FP32 VALU pair kernels. */
 #define XRTK_ENGINE_AUTO 0
@@ -333,9 +333,9 @@ typedef struct XrtkPlan {
     int32_t split, ray_src, persistent, n_items;
     int32_t n_fp64;     /* instances the bound routed to the FP64 kernel    */
     int32_t backend;    /* 1 = device, 0 = CPU fallback                     */
-    int32_t inst_flags; /* one-shot solves: variant flags the device planner
-                           chose for the first instance (XRTK_VAR_*)        */
-    int32_t reserved;
+    int32_t inst_flags; /* variant flags the device planner chose for the
+                           first instance (XRTK_VAR_*)                      */
+    int32_t n_mfma;     /* instances on the matrix cores (XRTK_VAR_MFMA)    */
 } XrtkPlan;
 /* Variant flags reported in XrtkPlan.inst_flags. */
 #define XRTK_VAR_CHECK     1   /* FP64: a pair may fall below the skip threshold */

@@ -151,7 +151,7 @@ static_assert(XRTK_VAR_CHECK == INST_F_CHECK && XRTK_VAR_TILE == INST_F_TILE &&
                   XRTK_VAR_NOFRACT == INST_F_NOFRACT && XRTK_VAR_LIN2 == INST_F_LIN2 &&
                   XRTK_VAR_PAX == INST_F_PAX && XRTK_VAR_SEPAMP == INST_F_SEPAMP &&
                   XRTK_VAR_LINE == INST_F_LINE && XRTK_VAR_LINEDEV == INST_F_LINEDEV &&
-                  XRTK_VAR_EPZERO == INST_F_EPZERO,
+                  XRTK_VAR_EPZERO == INST_F_EPZERO && XRTK_VAR_MFMA == INST_F_MFMA,
               "variant flags of the ABI match the device planner's");
 
 void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
@@ -169,7 +169,8 @@ void record_plan(xrtk_ctx* ctx, const Plan& p, int backend) {
   q.n_fp64 = (int32_t)p.n_f64;
   q.backend = backend;
   q.inst_flags = 0;
-  q.reserved = 0;
+  q.n_mfma = 0;
+  for (uint32_t f : p.flags) q.n_mfma += (f & INST_F_MFMA) ? 1 : 0;
 }
 
 // Pixel source of a one-shot solve.

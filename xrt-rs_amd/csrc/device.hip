@@ -758,10 +758,13 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     //     + ((o.e)^2 - 2 (q.e)(o.e)) / |d|^3,
     // where Rl = min|q| - |o|max - h bounds |q| and |d| from below over the
     // blocks, Q_i = max|q_i| (+ h in the plane), and |q.e| <= |(Q_x, Q_z)|,
-    // |o.e| <= |(o_x, o_z)|.  Allowed when the bound is <= 1e-7 rev and the
-    // phase steps stay small enough (|grad f . step| <= 2 kr |step| |o| / Rl
-    // per pixel, times 16 from a tile centre, plus the half revolution of the
-    // anchor: <= 8 rev) that FP32 holds them to ~5e-7 rev.
+    // |o.e| <= |(o_x, o_z)|.  A phase error of at most E rev per term moves
+    // the field by at most 2 pi E relative (rel-L2 included), so the variant
+    // is allowed when E <= 1e-6 rev (6.3e-6 rad) and the phase steps stay
+    // small enough that FP32 holds the factors' phases to ~5e-7 rev: per
+    // pixel along x, |df . step| = kr |gsx| |q_x (1/|d| - 1/|q|) - o_x/|d||
+    // <= kr |gsx| (Q_x dR / Rl^2 + o_x / Rl) (likewise z), times 16 from a
+    // tile centre plus the half revolution of the anchor, <= 8 rev.
     const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
                             INST_F_EPZERO;
     if ((static_flags & INST_F_MFMA_OK) && (f & need_m) == need_m && !(force_flags & 0x20000u)) {
@@ -776,9 +779,9 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         const double t1 = dR / R2;
         const double t2 = Qp2 * 3.0 * dR / (R2 * R2) + (2.0 * sqrt(Qp2) * op + op * op) / R3;
         const double E = 0.5 * kr * (t1 + t2) * h * h;
-        const double smax = 2.0 * kr * fmax(ax, az) * om / Rl;
+        const double smax = kr * fmax(ax * (Qx * t1 + ox / Rl), az * (Qz * t1 + oz / Rl));
         scr->blk_rev = E;
-        if (E <= 1e-7 && 16.0 * smax + 0.5 <= 8.0) f |= INST_F_MFMA;
+        if (E <= 1e-6 && 16.0 * smax + 0.5 <= 8.0) f |= INST_F_MFMA;
       }
     }
   }
@@ -1360,7 +1363,7 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
 // ---------------------------------------------------------------------------
 // Over a block of 64 x 64 grid pixels with centre a, a ray's reduced phase
 // is linear in the pixel offsets (u, v) from a, up to a remainder that
-// k_plan bounds (<= 1e-7 rev):
+// k_plan bounds (<= 1e-6 rev):
 //   f(u, v) = ph + u sx + v sz,
 // so its term factors into a row factor and a column factor,
 //   A e^{2 pi i f} = [A e^{2 pi i (ph + u sx)}] [e^{2 pi i v sz}],

@@ -153,7 +153,7 @@ enum : uint32_t {
   INST_F_EPZERO = 2048u,  // F32R: every ray has Ep == 0, so the Ep sums are exact
                           // zeros and the pair kernel skips them
   INST_F_MFMA = 4096u,    // F32R grid screen: blocks of 64 x 64 pixels run on the
-                          // matrix cores (phase linear over a block, bound <= 1e-7
+                          // matrix cores (phase linear over a block, bound <= 1e-6
                           // rev); the VALU pair kernels skip the instance
   INST_F_MFMA_OK = 8192u, // static flag from the host: this launch runs the MFMA
                           // items of the instance (a grid screen)

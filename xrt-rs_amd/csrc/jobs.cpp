@@ -282,7 +282,8 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       q.n_fp64 = (int32_t)p.n_f64;
       q.backend = 1;
       q.inst_flags = 0;
-      q.reserved = 0;
+      q.n_mfma = 0;
+      for (uint32_t f : p.flags) q.n_mfma += (f & INST_F_MFMA) ? 1 : 0;
     }
     // results -> host
     ph.next("xrtk.readback");

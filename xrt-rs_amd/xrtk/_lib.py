@@ -78,7 +78,7 @@ class XrtkJob(C.Structure):
 class XrtkPlan(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "mode", "pixels_per_lane", "waves_per_group", "tile_rays", "split", "ray_src",
-        "persistent", "n_items", "n_fp64", "backend", "inst_flags", "reserved")]
+        "persistent", "n_items", "n_fp64", "backend", "inst_flags", "n_mfma")]
 
 
 class XrtkFresnelScreen(C.Structure):
