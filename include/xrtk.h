@@ -428,6 +428,14 @@ void xrtk_server_destroy(xrtk_server *s);
 /* Steady-clock seconds, the clock of xrtk_server_wait's timestamps. */
 double xrtk_now(void);
 
+/* Page-locked host memory for result buffers that a caller reuses across
+ * solves (device-to-host copies into it run at the link's rate instead of
+ * through a staging copy).  No reference counterpart: the reference returns
+ * a fresh Vec per call (src/gpu/kirchhoff.rs:276-297).  The memory belongs
+ * to the process, not to a context; free it with xrtk_host_free. */
+int xrtk_host_alloc(size_t bytes, void **out);
+void xrtk_host_free(void *p);
+
 /* ---- testing ------------------------------------------------------------ */
 
 /* Makes the next n pair-kernel launches of this process fail before anything

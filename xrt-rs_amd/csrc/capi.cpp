@@ -796,4 +796,26 @@ int xrtk_last_plan(xrtk_ctx* ctx, XrtkPlan* plan) {
 
 double xrtk_now(void) { return now_s(); }
 
+int xrtk_host_alloc(size_t bytes, void** out) {
+  if (!out) return XRTK_EINVAL;
+  *out = nullptr;
+  if (!bytes) return XRTK_EINVAL;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return XRTK_ENODEV;
+  }
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return XRTK_ENOMEM;
+  }
+  *out = p;
+  return XRTK_OK;
+}
+
+void xrtk_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 }  // extern "C"

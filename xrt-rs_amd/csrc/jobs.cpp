@@ -295,11 +295,17 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       if (e != hipSuccess) return map_err(e);
     }
     e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess && ge->plan.scr0) {
-      // the variant flags the device planner chose for the first job
-      uint32_t fl = 0;
-      if (hipMemcpy(&fl, &ge->plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) == hipSuccess)
-        ctx->last_plan.inst_flags = (int32_t)fl;
+    if (e == hipSuccess) {
+      // the variant flags the device planner chose for the first job (known
+      // on the host when the items were routed)
+      if (ge->plan.routed && !ge->plan.flags.empty() && ge->plan.flags[0]) {
+        ctx->last_plan.inst_flags = (int32_t)ge->plan.flags[0];
+      } else if (ge->plan.scr0) {
+        uint32_t fl = 0;
+        if (hipMemcpy(&fl, &ge->plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) ==
+            hipSuccess)
+          ctx->last_plan.inst_flags = (int32_t)fl;
+      }
     }
     return map_err(e);
   } catch (...) {

@@ -33,6 +33,7 @@ __all__ = [
     "CHBAR", "GPU_RAY_DTYPE", "GPU_PIXEL_DTYPE", "GpuContext", "kirchhoff_gpu",
     "kirchhoff_auto", "diffraction_integral", "diffraction_integral_rs", "WaveParams",
     "prepare_wave", "beam_to_diffraction_rays", "RaySet", "Job", "solve_jobs", "autotune_jobs",
+    "pinned_result",
     "Server", "FresnelScreen", "FresnelResult", "fresnel_propagate", "Undulator",
     "UndulatorGpuResult", "undulator_gpu", "GPU_OBS_DTYPE",
     "XrtkError", "MODE_F32R", "MODE_F64", "RAY_FIELDS",
@@ -416,14 +417,43 @@ class Job:
         return j
 
 
-def solve_jobs(ctx: GpuContext, jobs: Sequence[Job], mode="f32r", **tune):
+def pinned_result(n_pixels: int) -> np.ndarray:
+    """An (n_pixels, 2) complex128 result buffer [(Es, Ep)] in page-locked
+    host memory (xrtk_host_alloc), freed when the array is collected: the
+    out= buffer of repeated solves."""
+    import weakref
+    nbytes = max(1, int(n_pixels)) * 32
+    p = C.c_void_p()
+    check(lib.xrtk_host_alloc(nbytes, C.byref(p)), "xrtk_host_alloc")
+    buf = (C.c_char * nbytes).from_address(p.value)
+    arr = np.frombuffer(buf, dtype=np.complex128, count=2 * int(n_pixels)).reshape(-1, 2)
+    weakref.finalize(buf, lib.xrtk_host_free, p.value)
+    return arr
+
+
+def _out4(a: np.ndarray, n_pixels: int) -> np.ndarray:
+    """The np x 4 f64 view of a caller's (n_pixels, 2) complex128 buffer."""
+    if (not isinstance(a, np.ndarray) or a.dtype != np.complex128 or a.shape != (n_pixels, 2)
+            or not a.flags.c_contiguous or not a.flags.writeable):
+        raise ValueError(f"out: need a writable C-contiguous ({n_pixels}, 2) complex128 array")
+    return a.view(np.float64).reshape(-1, 4)
+
+
+def solve_jobs(ctx: GpuContext, jobs: Sequence[Job], mode="f32r", out=None, **tune):
     """All jobs in one persistent launch (hipGraph-replayed while the batch
-    repeats).  Returns one (n_pixels, 2) complex128 array per job."""
-    outs = [np.zeros((j.screen.nx * j.screen.nz, 4)) for j in jobs]
+    repeats).  Returns one (n_pixels, 2) complex128 array per job; with out=
+    (one such array per job, e.g. from pinned_result) the results are written
+    there and those arrays are returned."""
+    if out is None:
+        outs = [np.zeros((j.screen.nx * j.screen.nz, 4)) for j in jobs]
+    else:
+        if len(out) != len(jobs):
+            raise ValueError("out: one buffer per job")
+        outs = [_out4(a, j.screen.nx * j.screen.nz) for a, j in zip(out, jobs)]
     arr = (_lib.XrtkJob * len(jobs))(*[j.to_c(o) for j, o in zip(jobs, outs)])
     o = _lib.opts(_mode(mode), **tune)
     check(lib.xrtk_solve_jobs(ctx.handle, arr, len(jobs), C.byref(o)), "xrtk_solve_jobs")
-    return [_to_complex(x) for x in outs]
+    return list(out) if out is not None else [_to_complex(x) for x in outs]
 
 
 def autotune_jobs(ctx: GpuContext, jobs: Sequence[Job], mode="f32r"):

@@ -162,6 +162,8 @@ PROTOTYPES = {
     "xrtk_server_stats": (_i, [_p, C.POINTER(XrtkServerStats)]),
     "xrtk_server_destroy": (None, [_p]),
     "xrtk_now": (C.c_double, []),
+    "xrtk_host_alloc": (_i, [_sz, C.POINTER(_p)]),
+    "xrtk_host_free": (None, [_p]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():
