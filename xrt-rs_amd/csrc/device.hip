@@ -765,8 +765,7 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     // pixel along x, |df . step| = kr |gsx| |q_x (1/|d| - 1/|q|) - o_x/|d||
     // <= kr |gsx| (Q_x dR / Rl^2 + o_x / Rl) (likewise z), times 16 from a
     // tile centre plus the half revolution of the anchor, <= 8 rev.
-    const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP |
-                            INST_F_EPZERO;
+    const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP;
     if ((static_flags & INST_F_MFMA_OK) && (f & need_m) == need_m && !(force_flags & 0x20000u)) {
       const double ax = fabs(gsx), az = fabs(gsz);
       const double h = 32.0 * sqrt(ax * ax + az * az);
@@ -800,8 +799,8 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
 // from delta kept in the record; the centroid-referenced variant, used only
 // when the whole phase is small, takes kr delta.u once per ray, u from the
 // centroid to the centre of the screen's bounding box.
-// Returns |cs| of the record (the MFMA kernel scales its FP16 factors by the
-// largest).
+// Returns max(|cs|, |pr| + |pi|) of the record, a bound of the moduli of
+// both amplitudes (the MFMA kernel scales its FP16 factors by the largest).
 __device__ __forceinline__ float ray_prep_one(const RayRec64* __restrict__ rays, uint32_t i,
                                               const InstScratch* __restrict__ scr, double kr,
                                               double kr_lo, uint32_t have_pix,
@@ -881,7 +880,7 @@ __device__ __forceinline__ float ray_prep_one(const RayRec64* __restrict__ rays,
   o.am = (float)am;
   o.off = (float)off;
   out[i] = o;
-  return fabsf(o.cs);
+  return fmaxf(fabsf(o.cs), fabsf(o.pr) + fabsf(o.pi));
 }
 
 __global__ __launch_bounds__(256) void k_ray_prep32(const RayRec64* __restrict__ rays,
@@ -1376,9 +1375,13 @@ __global__ __launch_bounds__(64 * W) void k_pair_f32r(
 // is ~2^-22 |x|); the three products hi.hi + hi.lo + lo.hi run on
 // v_mfma_f32_32x32x16_f16 with FP32 accumulation, leaving ~2^-21 per term.
 // Amplitudes are scaled by a power of two so the largest is in [1/2, 1).
-// Each wave owns one block (2 x 2 tiles of 32 x 32 pixels) and one ray
-// chunk, and writes the chunk's partial sums of the block's real pixels to
-// the slots the F32R kernel would write (k_merge is shared).
+// Each wave owns one block (2 x 2 tiles of 32 x 32 pixels), one ray chunk
+// and one of the two sums: the Es pass (real amplitude cs) writes the
+// chunk's partial Es of the block's real pixels, the Ep pass (complex
+// amplitude pr + i pi in the row factors, the same column factors) their
+// partial Ep, in the slots the F32R kernel would write (k_merge is
+// shared).  Rays that all have Ep = 0 take the Es pass alone, which then
+// writes Ep = 0.
 typedef _Float16 xh8 __attribute__((ext_vector_type(8)));
 typedef _Float16 xh2 __attribute__((ext_vector_type(2)));
 typedef float xf2 __attribute__((ext_vector_type(2)));
@@ -1392,9 +1395,10 @@ enum : int {
   MR_SX,      // phase step per row (grid x)
   MR_GZ,      // phase at the centre of the first column tile relative to a
   MR_SZ,      // phase step per column (grid z)
-  MR_AMP,     // scaled amplitude
+  MR_AMP,     // scaled amplitude: cs (Es pass) or pr (Ep pass)
   MR_RXC, MR_RXS,  // e^{2 pi i 32 sx}: first row tile to the second
   MR_RZC, MR_RZS,  // e^{2 pi i 32 sz}
+  MR_AMPI,    // Ep pass: scaled pi (the imaginary part of the amplitude)
   MR_N
 };
 
@@ -1426,6 +1430,7 @@ __device__ __forceinline__ xh8 as_h8(const uint32_t (&w)[4]) {
 // Factors of this lane for the 8 rays kb .. kb + 7 of the LDS block: row
 // (grid x) u0 of row tile 0 and, by one rotation, of row tile 1; column u0
 // likewise.
+template <bool EPP>
 __device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_t kb, float u0,
                                          float m1, MFrag& f) {
 #pragma unroll
@@ -1440,7 +1445,17 @@ __device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_
       const float rxc = lds[MR_RXC][j], rxs = lds[MR_RXS][j];
       const float rzc = lds[MR_RZC][j], rzs = lds[MR_RZS][j];
       const float a = fmaf(u0, sx, ph);
-      const float c = __builtin_amdgcn_cosf(a) * amp, s = __builtin_amdgcn_sinf(a) * amp;
+      float c, s;
+      if constexpr (EPP) {
+        // (pr + i pi) e^{2 pi i a}
+        const float ampi = lds[MR_AMPI][j];
+        const float ca = __builtin_amdgcn_cosf(a), sa = __builtin_amdgcn_sinf(a);
+        c = fmaf(amp, ca, -(ampi * sa));
+        s = fmaf(amp, sa, ampi * ca);
+      } else {
+        c = __builtin_amdgcn_cosf(a) * amp;
+        s = __builtin_amdgcn_sinf(a) * amp;
+      }
       fr[0][q] = c; fi[0][q] = s;
       fr[1][q] = fmaf(c, rxc, -(s * rxs));
       fi[1][q] = fmaf(c, rxs, s * rxc);
@@ -1489,6 +1504,7 @@ __device__ __forceinline__ void mfma_step(const MFrag& f, xf16 (&acc)[2][2][2]) 
   }
 }
 
+template <bool EPP>
 __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
                                           float (*lds)[MR_N][kBlockRays], uint32_t lane,
                                           float m1) {
@@ -1527,7 +1543,7 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   // per-(ray, block) values of the 64 rays from b0, one ray per lane, in
   // FP64, into LDS buffer `buf` (zero amplitude past r1)
   auto prologue = [&](uint32_t b0, uint32_t buf) {
-    float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f};
+    float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f, 0.f};
     if (b0 + lane < r1) {
       const RayRec32 rr = rays[b0 + lane];
       const float ok[3] = {rr.ox, rr.oy, rr.oz};
@@ -1556,7 +1572,8 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
       tz -= rint(tz);
       v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
       v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
-      v[MR_AMP] = rr.cs * scale;
+      v[MR_AMP] = (EPP ? rr.pr : rr.cs) * scale;
+      v[MR_AMPI] = EPP ? rr.pi * scale : 0.f;
       v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
       v[MR_RZC] = __builtin_amdgcn_cosf((float)tz); v[MR_RZS] = __builtin_amdgcn_sinf((float)tz);
     }
@@ -1579,13 +1596,13 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   prologue(r0, 0u);
   __syncthreads();
   MFrag f0, f1;
-  mfma_gen(step_ptr(0u), step_kb(0u), u0, m1, f0);
+  mfma_gen<EPP>(step_ptr(0u), step_kb(0u), u0, m1, f0);
   for (uint32_t ks = 0; ks < nsteps; ks += 2u) {
     if ((ks & 3u) == 0u && (ks >> 2) + 1u < nblk) {
       prologue(r0 + ((ks >> 2) + 1u) * kBlockRays, ((ks >> 2) + 1u) & 1u);
       __syncthreads();
     }
-    mfma_gen(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, f1);
+    mfma_gen<EPP>(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, f1);
     mfma_step(f0, acc);
     __builtin_amdgcn_sched_group_barrier(0x100, 18, 0);  // the LDS reads first
 #pragma unroll
@@ -1594,7 +1611,7 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
       __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // then VALU
     }
     const uint32_t kn = ks + 2u < nsteps ? ks + 2u : ks + 1u;  // (the last is unused)
-    mfma_gen(step_ptr(kn), step_kb(kn), u0, m1, f0);
+    mfma_gen<EPP>(step_ptr(kn), step_kb(kn), u0, m1, f0);
     mfma_step(f1, acc);
     __builtin_amdgcn_sched_group_barrier(0x100, 18, 1);
 #pragma unroll
@@ -1608,6 +1625,9 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   const uint32_t ntz = (I.gnz + tw - 1u) / tw;
   float4* part = reinterpret_cast<float4*>(I.partial) + (size_t)w.chunk * I.npad;
   const uint32_t n = lane & 31u;
+  // the Es pass writes Ep = 0 as well when every ray has Ep = 0 (no Ep pass
+  // runs); otherwise each pass writes its own half of the slot's partial
+  const bool ep0 = (sc->flags & INST_F_EPZERO) != 0u;
 #pragma unroll
   for (int tr = 0; tr < 2; ++tr)
 #pragma unroll
@@ -1618,14 +1638,24 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
         const uint32_t ix = ix0 + 32u * tr + m, iz = iz0 + 32u * tc + n;
         if (ix < I.gnx && iz < I.gnz) {
           const uint32_t slot = ((ix / th) * ntz + iz / tw) * tsz + (ix % th) * tw + iz % tw;
-          part[slot] = make_float4(acc[tr][tc][0][i] * unscale, acc[tr][tc][1][i] * unscale, 0.f,
-                                   0.f);
+          const float re = acc[tr][tc][0][i] * unscale, im = acc[tr][tc][1][i] * unscale;
+          float2* p2 = reinterpret_cast<float2*>(part + slot);
+          if constexpr (EPP) {
+            p2[1] = make_float2(re, im);
+          } else if (ep0) {
+            part[slot] = make_float4(re, im, 0.f, 0.f);
+          } else {
+            p2[0] = make_float2(re, im);
+          }
         }
       }
 }
 
 // Persistent: one wave per group takes items from the queue until it runs
-// dry; items of instances without the MFMA flag are skipped.
+// dry; items of instances without the MFMA flag are skipped.  EPP: the Ep
+// pass (its items, pad = 1), which instances with Ep = 0 everywhere skip;
+// each pass is its own kernel, so each has the registers of its own code.
+template <bool EPP>
 __global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ insts,
                                                   const WorkItem* __restrict__ items,
                                                   uint32_t n_items, uint32_t* __restrict__ queue,
@@ -1639,7 +1669,12 @@ __global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ inst
     const WorkItem w = ((const XRTK_CONST WorkItem*)items)[it];
     const KInst I = ((const XRTK_CONST KInst*)insts)[w.inst];
     const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
-    if (f & INST_F_MFMA) mfma_item(I, w, lds, lane, m1);
+    if (!(f & INST_F_MFMA)) continue;
+    if constexpr (EPP) {
+      if (!(f & INST_F_EPZERO)) mfma_item<true>(I, w, lds, lane, m1);
+    } else {
+      mfma_item<false>(I, w, lds, lane, m1);
+    }
   }
 }
 
@@ -1970,13 +2005,18 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 }
 
 hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
-                       uint32_t* queue, uint32_t grid, hipStream_t st) {
+                       uint32_t* queue, uint32_t grid, bool ep_pass, hipStream_t st) {
   if (!n_items) return hipSuccess;
   for (uint32_t f = g_fail_launches.load(); f > 0;)
     if (g_fail_launches.compare_exchange_weak(f, f - 1)) return hipErrorLaunchFailure;
   hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 1u);
-  hipLaunchKernelGGL(k_pair_mfma, dim3(std::max<uint32_t>(1u, grid)), dim3(64), 0, st, insts,
-                     items, n_items, queue, -1.0f);
+  const dim3 g(std::max<uint32_t>(1u, grid));
+  if (ep_pass)
+    hipLaunchKernelGGL(k_pair_mfma<true>, g, dim3(64), 0, st, insts, items, n_items, queue,
+                       -1.0f);
+  else
+    hipLaunchKernelGGL(k_pair_mfma<false>, g, dim3(64), 0, st, insts, items, n_items, queue,
+                       -1.0f);
   return hipGetLastError();
 }
 

@@ -119,7 +119,8 @@ struct InstScratch {
   uint32_t flags;          // INST_F_* decided by the plan kernel
   uint32_t ep_nonzero;     // 1 if any ray has Ep != 0
   uint32_t n_differs;      // 1 if some ray's normal differs from ray 0's
-  uint32_t amax_bits;      // max |cs| of the FP32 ray records (float bits)
+  uint32_t amax_bits;      // max of max(|cs|, |pr| + |pi|) over the FP32 ray
+                           // records (float bits)
   double n0x, n0y, n0z;    // normal of ray 0
   double v0_rev, v2_rev;   // bounds of the reduced phase of the two variants
   uint32_t qlo_key[3];     // min q_x, q_y, q_z (order-preserving float keys)
@@ -191,7 +192,8 @@ struct KInst {
 static_assert(sizeof(KInst) % 16 == 0, "KInst size");
 
 // One unit of pair-kernel work: a block of slots of one instance against one
-// chunk of its rays.  A workgroup of W waves handles W * 64 * P slots.
+// chunk of its rays.  A workgroup of W waves handles W * 64 * P slots.  MFMA
+// items: pix0 is the 64 x 64-pixel block and pad the sum (0 = Es, 1 = Ep).
 struct WorkItem {
   uint32_t inst, pix0, chunk, pad;
 };

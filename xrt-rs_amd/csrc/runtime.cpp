@@ -335,15 +335,18 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
   const size_t a_i64 = align_up(sizeof(WorkItem) * std::max<size_t>(1, cap64), 256);
   const size_t a_imf = align_up(sizeof(WorkItem) * std::max<size_t>(1, capmf), 256);
   const size_t a_lst = align_up(sizeof(uint32_t) * (lists.size() + 1), 256);
-  e = meta.ensure(a_ins + a_i32 + a_i64 + a_imf + a_lst + 256);
+  e = meta.ensure(a_ins + a_i32 + a_i64 + 2 * a_imf + a_lst + 256);
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* M = meta.as<char>();
   KInst* d_insts = reinterpret_cast<KInst*>(M);
   WorkItem* d_items32 = reinterpret_cast<WorkItem*>(M + a_ins);
   WorkItem* d_items64 = reinterpret_cast<WorkItem*>(M + a_ins + a_i32);
-  WorkItem* d_items_mfma = reinterpret_cast<WorkItem*>(M + a_ins + a_i32 + a_i64);
-  uint32_t* d_lists = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_imf);
-  uint32_t* d_queue = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + a_imf + a_lst);
+  // the MFMA kernels' items: the Es pass, then the Ep pass
+  WorkItem* d_items_mf[2] = {reinterpret_cast<WorkItem*>(M + a_ins + a_i32 + a_i64),
+                             reinterpret_cast<WorkItem*>(M + a_ins + a_i32 + a_i64 + a_imf)};
+  uint32_t* d_lists = reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + 2 * a_imf);
+  uint32_t* d_queue =
+      reinterpret_cast<uint32_t*>(M + a_ins + a_i32 + a_i64 + 2 * a_imf + a_lst);
   if (upload) {
     if (n) e = hipMemcpyAsync(d_insts, insts.data(), sizeof(KInst) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !lists.empty())
@@ -398,7 +401,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     have_flags = true;
   }
   // ---- work items
-  std::vector<WorkItem> items32, items64, items_mfma;
+  std::vector<WorkItem> items32, items64, items_mf[2];
   for (size_t i = 0; i < n; ++i) {
     const InstSpec& s = specs[i];
     const Lay& L = lay[i];
@@ -413,9 +416,14 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
         for (uint32_t p0 = 0; p0 < s.npad; p0 += B)
           items.push_back(WorkItem{(uint32_t)i, p0, c, 0});
     if (mfma_items) {
+      // the Es pass, and the Ep pass unless every ray has Ep = 0 (without
+      // routing both: an Ep item of such an instance returns at once)
       const uint32_t nb = ((s.gnx + 63u) / 64u) * ((s.gnz + 63u) / 64u);
-      for (uint32_t c = 0; c < L.n_chunks; ++c)
-        for (uint32_t b = 0; b < nb; ++b) items_mfma.push_back(WorkItem{(uint32_t)i, b, c, 0});
+      const uint32_t passes = (have_flags && (flags[i] & INST_F_EPZERO)) ? 1u : 2u;
+      for (uint32_t pass = 0; pass < passes; ++pass)
+        for (uint32_t c = 0; c < L.n_chunks; ++c)
+          for (uint32_t b = 0; b < nb; ++b)
+            items_mf[pass].push_back(WorkItem{(uint32_t)i, b, c, pass});
     }
   }
   // Batched launches: longest items first so the queue drains evenly.
@@ -429,7 +437,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     };
     std::stable_sort(items32.begin(), items32.end(), by_len);
     std::stable_sort(items64.begin(), items64.end(), by_len);
-    std::stable_sort(items_mfma.begin(), items_mfma.end(), by_len);
+    for (auto& v : items_mf) std::stable_sort(v.begin(), v.end(), by_len);
   }
   if (upload) {
     if (!items32.empty())
@@ -438,9 +446,10 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     if (e == hipSuccess && !items64.empty())
       e = hipMemcpyAsync(d_items64, items64.data(), sizeof(WorkItem) * items64.size(),
                          hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !items_mfma.empty())
-      e = hipMemcpyAsync(d_items_mfma, items_mfma.data(), sizeof(WorkItem) * items_mfma.size(),
-                         hipMemcpyHostToDevice, st);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      if (!items_mf[k].empty())
+        e = hipMemcpyAsync(d_items_mf[k], items_mf[k].data(), sizeof(WorkItem) * items_mf[k].size(),
+                           hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return map_err(e);
   }
   // ---- pair kernels
@@ -460,10 +469,12 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                     grid_of(cfg32, MODE_F32R, items32.size()), remap, st);
     if (e != hipSuccess) return map_err(e);
   }
-  if (!items_mfma.empty()) {
+  for (int k = 0; k < 2; ++k) {
+    if (items_mf[k].empty()) continue;
     // one wave per group, up to two groups per SIMD resident
-    const uint32_t g = std::min<uint32_t>((uint32_t)items_mfma.size(), (uint32_t)ctx->n_cu * 8u);
-    e = launch_mfma(d_insts, d_items_mfma, (uint32_t)items_mfma.size(), d_queue + 32, g, st);
+    const uint32_t g = std::min<uint32_t>((uint32_t)items_mf[k].size(), (uint32_t)ctx->n_cu * 8u);
+    e = launch_mfma(d_insts, d_items_mf[k], (uint32_t)items_mf[k].size(), d_queue + 32 + k, g,
+                    k == 1, st);
     if (e != hipSuccess) return map_err(e);
   }
   if (!items64.empty()) {
@@ -489,7 +500,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     plan_out->cfg64 = cfg64;
     plan_out->n_items32 = (uint32_t)items32.size();
     plan_out->n_items64 = (uint32_t)items64.size();
-    plan_out->n_items_mfma = (uint32_t)items_mfma.size();
+    plan_out->n_items_mfma = (uint32_t)(items_mf[0].size() + items_mf[1].size());
     plan_out->n_f64 = n_f64;
     plan_out->n_inst = (uint32_t)n;
     plan_out->scr0 = n ? reinterpret_cast<const InstScratch*>(W + lay[0].scr) : nullptr;

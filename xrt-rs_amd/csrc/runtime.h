@@ -49,9 +49,10 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c);
 // Block items (inst, block, chunk) of grid instances on the matrix cores,
 // persistent groups of one wave on queue[0]; items of instances the planner
-// did not give the MFMA flag are skipped.
+// did not give the MFMA flag are skipped.  ep_pass: the kernel of the Ep
+// sums (items with pad = 1), else that of the Es sums (pad = 0).
 hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
-                       uint32_t* queue, uint32_t grid, hipStream_t st);
+                       uint32_t* queue, uint32_t grid, bool ep_pass, hipStream_t st);
 // Pair-kernel launches still to be failed on purpose (xrtk_debug_fail_launches):
 // launch_pair takes one and returns hipErrorLaunchFailure before enqueuing
 // anything.
@@ -154,7 +155,7 @@ bool mfma_allowed(const InstSpec& s, uint32_t force_flags);
 
 struct Plan {
   LaunchCfg cfg32, cfg64;
-  uint32_t n_items32 = 0, n_items64 = 0, n_items_mfma = 0;
+  uint32_t n_items32 = 0, n_items64 = 0, n_items_mfma = 0;  // (MFMA: both passes)
   uint32_t n_f64 = 0;     // instances on the FP64 kernel
   uint32_t n_inst = 0;
   const InstScratch* scr0 = nullptr;  // device scratch of the first instance

@@ -136,6 +136,7 @@ struct Req {
   hipEvent_t prep_ev = nullptr;
   bool routed = true;
   uint32_t mblocks = 0;  // 64 x 64-pixel blocks of the screen (MFMA items)
+  uint32_t mpasses = 1;  // MFMA sums: Es, and Ep unless every ray has Ep = 0
   bool mfma = false;
   std::shared_ptr<std::atomic<int>> users;  // pending requests on the ray set
 };
@@ -149,7 +150,7 @@ struct Launch {
   void* hmeta = nullptr; // pinned host staging of the same bytes
   size_t meta_size = 0;
   std::vector<KInst> h_insts;
-  std::vector<WorkItem> h_items32, h_items64, h_items_mf;
+  std::vector<WorkItem> h_items32, h_items64, h_items_mf[2];
 };
 
 }  // namespace
@@ -395,9 +396,12 @@ struct xrtk_server {
     uint32_t fl = 0;
     std::memcpy(&fl, r->hblk, sizeof(fl));
     if (fl & INST_F_MFMA) {
+      // items: (pass, chunk, block); an item forms one of the two sums, so
+      // it counts its pairs once per pass
       r->mfma = true;
-      r->n_items = (uint64_t)r->n_chunks * r->mblocks;
-      r->item_pairs = 64.0 * 64.0 * r->chunk_len;
+      r->mpasses = (fl & INST_F_EPZERO) ? 1u : 2u;
+      r->n_items = (uint64_t)r->mpasses * r->n_chunks * r->mblocks;
+      r->item_pairs = 64.0 * 64.0 * r->chunk_len / r->mpasses;
     }
   }
 
@@ -432,26 +436,30 @@ struct xrtk_server {
     struct Undo { Req* r; uint64_t next_item; double t2; };
     std::vector<Undo> undo;
     // items with their pair counts, ordered longest first below
-    std::vector<std::pair<double, WorkItem>> it32, it64, itmf;
+    std::vector<std::pair<double, WorkItem>> it32, it64, itmf[2];
     // take up to `max_items` items of r (0: until `limit` pairs are in)
     auto take = [&](Req* r, uint64_t max_items, double limit) {
       undo.push_back(Undo{r, r->next_item, r->t[2]});
       const uint32_t li = (uint32_t)L.h_insts.size();
       L.h_insts.push_back(r->kinst);
-      const uint32_t B = r->mfma ? 64u * 64u : 64u * r->cfg.W * r->cfg.P;
+      const double B = r->mfma ? 64.0 * 64.0 / r->mpasses : 64.0 * r->cfg.W * r->cfg.P;
       const uint32_t nblk = r->mfma ? r->mblocks : r->pblocks;
-      auto& items = r->mfma ? itmf : (r->spec.mode == MODE_F32R ? it32 : it64);
+      const uint64_t per_pass = (uint64_t)r->n_chunks * nblk;
+      auto& items = r->spec.mode == MODE_F32R ? it32 : it64;
       uint64_t taken = 0;
       while (r->next_item < r->n_items &&
              (max_items ? taken < max_items : (pairs < limit || taken == 0))) {
         const uint64_t it = r->next_item++;
-        const uint32_t chunk = (uint32_t)(it / nblk);
-        const uint32_t pb = (uint32_t)(it % nblk);
+        const uint32_t pass = (uint32_t)(it / per_pass);
+        const uint64_t rem = it % per_pass;
+        const uint32_t chunk = (uint32_t)(rem / nblk);
+        const uint32_t pb = (uint32_t)(rem % nblk);
         const double len =
             std::min<double>(r->chunk_len, (double)r->spec.nr - (double)chunk * r->chunk_len);
-        // VALU items start at a slot, MFMA items name a block
-        items.emplace_back((double)B * len, WorkItem{li, r->mfma ? pb : pb * B, chunk, 0});
-        pairs += (double)B * len;
+        // VALU items start at a slot, MFMA items name a block (one list per sum)
+        if (r->mfma) itmf[pass].emplace_back(B * len, WorkItem{li, pb, chunk, pass});
+        else items.emplace_back(B * len, WorkItem{li, pb * (uint32_t)B, chunk, 0});
+        pairs += B * len;
         ++taken;
       }
       if (r->t[2] == 0) r->t[2] = now_s();
@@ -477,15 +485,19 @@ struct xrtk_server {
       out.reserve(v.size());
       for (const auto& x : v) out.push_back(x.second);
     };
-    L.h_items_mf.clear();
     lpt(it32, L.h_items32);
     lpt(it64, L.h_items64);
-    lpt(itmf, L.h_items_mf);
+    for (int k = 0; k < 2; ++k) {
+      L.h_items_mf[k].clear();
+      lpt(itmf[k], L.h_items_mf[k]);
+    }
     // metadata
     const size_t a_ins = au(sizeof(KInst) * L.h_insts.size());
     const size_t a_32 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items32.size()));
     const size_t a_64 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items64.size()));
-    const size_t a_mf = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items_mf.size()));
+    const size_t a_mf0 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items_mf[0].size()));
+    const size_t a_mf1 = au(sizeof(WorkItem) * std::max<size_t>(1, L.h_items_mf[1].size()));
+    const size_t a_mf = a_mf0 + a_mf1;
     const size_t need = a_ins + a_32 + a_64 + a_mf + 256;
     if (need > L.meta_size) {
       if (L.meta) (void)hipFree(L.meta);
@@ -509,12 +521,15 @@ struct xrtk_server {
     std::memcpy(H, L.h_insts.data(), sizeof(KInst) * L.h_insts.size());
     std::memcpy(H + a_ins, L.h_items32.data(), sizeof(WorkItem) * L.h_items32.size());
     std::memcpy(H + a_ins + a_32, L.h_items64.data(), sizeof(WorkItem) * L.h_items64.size());
-    std::memcpy(H + a_ins + a_32 + a_64, L.h_items_mf.data(),
-                sizeof(WorkItem) * L.h_items_mf.size());
+    std::memcpy(H + a_ins + a_32 + a_64, L.h_items_mf[0].data(),
+                sizeof(WorkItem) * L.h_items_mf[0].size());
+    std::memcpy(H + a_ins + a_32 + a_64 + a_mf0, L.h_items_mf[1].data(),
+                sizeof(WorkItem) * L.h_items_mf[1].size());
     KInst* d_ins = reinterpret_cast<KInst*>(M);
     WorkItem* d_32 = reinterpret_cast<WorkItem*>(M + a_ins);
     WorkItem* d_64 = reinterpret_cast<WorkItem*>(M + a_ins + a_32);
-    WorkItem* d_mf = reinterpret_cast<WorkItem*>(M + a_ins + a_32 + a_64);
+    WorkItem* d_mf[2] = {reinterpret_cast<WorkItem*>(M + a_ins + a_32 + a_64),
+                         reinterpret_cast<WorkItem*>(M + a_ins + a_32 + a_64 + a_mf0)};
     uint32_t* d_q = reinterpret_cast<uint32_t*>(M + a_ins + a_32 + a_64 + a_mf);
     hipError_t e =
         hipMemcpyAsync(M, H, a_ins + a_32 + a_64 + a_mf, hipMemcpyHostToDevice, stream);
@@ -527,10 +542,12 @@ struct xrtk_server {
     if (e == hipSuccess && !L.h_items32.empty())
       e = launch_pair(MODE_F32R, c32, d_ins, d_32, (uint32_t)L.h_items32.size(), d_q,
                       grid_of(c32, MODE_F32R, L.h_items32.size()), 0, stream);
-    if (e == hipSuccess && !L.h_items_mf.empty())
-      e = launch_mfma(d_ins, d_mf, (uint32_t)L.h_items_mf.size(), d_q + 32,
-                      std::min<uint32_t>((uint32_t)L.h_items_mf.size(), (uint32_t)ctx->n_cu * 8u),
-                      stream);
+    for (int k = 0; k < 2; ++k)
+      if (e == hipSuccess && !L.h_items_mf[k].empty())
+        e = launch_mfma(d_ins, d_mf[k], (uint32_t)L.h_items_mf[k].size(), d_q + 32 + k,
+                        std::min<uint32_t>((uint32_t)L.h_items_mf[k].size(),
+                                           (uint32_t)ctx->n_cu * 8u),
+                        k == 1, stream);
     if (e == hipSuccess && !L.h_items64.empty())
       e = launch_pair(MODE_F64, c64, d_ins, d_64, (uint32_t)L.h_items64.size(), d_q + 16,
                       grid_of(c64, MODE_F64, L.h_items64.size()), 0, stream);
