@@ -1540,12 +1540,28 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   const RayRec32* rays = I.rays32;  // one ray per lane: vector loads
   const uint32_t r0 = w.chunk * I.chunk_len;
   const uint32_t r1 = min(r0 + I.chunk_len, I.nr);
-  // per-(ray, block) values of the 64 rays from b0, one ray per lane, in
+  // The fields of one ray record that the block values need, loaded one LDS
+  // block ahead of their use so the load has four steps to arrive.
+  struct RawRay {
+    float ox, oy, oz, oo, amp, ampi, off, dx, dy;
+    bool ok;
+  };
+  auto load = [&](uint32_t b0) {
+    RawRay q;
+    q.ok = b0 + lane < r1;
+    const RayRec32* p = rays + (q.ok ? b0 + lane : r0);  // in bounds either way
+    const float4 o = *reinterpret_cast<const float4*>(&p->ox);
+    q.ox = o.x; q.oy = o.y; q.oz = o.z; q.oo = o.w;
+    q.amp = EPP ? p->pr : p->cs;
+    q.ampi = EPP ? p->pi : 0.f;
+    q.off = p->off; q.dx = p->dx; q.dy = p->dy;
+    return q;
+  };
+  // per-(ray, block) values of the 64 rays of `rr`, one ray per lane, in
   // FP64, into LDS buffer `buf` (zero amplitude past r1)
-  auto prologue = [&](uint32_t b0, uint32_t buf) {
+  auto prologue = [&](const RawRay& rr, uint32_t buf) {
     float v[MR_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f, 0.f};
-    if (b0 + lane < r1) {
-      const RayRec32 rr = rays[b0 + lane];
+    if (rr.ok) {
       const float ok[3] = {rr.ox, rr.oy, rr.oz};
       const float dk[3] = {rr.dx, rr.dy, rr.oo};  // TILE: oo holds the third delta
       double o[3], dl[3];
@@ -1572,8 +1588,8 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
       tz -= rint(tz);
       v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
       v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
-      v[MR_AMP] = (EPP ? rr.pr : rr.cs) * scale;
-      v[MR_AMPI] = EPP ? rr.pi * scale : 0.f;
+      v[MR_AMP] = rr.amp * scale;
+      v[MR_AMPI] = rr.ampi * scale;
       v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
       v[MR_RZC] = __builtin_amdgcn_cosf((float)tz); v[MR_RZS] = __builtin_amdgcn_sinf((float)tz);
     }
@@ -1593,13 +1609,15 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
     return (const float(*)[kBlockRays])lds[(k >> 2) & 1u];
   };
   auto step_kb = [&](uint32_t k) { return (k & 3u) * 16u + 8u * half; };
-  prologue(r0, 0u);
+  prologue(load(r0), 0u);
+  RawRay next = load(r0 + kBlockRays);
   __syncthreads();
   MFrag f0, f1;
   mfma_gen<EPP>(step_ptr(0u), step_kb(0u), u0, m1, f0);
   for (uint32_t ks = 0; ks < nsteps; ks += 2u) {
     if ((ks & 3u) == 0u && (ks >> 2) + 1u < nblk) {
-      prologue(r0 + ((ks >> 2) + 1u) * kBlockRays, ((ks >> 2) + 1u) & 1u);
+      prologue(next, ((ks >> 2) + 1u) & 1u);
+      next = load(r0 + ((ks >> 2) + 2u) * kBlockRays);
       __syncthreads();
     }
     mfma_gen<EPP>(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, f1);
