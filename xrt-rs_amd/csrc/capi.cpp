@@ -137,6 +137,74 @@ void grid_dq(const XrtkWaveParams& w, int P, double dq[3]) {
   dq[2] = pz * tw;
 }
 
+// A float32 pixel list on a grid: the grid (prepare_wave's parameters), the
+// offsets of its rows and columns from it in pixels, and their largest size.
+struct ListGrid {
+  XrtkWaveParams w;
+  std::vector<float> ex, ez;
+  double emax = 0.0;
+};
+
+// Whether a float32 pixel list lies on an nx x nz grid in the prepare_wave
+// order (src/waves/prepare_wave.rs:40-68: entry ix * nz + iz, x a function of
+// ix alone, z of iz alone, y the same everywhere), nx, nz >= 2, as prepare_wave
+// gives it once rounded to float32.  The grid is the least-squares fit of the
+// rows' and the columns' positions; each row's and column's float32 position
+// is taken as an offset from it (the rounding, or any spacing that departs
+// from it by at most 1/100 of a pixel).
+bool fit_list_grid(const XrtkGpuPixel* p, size_t np, ListGrid& g) {
+  if (np < 4) return false;
+  size_t nz = 1;
+  while (nz < np && p[nz].x == p[0].x) ++nz;
+  if (nz < 2 || np % nz) return false;
+  const size_t nx = np / nz;
+  if (nx < 2) return false;
+  const float y = p[0].y;
+  for (size_t ix = 0; ix < nx; ++ix) {
+    const XrtkGpuPixel* r = p + ix * nz;
+    for (size_t iz = 0; iz < nz; ++iz)
+      if (r[iz].x != r[0].x || r[iz].z != p[iz].z || r[iz].y != y) return false;
+  }
+  // least-squares line through the rows' (columns') positions: its centre
+  // and half span are the grid's
+  auto fit = [](size_t n, auto at, double* centre, double* half) {
+    const double im = 0.5 * (double)(n - 1);
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) m += at(i);
+    m /= (double)n;
+    double sxy = 0.0, sxx = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      const double d = (double)i - im;
+      sxy += d * (at(i) - m);
+      sxx += d * d;
+    }
+    *centre = m;
+    *half = sxy / sxx * im;
+  };
+  fit(nx, [&](size_t i) { return (double)p[i * nz].x; }, &g.w.center[0], &g.w.dx);
+  fit(nz, [&](size_t i) { return (double)p[i].z; }, &g.w.center[2], &g.w.dz);
+  g.w.center[1] = y;
+  g.w.nx = (uint32_t)nx;
+  g.w.nz = (uint32_t)nz;
+  if (g.w.dx == 0.0 || g.w.dz == 0.0) return false;
+  InstSpec t;  // the grid exactly as the kernels form it
+  spec_set_grid(t, g.w, 1, 1);
+  g.emax = 0.0;
+  g.ex.resize(nx);
+  g.ez.resize(nz);
+  for (size_t ix = 0; ix < nx; ++ix) {
+    const double e = ((double)p[ix * nz].x - (t.gx0 + (double)ix * t.gsx)) / t.gsx;
+    g.emax = std::max(g.emax, std::fabs(e));
+    g.ex[ix] = (float)e;
+  }
+  for (size_t iz = 0; iz < nz; ++iz) {
+    const double e = ((double)p[iz].z - (t.gz0 + (double)iz * t.gsz)) / t.gsz;
+    g.emax = std::max(g.emax, std::fabs(e));
+    g.ez[iz] = (float)e;
+  }
+  return g.emax <= 0.01;
+}
+
 void record_timing(xrtk_ctx* ctx) {
   float a = 0, b = 0;
   if (hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]) != hipSuccess) a = 0;
@@ -238,8 +306,15 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   ph.next("xrtk.plan");
   const uint32_t want = mode_from(opts);
   const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
+  // A float32 list on a grid is laid out as that grid, so that it may run on
+  // the matrix cores (XRTK_ENGINE_VALU keeps the list layout).
+  ListGrid lg;
+  const bool list_grid = ps.f32 && want == MODE_F32R && nr &&
+                         !(opts && opts->engine == XRTK_ENGINE_VALU) &&
+                         fit_list_grid(ps.f32, np, lg);
+  const XrtkWaveParams* grid = list_grid ? &lg.w : ps.grid;
   // the F32R shape fixes the slot layout even when only the FP64 kernel runs
-  const bool f32_pixels = ps.f32 != nullptr;
+  const bool f32_pixels = ps.f32 != nullptr && !list_grid;
   LaunchCfg c32 =
       choose_cfg(ctx, MODE_F32R, nr, np, want == MODE_F32R ? opts : nullptr, f32_pixels);
   LaunchCfg c64 = cfg64_for(c32);
@@ -257,10 +332,25 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   Box pbox;
   double dq[3] = {0, 0, 0};
   uint32_t npad;
-  if (ps.grid) {
-    npad = grid_slots(ps.grid->nx, ps.grid->nz, c32, c64);
-    pbox = grid_box(*ps.grid);
-    grid_dq(*ps.grid, c32.P, dq);
+  // (a list on a grid: its own box, and tiles wider by its offsets)
+  auto list_box = [&]() {
+    for (size_t i = 0; i < np; ++i) {
+      double v[3];
+      ps.get(i, v);
+      pbox.add(v[0], v[1], v[2]);
+    }
+  };
+  auto tile_dq = [&](int P, double* d) {
+    grid_dq(*grid, P, d);
+    if (list_grid) {
+      d[0] += 2.0 * lg.emax * 2.0 * std::fabs(grid->dx) / (grid->nx - 1);
+      d[2] += 2.0 * lg.emax * 2.0 * std::fabs(grid->dz) / (grid->nz - 1);
+    }
+  };
+  if (grid) {
+    npad = grid_slots(grid->nx, grid->nz, c32, c64);
+    if (list_grid) list_box(); else pbox = grid_box(*grid);
+    tile_dq(c32.P, dq);
   } else {
     npad = padded_slots(np, c32, c64);
     for (size_t i = 0; i < np; ++i) {
@@ -297,9 +387,9 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     // groups off the FP64 kernel: the routing bound grows with the tile's
     // extent (near-field screens at P = 16 can exceed it where P = 4 does
     // not).
-    if (ps.grid && !cfg_shape_given(ctx, MODE_F32R, nr, np, opts)) {
+    if (grid && !cfg_shape_given(ctx, MODE_F32R, nr, np, opts)) {
       auto n64 = [&](int P, double* d) {
-        grid_dq(*ps.grid, P, d);
+        tile_dq(P, d);
         int n = 0;
         for (size_t g = 0; g < gs.size(); ++g)
           n += route_mode(MODE_F32R, boxes[g], pbox, d, gs[g].energy) == MODE_F64;
@@ -317,7 +407,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
         for (size_t g = 0; g < gs.size(); ++g)
           gs[g].mode = route_mode(MODE_F32R, boxes[g], pbox, dq, gs[g].energy);
         c64 = cfg64_for(c32);
-        npad = grid_slots(ps.grid->nx, ps.grid->nz, c32, c64);
+        npad = grid_slots(grid->nx, grid->nz, c32, c64);
       }
     }
   } else if (nr) {
@@ -327,8 +417,10 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   ph.next("xrtk.upload");
   const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
                                   : align256(13 * sizeof(double) * (size_t)nr);
-  const size_t pix_stage = ps.grid ? 0 : (ps.f32 ? sizeof(XrtkGpuPixel) * np
-                                                 : 3 * sizeof(double) * (size_t)np);
+  const size_t pix_stage =
+      ps.grid ? 0
+      : ps.f32 ? sizeof(XrtkGpuPixel) * np + (list_grid ? sizeof(float) * (lg.ex.size() + lg.ez.size()) : 0)
+               : 3 * sizeof(double) * (size_t)np;
   hipError_t e = ws.stage.ensure(ray_stage + pix_stage + 256);
   if (e == hipSuccess) e = ws.rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
   if (e == hipSuccess) e = ws.pix64.ensure(sizeof(PixF64) * npad);
@@ -358,7 +450,26 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   }
   // ---- pixels
   const uint32_t* d_perm = nullptr;
-  if (ps.grid) {
+  const float *d_ex = nullptr, *d_ez = nullptr;
+  if (list_grid) {
+    uint32_t th, tw;
+    grid_tile(c32.P, grid->nx, grid->nz, &th, &tw);
+    char* L = S + ray_stage;
+    float* X = reinterpret_cast<float*>(L + sizeof(XrtkGpuPixel) * np);
+    d_ex = X;
+    d_ez = X + lg.ex.size();
+    e = hipMemcpyAsync(L, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(X, lg.ex.data(), sizeof(float) * lg.ex.size(), hipMemcpyHostToDevice,
+                         ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(X + lg.ex.size(), lg.ez.data(), sizeof(float) * lg.ez.size(),
+                         hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_pix_grid_list(L, grid->nx, grid->nz, th, tw, npad, ws.pix64.as<PixF64>(),
+                               ws.perm.as<uint32_t>(), ctx->stream);
+    d_perm = ws.perm.as<uint32_t>();
+  } else if (ps.grid) {
     uint32_t th, tw;
     grid_tile(c32.P, ps.grid->nx, ps.grid->nz, &th, &tw);
     e = launch_pix_grid(ps.grid->center, ps.grid->dx, ps.grid->dz, ps.grid->nx, ps.grid->nz, th,
@@ -399,10 +510,15 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     s.skip = skip;
     s.out = ws.outbuf.as<double>();
     s.accumulate = gi > 0;
-    if (ps.grid && np) {
+    if (grid && np) {
       uint32_t th, tw;
-      grid_tile(c32.P, ps.grid->nx, ps.grid->nz, &th, &tw);
-      spec_set_grid(s, *ps.grid, th, tw);
+      grid_tile(c32.P, grid->nx, grid->nz, &th, &tw);
+      spec_set_grid(s, *grid, th, tw);
+      if (list_grid) {
+        s.gex = d_ex;
+        s.gez = d_ez;
+        s.gemax = lg.emax;
+      }
     }
     specs.push_back(s);
   }

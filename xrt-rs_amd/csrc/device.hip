@@ -299,6 +299,26 @@ __global__ void k_pix_grid(double cx, double cy, double cz, double dx, double dz
   perm[s] = real ? ix * nz + iz : ~0u;
 }
 
+// The slots of a float32 pixel list that lies on an nx x nz grid in the
+// prepare_wave order (entry ix * nz + iz): k_pix_grid's layout, each slot
+// holding the list's own point.
+__global__ void k_pix_grid_list(const float4* __restrict__ list, uint32_t nx, uint32_t nz,
+                                uint32_t th, uint32_t tw, uint32_t npad,
+                                PixF64* __restrict__ dst, uint32_t* __restrict__ perm) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  const uint32_t tsz = th * tw;
+  const uint32_t ntz = (nz + tw - 1) / tw;
+  const uint32_t t = s / tsz, r = s - t * tsz;
+  const uint32_t tix = t / ntz, tiz = t - tix * ntz;
+  uint32_t ix = tix * th + r / tw, iz = tiz * tw + r % tw;
+  const bool real = ix < nx && iz < nz;
+  if (!real) { ix = min(ix, nx - 1u); iz = min(iz, nz - 1u); }
+  const float4 v = list[ix * nz + iz];
+  dst[s] = PixF64{v.x, v.y, v.z, 0.0};
+  perm[s] = real ? ix * nz + iz : ~0u;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 centroid (fixed-order tree, deterministic) and bounds
 // ---------------------------------------------------------------------------
@@ -676,7 +696,7 @@ __global__ __launch_bounds__(256) void k_pix_bounds(const PixF64* __restrict__ p
 // EPZERO.
 __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, uint32_t mode,
                        uint32_t force_flags, uint32_t static_flags, uint32_t P, double gsx,
-                       double gsz) {
+                       double gsz, double gemax) {
   const double ox = __uint_as_float(scr->omax_bits[0]), oy = __uint_as_float(scr->omax_bits[1]);
   const double oz = __uint_as_float(scr->omax_bits[2]), om = __uint_as_float(scr->omax_bits[3]);
   const double qx = __uint_as_float(scr->qmax_bits[0]), qy = __uint_as_float(scr->qmax_bits[1]);
@@ -765,10 +785,15 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     // pixel along x, |df . step| = kr |gsx| |q_x (1/|d| - 1/|q|) - o_x/|d||
     // <= kr |gsx| (Q_x dR / Rl^2 + o_x / Rl) (likewise z), times 16 from a
     // tile centre plus the half revolution of the anchor, <= 8 rev.
+    // A float32 pixel list on the grid (offsets |e| <= gemax pixels from it)
+    // widens the block to h = (32 + gemax) |(gsx, gsz)|; the kernel applies
+    // the offsets of the second row and column tile to first order, which
+    // leaves (2 pi smax 2 gemax)^2 / 2 per term, held below 2 pi 1e-7 by
+    // smax gemax <= 8.9e-5.
     const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP;
     if ((static_flags & INST_F_MFMA_OK) && (f & need_m) == need_m && !(force_flags & 0x20000u)) {
       const double ax = fabs(gsx), az = fabs(gsz);
-      const double h = 32.0 * sqrt(ax * ax + az * az);
+      const double h = (32.0 + gemax) * sqrt(ax * ax + az * az);
       const double Rl = rmin - om - h;
       if (Rl > 0.0) {
         const double Qx = qx + h, Qy = qy, Qz = qz + h;
@@ -780,7 +805,7 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         const double E = 0.5 * kr * (t1 + t2) * h * h;
         const double smax = kr * fmax(ax * (Qx * t1 + ox / Rl), az * (Qz * t1 + oz / Rl));
         scr->blk_rev = E;
-        if (E <= 1e-6 && 16.0 * smax + 0.5 <= 8.0) f |= INST_F_MFMA;
+        if (E <= 1e-6 && 16.0 * smax + 0.5 <= 8.0 && smax * gemax <= 8.9e-5) f |= INST_F_MFMA;
       }
     }
   }
@@ -1427,12 +1452,20 @@ __device__ __forceinline__ xh8 as_h8(const uint32_t (&w)[4]) {
   return __builtin_bit_cast(xh8, v);
 }
 
+// Offsets of a float32 pixel list from the grid, for this lane's row and
+// column (in pixels): ex, ez of tile 0; cx, cz = 2 pi (offset of tile 1 -
+// offset of tile 0).
+struct LaneOff {
+  float ex, ez, cx, cz;
+};
+
 // Factors of this lane for the 8 rays kb .. kb + 7 of the LDS block: row
 // (grid x) u0 of row tile 0 and, by one rotation, of row tile 1; column u0
-// likewise.
-template <bool EPP>
+// likewise.  OFF: rows and columns at u0 + offset, the rotation to tile 1
+// followed by a first-order turn by the difference of the two offsets.
+template <bool EPP, bool OFF>
 __device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_t kb, float u0,
-                                         float m1, MFrag& f) {
+                                         float m1, const LaneOff& lo, MFrag& f) {
 #pragma unroll
   for (int jp = 0; jp < 4; ++jp) {
     float fr[2][2], fi[2][2], gr[2][2], gi[2][2];  // [tile][ray of the pair]
@@ -1444,7 +1477,7 @@ __device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_
       const float amp = lds[MR_AMP][j];
       const float rxc = lds[MR_RXC][j], rxs = lds[MR_RXS][j];
       const float rzc = lds[MR_RZC][j], rzs = lds[MR_RZS][j];
-      const float a = fmaf(u0, sx, ph);
+      const float a = fmaf(u0, sx, OFF ? fmaf(lo.ex, sx, ph) : ph);
       float c, s;
       if constexpr (EPP) {
         // (pr + i pi) e^{2 pi i a}
@@ -1459,11 +1492,19 @@ __device__ __forceinline__ void mfma_gen(const float (*lds)[kBlockRays], uint32_
       fr[0][q] = c; fi[0][q] = s;
       fr[1][q] = fmaf(c, rxc, -(s * rxs));
       fi[1][q] = fmaf(c, rxs, s * rxc);
-      const float b = fmaf(u0, sz, gz);
+      const float b = fmaf(u0, sz, OFF ? fmaf(lo.ez, sz, gz) : gz);
       const float gc = __builtin_amdgcn_cosf(b), gs = __builtin_amdgcn_sinf(b);
       gr[0][q] = gc; gi[0][q] = gs;
       gr[1][q] = fmaf(gc, rzc, -(gs * rzs));
       gi[1][q] = fmaf(gc, rzs, gs * rzc);
+      if constexpr (OFF) {
+        const float tx = lo.cx * sx, tz = lo.cz * sz;
+        const float r1 = fr[1][q], g1 = gr[1][q];
+        fr[1][q] = fmaf(-tx, fi[1][q], r1);
+        fi[1][q] = fmaf(tx, r1, fi[1][q]);
+        gr[1][q] = fmaf(-tz, gi[1][q], g1);
+        gi[1][q] = fmaf(tz, g1, gi[1][q]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -1504,7 +1545,7 @@ __device__ __forceinline__ void mfma_step(const MFrag& f, xf16 (&acc)[2][2][2]) 
   }
 }
 
-template <bool EPP>
+template <bool EPP, bool OFF>
 __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
                                           float (*lds)[MR_N][kBlockRays], uint32_t lane,
                                           float m1) {
@@ -1527,6 +1568,16 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
 
   const uint32_t half = lane >> 5;
   const float u0 = (float)(lane & 31u) - 15.5f;  // offset from a tile's centre
+  LaneOff lo = {0.f, 0.f, 0.f, 0.f};
+  if (OFF && I.gex) {
+    // this lane's row ix0 + m and column iz0 + m (m = lane & 31) of tile 0
+    // and of tile 1 (32 further; rows past the screen take offset 0)
+    const uint32_t m = lane & 31u;
+    auto off = [](const float* e, uint32_t i, uint32_t n) { return i < n ? e[i] : 0.f; };
+    const float x0 = off(I.gex, ix0 + m, I.gnx), x1 = off(I.gex, ix0 + 32u + m, I.gnx);
+    const float z0 = off(I.gez, iz0 + m, I.gnz), z1 = off(I.gez, iz0 + 32u + m, I.gnz);
+    lo = LaneOff{x0, z0, 6.283185307179586f * (x1 - x0), 6.283185307179586f * (z1 - z0)};
+  }
   xf16 acc[2][2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -1613,14 +1664,14 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
   RawRay next = load(r0 + kBlockRays);
   __syncthreads();
   MFrag f0, f1;
-  mfma_gen<EPP>(step_ptr(0u), step_kb(0u), u0, m1, f0);
+  mfma_gen<EPP, OFF>(step_ptr(0u), step_kb(0u), u0, m1, lo, f0);
   for (uint32_t ks = 0; ks < nsteps; ks += 2u) {
     if ((ks & 3u) == 0u && (ks >> 2) + 1u < nblk) {
       prologue(next, ((ks >> 2) + 1u) & 1u);
       next = load(r0 + ((ks >> 2) + 2u) * kBlockRays);
       __syncthreads();
     }
-    mfma_gen<EPP>(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, f1);
+    mfma_gen<EPP, OFF>(step_ptr(ks + 1u), step_kb(ks + 1u), u0, m1, lo, f1);
     mfma_step(f0, acc);
     __builtin_amdgcn_sched_group_barrier(0x100, 18, 0);  // the LDS reads first
 #pragma unroll
@@ -1629,7 +1680,7 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
       __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // then VALU
     }
     const uint32_t kn = ks + 2u < nsteps ? ks + 2u : ks + 1u;  // (the last is unused)
-    mfma_gen<EPP>(step_ptr(kn), step_kb(kn), u0, m1, f0);
+    mfma_gen<EPP, OFF>(step_ptr(kn), step_kb(kn), u0, m1, lo, f0);
     mfma_step(f1, acc);
     __builtin_amdgcn_sched_group_barrier(0x100, 18, 1);
 #pragma unroll
@@ -1673,7 +1724,9 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
 // dry; items of instances without the MFMA flag are skipped.  EPP: the Ep
 // pass (its items, pad = 1), which instances with Ep = 0 everywhere skip;
 // each pass is its own kernel, so each has the registers of its own code.
-template <bool EPP>
+// OFF: the kernel of float32 pixel lists on a grid (instances with offsets;
+// null offsets are zeros).
+template <bool EPP, bool OFF>
 __global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ insts,
                                                   const WorkItem* __restrict__ items,
                                                   uint32_t n_items, uint32_t* __restrict__ queue,
@@ -1689,9 +1742,9 @@ __global__ __launch_bounds__(64) void k_pair_mfma(const KInst* __restrict__ inst
     const uint32_t f = ((const XRTK_CONST InstScratch*)I.scr)->flags;
     if (!(f & INST_F_MFMA)) continue;
     if constexpr (EPP) {
-      if (!(f & INST_F_EPZERO)) mfma_item<true>(I, w, lds, lane, m1);
+      if (!(f & INST_F_EPZERO)) mfma_item<true, OFF>(I, w, lds, lane, m1);
     } else {
-      mfma_item<false>(I, w, lds, lane, m1);
+      mfma_item<false, OFF>(I, w, lds, lane, m1);
     }
   }
 }
@@ -1926,13 +1979,22 @@ hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx,
   return hipGetLastError();
 }
 
+hipError_t launch_pix_grid_list(const void* list, uint32_t nx, uint32_t nz, uint32_t th,
+                                uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
+                                hipStream_t st) {
+  if (!npad) return hipSuccess;
+  hipLaunchKernelGGL(k_pix_grid_list, dim3(cdiv(npad, 256)), dim3(256), 0, st,
+                     static_cast<const float4*>(list), nx, nz, th, tw, npad, dst, perm);
+  return hipGetLastError();
+}
+
 // Centroid, bounds, plan and the per-solve FP32 records of one instance.
 // part: kRedBlocks * 4 doubles of scratch.  P is the pixels-per-lane of the
 // F32R launch (tile = 64 * P slots; npad a multiple of it).
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
                        const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
                        uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
-                       uint32_t static_flags, int P, double gsx, double gsz,
+                       uint32_t static_flags, int P, double gsx, double gsz, double gemax,
                        RayRec32* rays32, PixRec32* pix32, TileRec* tiles, double* R64,
                        hipStream_t st) {
   const unsigned nb = nr ? std::min<unsigned>(kRedBlocks, cdiv(nr, 256)) : 1u;
@@ -1961,7 +2023,7 @@ hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uin
     }
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, st, scr, skip, kr, mode, force_flags,
-                     static_flags, (uint32_t)P, gsx, gsz);
+                     static_flags, (uint32_t)P, gsx, gsz, gemax);
   if (mode == MODE_F32R && nr)
     hipLaunchKernelGGL(k_ray_prep32, dim3(cdiv(nr, 256)), dim3(256), 0, st, rays, nr, scr, kr,
                        kr_lo, npad ? 1u : 0u, rays32);
@@ -2023,18 +2085,16 @@ hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
 }
 
 hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
-                       uint32_t* queue, uint32_t grid, bool ep_pass, hipStream_t st) {
+                       uint32_t* queue, uint32_t grid, bool ep_pass, bool offsets,
+                       hipStream_t st) {
   if (!n_items) return hipSuccess;
   for (uint32_t f = g_fail_launches.load(); f > 0;)
     if (g_fail_launches.compare_exchange_weak(f, f - 1)) return hipErrorLaunchFailure;
   hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, st, queue, 1u);
   const dim3 g(std::max<uint32_t>(1u, grid));
-  if (ep_pass)
-    hipLaunchKernelGGL(k_pair_mfma<true>, g, dim3(64), 0, st, insts, items, n_items, queue,
-                       -1.0f);
-  else
-    hipLaunchKernelGGL(k_pair_mfma<false>, g, dim3(64), 0, st, insts, items, n_items, queue,
-                       -1.0f);
+  auto* k = ep_pass ? (offsets ? k_pair_mfma<true, true> : k_pair_mfma<true, false>)
+                    : (offsets ? k_pair_mfma<false, true> : k_pair_mfma<false, false>);
+  hipLaunchKernelGGL(k, g, dim3(64), 0, st, insts, items, n_items, queue, -1.0f);
   return hipGetLastError();
 }
 

@@ -185,9 +185,14 @@ struct KInst {
   double skip;              // skip pairs with path < skip [mm]
   double pad3;
   // prepare_wave grid of the screen (MFMA items): pixel (ix, iz) at
-  // (gx0 + ix gsx, gy, gz0 + iz gsz); slots in tiles of gth x gtw pixels
-  double gx0, gy, gz0, gsx, gsz, pad4;
+  // (gx0 + ix gsx, gy, gz0 + iz gsz); slots in tiles of gth x gtw pixels.
+  // A float32 pixel list on such a grid: pixel (ix, iz) at
+  // (gx0 + (ix + gex[ix]) gsx, gy, gz0 + (iz + gez[iz]) gsz), the offsets in
+  // pixels, |offset| <= gemax (null offsets: the grid itself).
+  double gx0, gy, gz0, gsx, gsz, gemax;
   uint32_t gnx, gnz, gth, gtw;
+  const float* gex;
+  const float* gez;
 };
 static_assert(sizeof(KInst) % 16 == 0, "KInst size");
 

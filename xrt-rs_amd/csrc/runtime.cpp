@@ -313,6 +313,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     I.skip = s.skip;
     I.gx0 = s.gx0; I.gy = s.gy; I.gz0 = s.gz0; I.gsx = s.gsx; I.gsz = s.gsz;
     I.gnx = s.gnx; I.gnz = s.gnz; I.gth = s.gth; I.gtw = s.gtw;
+    I.gex = s.gex; I.gez = s.gez; I.gemax = s.gemax;
     if (!f32) ++n_f64;
     const uint32_t B = f32 ? 64u * cfg32.W * cfg32.P : 64u * cfg64.W * cfg64.P;
     (f32 ? cap32 : cap64) += (size_t)L.n_chunks * (s.npad / B);
@@ -375,6 +376,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                     reinterpret_cast<double*>(W + L.part), s.mode, insts[i].kr_hi,
                     insts[i].kr_lo, s.skip, force_flags,
                     s.static_flags | (mf_ok[i] ? INST_F_MFMA_OK : 0u), cfg32.P, s.gsx, s.gsz,
+                    s.gemax,
                     reinterpret_cast<RayRec32*>(W + L.rays32),
                     reinterpret_cast<PixRec32*>(W + L.pix32),
                     reinterpret_cast<TileRec*>(W + L.tiles), reinterpret_cast<double*>(W + L.R64),
@@ -469,12 +471,14 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                     grid_of(cfg32, MODE_F32R, items32.size()), remap, st);
     if (e != hipSuccess) return map_err(e);
   }
+  bool offsets = false;  // float32 pixel lists on a grid
+  for (size_t i = 0; i < n; ++i) offsets = offsets || (mf_ok[i] && specs[i].gex);
   for (int k = 0; k < 2; ++k) {
     if (items_mf[k].empty()) continue;
     // one wave per group, up to two groups per SIMD resident
     const uint32_t g = std::min<uint32_t>((uint32_t)items_mf[k].size(), (uint32_t)ctx->n_cu * 8u);
     e = launch_mfma(d_insts, d_items_mf[k], (uint32_t)items_mf[k].size(), d_queue + 32 + k, g,
-                    k == 1, st);
+                    k == 1, offsets, st);
     if (e != hipSuccess) return map_err(e);
   }
   if (!items64.empty()) {

@@ -36,12 +36,18 @@ hipError_t launch_pix_from_cols(const double* cols, PixF64* dst, uint32_t* perm,
 hipError_t launch_pix_grid(const double c[3], double dx, double dz, uint32_t nx, uint32_t nz,
                            uint32_t th, uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
                            hipStream_t st);
+// k_pix_grid's slots for a float32 pixel list (XrtkGpuPixel) on an nx x nz
+// grid in the prepare_wave order, each slot holding the list's own point.
+hipError_t launch_pix_grid_list(const void* list, uint32_t nx, uint32_t nz, uint32_t th,
+                                uint32_t tw, uint32_t npad, PixF64* dst, uint32_t* perm,
+                                hipStream_t st);
 // perm: slot -> pixel index (~0u for padding), or null when slots [np, npad)
 // are the padding.
 hipError_t launch_prep(const RayRec64* rays, uint32_t nr, const PixF64* pix, uint32_t npad,
                        const uint32_t* perm, uint32_t np, InstScratch* scr, double* part,
                        uint32_t mode, double kr, double kr_lo, double skip, uint32_t force_flags,
-                       uint32_t static_flags, int P, double gsx, double gsz, RayRec32* rays32,
+                       uint32_t static_flags, int P, double gsx, double gsz, double gemax,
+                       RayRec32* rays32,
                        PixRec32* pix32, TileRec* tiles, double* R64, hipStream_t st);
 hipError_t launch_pair(uint32_t mode, const LaunchCfg& cfg, const KInst* insts,
                        const WorkItem* items, uint32_t n_items, uint32_t* queue, uint32_t grid,
@@ -50,9 +56,12 @@ bool pair_cfg_supported(uint32_t mode, const LaunchCfg& c);
 // Block items (inst, block, chunk) of grid instances on the matrix cores,
 // persistent groups of one wave on queue[0]; items of instances the planner
 // did not give the MFMA flag are skipped.  ep_pass: the kernel of the Ep
-// sums (items with pad = 1), else that of the Es sums (pad = 0).
+// sums (items with pad = 1), else that of the Es sums (pad = 0).  offsets:
+// the kernel that applies the grid offsets of float32 pixel lists (KInst
+// gex, gez; instances without them run on it as well).
 hipError_t launch_mfma(const KInst* insts, const WorkItem* items, uint32_t n_items,
-                       uint32_t* queue, uint32_t grid, bool ep_pass, hipStream_t st);
+                       uint32_t* queue, uint32_t grid, bool ep_pass, bool offsets,
+                       hipStream_t st);
 // Pair-kernel launches still to be failed on purpose (xrtk_debug_fail_launches):
 // launch_pair takes one and returns hipErrorLaunchFailure before enqueuing
 // anything.
@@ -143,6 +152,11 @@ struct InstSpec {
   // such an instance may run on the matrix cores
   double gx0 = 0, gy = 0, gz0 = 0, gsx = 0, gsz = 0;
   uint32_t gnx = 0, gnz = 0, gth = 0, gtw = 0;
+  // a float32 pixel list on that grid: device offsets of its rows (gnx) and
+  // columns (gnz) from it, in pixels, at most gemax in magnitude
+  const float* gex = nullptr;
+  const float* gez = nullptr;
+  double gemax = 0;
 };
 // Records the grid of a prepare_wave screen (src/waves/prepare_wave.rs:40-68:
 // x = cx + (ix/(nx-1) - 1/2) 2 dx, likewise z; a dimension of 1 sits at the

@@ -314,7 +314,7 @@ struct xrtk_server {
     if (e == hipSuccess && s.nr)
       e = launch_prep(s.rays, s.nr, pix, s.npad, perm, s.np, scr, part, s.mode, I.kr_hi, I.kr_lo,
                       s.skip, 0, s.static_flags | (r->routed ? 0u : INST_F_MFMA_OK), c32.P,
-                      s.gsx, s.gsz, rays32, pix32, tiles, R64, st);
+                      s.gsx, s.gsz, 0.0, rays32, pix32, tiles, R64, st);
     if (e == hipSuccess && !r->routed) {
       // the planner's flags, for the routing (hblk takes the result later)
       e = hipMemcpyAsync(r->hblk, &scr->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
@@ -547,7 +547,7 @@ struct xrtk_server {
         e = launch_mfma(d_ins, d_mf[k], (uint32_t)L.h_items_mf[k].size(), d_q + 32 + k,
                         std::min<uint32_t>((uint32_t)L.h_items_mf[k].size(),
                                            (uint32_t)ctx->n_cu * 8u),
-                        k == 1, stream);
+                        k == 1, false, stream);
     if (e == hipSuccess && !L.h_items64.empty())
       e = launch_pair(MODE_F64, c64, d_ins, d_64, (uint32_t)L.h_items64.size(), d_q + 16,
                       grid_of(c64, MODE_F64, L.h_items64.size()), 0, stream);
