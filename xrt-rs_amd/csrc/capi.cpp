@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "runtime.h"
@@ -265,6 +266,41 @@ struct RaySrc {
   }
 };
 
+// Whether every ray has the first ray's energy, and (if so) the box of the
+// ray positions: one pass over the records, split over a few host threads.
+bool one_energy_box(const RaySrc& rs, Box& box) {
+  const size_t n = rs.nr;
+  if (!n) return false;
+  const double e0 = rs.energy(0);
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n / 65536));
+  std::vector<Box> part(T);
+  std::vector<char> same(T, 1);
+  auto run = [&](size_t t) {
+    const size_t i0 = n * t / T, i1 = n * (t + 1) / T;
+    Box b;
+    for (size_t i = i0; i < i1; ++i) {
+      if (rs.energy(i) != e0) { same[t] = 0; return; }
+      double v[3];
+      rs.pos(i, v);
+      b.add(v[0], v[1], v[2]);
+    }
+    part[t] = b;
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < T; ++t) th.emplace_back(run, t);
+  run(0);
+  for (auto& x : th) x.join();
+  for (size_t t = 0; t < T; ++t) {
+    if (!same[t]) return false;
+    for (int k = 0; k < 3; ++k) {
+      box.lo[k] = std::min(box.lo[k], part[t].lo[k]);
+      box.hi[k] = std::max(box.hi[k], part[t].hi[k]);
+    }
+  }
+  return true;
+}
+
 // Uploads 13 f64 columns (path_up may be null: zeros) into `stage` at
 // offset 0, optionally permuted, and converts them into dst.
 int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* path_up,
@@ -321,10 +357,13 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   if (want == MODE_F64) c64 = choose_cfg(ctx, MODE_F64, nr, np, opts);
   if (!pair_cfg_supported(MODE_F32R, c32) || !pair_cfg_supported(MODE_F64, c64))
     return XRTK_EINVAL;
-  // monochromatic groups for the reduced kernel
+  // monochromatic groups for the reduced kernel (one energy: one group in
+  // the given order, its box from the same pass)
   std::vector<std::vector<uint32_t>> groups;
   bool broadband = false;
-  if (nr) {
+  Box mono_box;
+  const bool mono = nr && want == MODE_F32R && one_energy_box(rs, mono_box);
+  if (nr && !mono) {
     groups = group_by_energy(nr, [&](size_t i) { return rs.energy(i); });
     broadband = groups.size() > 64;
   }
@@ -367,7 +406,12 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   std::vector<G> gs;
   std::vector<Box> boxes;  // each group's ray box
   std::vector<uint32_t> perm;
-  if (nr && want == MODE_F32R && !broadband) {
+  const bool reduced = nr && want == MODE_F32R && !broadband;
+  if (mono) {
+    const double e = rs.energy(0);
+    gs.push_back(G{0, nr, route_mode(MODE_F32R, mono_box, pbox, dq, e), e});
+    boxes.push_back(mono_box);
+  } else if (reduced) {
     uint32_t off = 0;
     for (auto& gr : groups) {
       Box rb;
@@ -383,44 +427,44 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     }
     if (groups.size() > 1)
       for (auto& gr : groups) perm.insert(perm.end(), gr.begin(), gr.end());
-    // An untuned grid solve takes smaller tiles while that keeps more
-    // groups off the FP64 kernel: the routing bound grows with the tile's
-    // extent (near-field screens at P = 16 can exceed it where P = 4 does
-    // not).
-    if (grid && !cfg_shape_given(ctx, MODE_F32R, nr, np, opts)) {
-      auto n64 = [&](int P, double* d) {
-        tile_dq(P, d);
-        int n = 0;
-        for (size_t g = 0; g < gs.size(); ++g)
-          n += route_mode(MODE_F32R, boxes[g], pbox, d, gs[g].energy) == MODE_F64;
-        return n;
-      };
-      int cur = 0;
-      for (const G& g : gs) cur += g.mode == MODE_F64;
-      while (cur > 0 && c32.P > 4) {
-        double d2[3];
-        const int m = n64(c32.P / 2, d2);
-        if (m >= cur) break;
-        c32.P /= 2;
-        cur = m;
-        std::memcpy(dq, d2, sizeof(dq));
-        for (size_t g = 0; g < gs.size(); ++g)
-          gs[g].mode = route_mode(MODE_F32R, boxes[g], pbox, dq, gs[g].energy);
-        c64 = cfg64_for(c32);
-        npad = grid_slots(grid->nx, grid->nz, c32, c64);
-      }
-    }
   } else if (nr) {
     gs.push_back(G{0, nr, MODE_F64, 0.0});
+  }
+  // An untuned grid solve takes smaller tiles while that keeps more
+  // groups off the FP64 kernel: the routing bound grows with the tile's
+  // extent (near-field screens at P = 16 can exceed it where P = 4 does
+  // not).
+  if (reduced && grid && !cfg_shape_given(ctx, MODE_F32R, nr, np, opts)) {
+    auto n64 = [&](int P, double* d) {
+      tile_dq(P, d);
+      int n = 0;
+      for (size_t g = 0; g < gs.size(); ++g)
+        n += route_mode(MODE_F32R, boxes[g], pbox, d, gs[g].energy) == MODE_F64;
+      return n;
+    };
+    int cur = 0;
+    for (const G& g : gs) cur += g.mode == MODE_F64;
+    while (cur > 0 && c32.P > 4) {
+      double d2[3];
+      const int m = n64(c32.P / 2, d2);
+      if (m >= cur) break;
+      c32.P /= 2;
+      cur = m;
+      std::memcpy(dq, d2, sizeof(dq));
+      for (size_t g = 0; g < gs.size(); ++g)
+        gs[g].mode = route_mode(MODE_F32R, boxes[g], pbox, dq, gs[g].energy);
+      c64 = cfg64_for(c32);
+      npad = grid_slots(grid->nx, grid->nz, c32, c64);
+    }
   }
   // ---- buffers
   ph.next("xrtk.upload");
   const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
                                   : align256(13 * sizeof(double) * (size_t)nr);
-  const size_t pix_stage =
-      ps.grid ? 0
-      : ps.f32 ? sizeof(XrtkGpuPixel) * np + (list_grid ? sizeof(float) * (lg.ex.size() + lg.ez.size()) : 0)
-               : 3 * sizeof(double) * (size_t)np;
+  const size_t offs = list_grid ? sizeof(float) * (lg.ex.size() + lg.ez.size()) : 0;
+  const size_t pix_stage = ps.grid  ? 0
+                           : ps.f32 ? sizeof(XrtkGpuPixel) * np + offs
+                                    : 3 * sizeof(double) * (size_t)np;
   hipError_t e = ws.stage.ensure(ray_stage + pix_stage + 256);
   if (e == hipSuccess) e = ws.rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
   if (e == hipSuccess) e = ws.pix64.ensure(sizeof(PixF64) * npad);
