@@ -393,16 +393,23 @@ def leg_headline(args, ctx, xrtk, W):
         px, py, pz = xrtk.prepare_wave(wave)
         p32 = xrtk.gpu_pixels(px, py, pz)
         xrtk.kirchhoff_gpu(ctx, r32, p32)
-        t_os = []
-        for _ in range(2):
+        t_os, pair_os = [], []
+        for _ in range(3):
             t = time.perf_counter()
             xrtk.kirchhoff_gpu(ctx, r32, p32)
             t_os.append(time.perf_counter() - t)
+            pair_os.append(ctx.last_timing()[0])
+        plan_os = ctx.last_plan()
         oneshot = {"value": pairs / min(t_os) / 1e9, "unit": "Gpair-evals/s",
-                   "ms_per_call": 1e3 * min(t_os),
-                   "what": "xrtk_kirchhoff_gpu_f32 from host buffers: upload, prep, pair, "
-                           "merge, result copy (PCIe-inclusive)"}
-        log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call")
+                   "ms_per_call": 1e3 * min(t_os), "pair_ms": min(pair_os),
+                   "pair_engine": "mfma" if plan_os["n_mfma"] else "valu",
+                   "inst_flags": plan_os["inst_flags"],
+                   "what": "xrtk_kirchhoff_gpu_f32 from host buffers (float32 GpuRay and "
+                           "GpuPixel records; the prepare_wave pixel list laid out as its "
+                           "grid): plan, upload, prep, pair, merge, result copy "
+                           "(PCIe-inclusive)"}
+        log(f"one-shot kirchhoff_gpu from host buffers: {1e3 * min(t_os):.1f} ms/call "
+            f"({oneshot['pair_engine']} pair kernel {min(pair_os):.2f} ms)")
 
     if on_mfma:
         n_chunks = max(1, plan["n_items"] // (((args.side + 63) // 64) ** 2))
