@@ -138,6 +138,31 @@ def test_planner_refusals(ctx):
         assert np.array_equal(o, ov), what
 
 
+@pytest.mark.parametrize("case", ["10m_20keV", "20m_25keV", "config3_instance7"])
+def test_steep_phase_steps(ctx, case):
+    """Grids whose phase moves 0.8-1.3 rev per pixel (more than half a
+    revolution: the kernel takes the steps mod 1) while the block remainder
+    stays within its bound: on the matrix cores, against the long-double
+    oracle and the VALU kernel."""
+    if case == "config3_instance7":
+        spec = W.batch_instances()[7]
+        rays, scr, e = W.instance_rays(spec), W.instance_screen(spec), spec["energy"]
+    else:
+        dist, e, half = {"10m_20keV": (10000.0, 20000.0, 1.0),
+                         "20m_25keV": (20000.0, 25000.0, 1.5)}[case]
+        rays = W.point_source_aperture(65536, energy=e, seed=21)
+        scr = W.screen(dist, half, 256)
+    wave = grid(scr)
+    o, p = solve(ctx, rays, wave, energy=e)
+    ov, _ = solve(ctx, rays, wave, energy=e, engine=VALU)
+    assert p["n_mfma"] == 1, (case, p)
+    stride, ld = ld_sample(rays, wave)
+    e_ld, e_v = rel_l2(o[::stride], ld), rel_l2(o, ov)
+    print(f"{case}: block kernel vs long double {e_ld:.2e}, vs VALU {e_v:.2e}")
+    assert e_ld < REL_BLOCK
+    assert e_v < 2 * REL_KERNELS
+
+
 def test_ep_pass(ctx):
     """Rays with Ep != 0 take both passes of the block kernel: the Es pass
     gives the same bits as for the same rays with Ep = 0, and Ep (here

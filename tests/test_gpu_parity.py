@@ -354,6 +354,7 @@ def test_zero_axis_variant_is_exact(ctx, monkeypatch):
 
 VAR_TILE, VAR_NOFRACT, VAR_LIN2, VAR_PAX, VAR_SEPAMP, VAR_LINE = 2, 16, 32, 64, 128, 256
 VAR_LINEDEV, VAR_EPZERO, VAR_UNIFORM_N = 1024, 2048, 8
+VALU = 1  # XRTK_ENGINE_VALU: the VALU kernels only
 
 
 @pytest.mark.parametrize("form", ["line", "linedev", "per_slot", "per_ray_normal", "exact"])
@@ -440,10 +441,12 @@ def test_separable_amplitude_matches_exact_form(ctx, monkeypatch, case):
     else:
         rays, scr, _ = W.stress_cell(20000.0, float(case[1:]), n_rays=32768, side=96)
         g = grid(scr)
-    sep = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    # both on the VALU kernel, whose amplitude forms these are (grids within
+    # the block bound would otherwise take the matrix cores)
+    sep = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", engine=VALU)
     flags = ctx.last_plan()["inst_flags"]
     monkeypatch.setenv("XRTK_FORCE_FLAGS", "0x1000")
-    per_pair = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r")
+    per_pair = xrtk.diffraction_integral(rays, g, ctx=ctx, mode="f32r", engine=VALU)
     assert not ctx.last_plan()["inst_flags"] & VAR_SEPAMP
     monkeypatch.delenv("XRTK_FORCE_FLAGS")
     between = rel_l2(sep, per_pair)

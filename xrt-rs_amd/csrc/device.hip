@@ -780,16 +780,17 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
     // blocks, Q_i = max|q_i| (+ h in the plane), and |q.e| <= |(Q_x, Q_z)|,
     // |o.e| <= |(o_x, o_z)|.  A phase error of at most E rev per term moves
     // the field by at most 2 pi E relative (rel-L2 included), so the variant
-    // is allowed when E <= 1e-6 rev (6.3e-6 rad) and the phase steps stay
-    // small enough that FP32 holds the factors' phases to ~5e-7 rev: per
-    // pixel along x, |df . step| = kr |gsx| |q_x (1/|d| - 1/|q|) - o_x/|d||
-    // <= kr |gsx| (Q_x dR / Rl^2 + o_x / Rl) (likewise z), times 16 from a
-    // tile centre plus the half revolution of the anchor, <= 8 rev.
+    // is allowed when E <= 1e-6 rev (6.3e-6 rad).  The phase step per pixel
+    // along x is |df . step| = kr |gsx| |q_x (1/|d| - 1/|q|) - o_x/|d||
+    // <= smax = kr |gsx| (Q_x dR / Rl^2 + o_x / Rl) (likewise z).  The
+    // kernel takes the steps mod 1, so at any slope the factors' FP32 phases
+    // stay within 1/2 + 15.5/2 rev (~5e-7 rev of rounding).
     // A float32 pixel list on the grid (offsets |e| <= gemax pixels from it)
-    // widens the block to h = (32 + gemax) |(gsx, gsz)|; the kernel applies
-    // the offsets of the second row and column tile to first order, which
-    // leaves (2 pi smax 2 gemax)^2 / 2 per term, held below 2 pi 1e-7 by
-    // smax gemax <= 8.9e-5.
+    // widens the block to h = (32 + gemax) |(gsx, gsz)|, and its offset term
+    // e sx needs the step itself, so its steps stay below 1/2 rev
+    // (16 smax + 1/2 <= 8).  The kernel applies the offsets of the second
+    // row and column tile to first order, which leaves (2 pi smax 2 gemax)^2
+    // / 2 per term, held below 2 pi 1e-7 by smax gemax <= 8.9e-5.
     const uint32_t need_m = INST_F_TILE | INST_F_PAX | INST_F_UNIFORM_N | INST_F_SEPAMP;
     if ((static_flags & INST_F_MFMA_OK) && (f & need_m) == need_m && !(force_flags & 0x20000u)) {
       const double ax = fabs(gsx), az = fabs(gsz);
@@ -805,7 +806,8 @@ __global__ void k_plan(InstScratch* __restrict__ scr, double skip, double kr, ui
         const double E = 0.5 * kr * (t1 + t2) * h * h;
         const double smax = kr * fmax(ax * (Qx * t1 + ox / Rl), az * (Qz * t1 + oz / Rl));
         scr->blk_rev = E;
-        if (E <= 1e-6 && 16.0 * smax + 0.5 <= 8.0 && smax * gemax <= 8.9e-5) f |= INST_F_MFMA;
+        const bool slope_ok = gemax == 0.0 || 16.0 * smax + 0.5 <= 8.0;
+        if (E <= 1e-6 && slope_ok && smax * gemax <= 8.9e-5) f |= INST_F_MFMA;
       }
     }
   }
@@ -1417,9 +1419,9 @@ static constexpr uint32_t kMfmaBlk = 64;  // block side in pixels
 // Per-(ray, block) values, one array each, 64 rays per LDS block.
 enum : int {
   MR_PH = 0,  // phase at the centre of the first row tile, revolutions in [-1/2, 1/2]
-  MR_SX,      // phase step per row (grid x)
+  MR_SX,      // phase step per row (grid x), mod 1
   MR_GZ,      // phase at the centre of the first column tile relative to a
-  MR_SZ,      // phase step per column (grid z)
+  MR_SZ,      // phase step per column (grid z), mod 1
   MR_AMP,     // scaled amplitude: cs (Es pass) or pr (Ep pass)
   MR_RXC, MR_RXS,  // e^{2 pi i 32 sx}: first row tile to the second
   MR_RZC, MR_RZS,  // e^{2 pi i 32 sz}
@@ -1630,15 +1632,21 @@ __device__ __forceinline__ void mfma_item(const KInst& I, const WorkItem& w,
       // d/dt of |a + t s - o| - |a + t s| = (a.s) g - (o.s) / |a - o|
       const double sx = kr * I.gsx * (qa[0] * g - o[0] * ip);
       const double sz = kr * I.gsz * (qa[2] * g - o[2] * ip);
-      double ph = kr * Bq * iden + ((double)rr.off + corr) - 16.0 * sx;
+      // The factors take the steps mod 1, fx = sx - nx: row m of a tile
+      // (u0 = m - 15.5 from its centre) has phase u0 sx = u0 fx + m nx -
+      // 15.5 nx, m nx an integer, so -15.5 nx goes into the anchor phase.
+      // The FP32 phase then stays within 1/2 + 15.5/2 rev at any slope.
+      const double nx = rint(sx), nz = rint(sz);
+      const double fx = sx - nx, fz = sz - nz;
+      double ph = kr * Bq * iden + ((double)rr.off + corr) - 16.0 * fx - 15.5 * nx;
       ph -= rint(ph);
-      double gz = -16.0 * sz;
+      double gz = -16.0 * fz - 15.5 * nz;
       gz -= rint(gz);
-      double tx = 32.0 * sx, tz = 32.0 * sz;
+      double tx = 32.0 * fx, tz = 32.0 * fz;
       tx -= rint(tx);
       tz -= rint(tz);
-      v[MR_PH] = (float)ph; v[MR_SX] = (float)sx;
-      v[MR_GZ] = (float)gz; v[MR_SZ] = (float)sz;
+      v[MR_PH] = (float)ph; v[MR_SX] = (float)fx;
+      v[MR_GZ] = (float)gz; v[MR_SZ] = (float)fz;
       v[MR_AMP] = rr.amp * scale;
       v[MR_AMPI] = rr.ampi * scale;
       v[MR_RXC] = __builtin_amdgcn_cosf((float)tx); v[MR_RXS] = __builtin_amdgcn_sinf((float)tx);
