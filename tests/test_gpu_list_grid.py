@@ -106,6 +106,22 @@ def test_uneven_grid_offsets(ctx, center, amp):
     assert e_v < REL_KERNELS
 
 
+@pytest.mark.parametrize("sx,sz", [(-1.0, 1.0), (1.0, -1.0), (-1.0, -1.0)])
+def test_mirrored_grids(ctx, sx, sz):
+    """prepare_wave screens with negative half-widths (x or z descending
+    along the list): still grids, on the matrix cores, against the oracle."""
+    rays = W.point_source_aperture(32768, seed=11)
+    r32 = to_gpu_rays(rays)
+    g = xrtk.WaveParams((0.003, 1000.0, -0.002), 0.02 * sx, 0.015 * sz, 200, 136)
+    p32 = xrtk.gpu_pixels(*xrtk.prepare_wave(g))
+    got, plan = run(ctx, r32, p32)
+    assert plan["n_mfma"] == 1, plan
+    stride, ld = ld_at(r32, p32)
+    e = rel_l2(got[::stride], ld)
+    print(f"mirrored ({sx}, {sz}): vs long double {e:.2e}")
+    assert e < REL_BLOCK
+
+
 def test_ep_pass_with_offsets(ctx):
     """Rays with Ep != 0 on an uneven grid: both passes of the offsets
     kernel, against the oracle."""
