@@ -266,6 +266,32 @@ struct RaySrc {
   }
 };
 
+// memcpy split over up to 8 host threads (at least 4 MiB each).
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 22));
+  auto part = [&](size_t t) {
+    const size_t a = (n * t / T) & ~size_t(63), b = t + 1 == T ? n : (n * (t + 1) / T) & ~size_t(63);
+    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < T; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto& x : th) x.join();
+}
+
+// Host memory to device memory on the context's stream.  With `pin` (the
+// workspace's page-locked block, or null) a large copy is first gathered
+// into the block by a few host threads and then moved by one DMA, about
+// twice the rate of a copy from pageable memory; the block must stay as it
+// is until the stream has run the copy.
+hipError_t copy_h2d(xrtk_ctx* ctx, char* pin, void* dst, const void* src, size_t bytes) {
+  if (!pin || bytes < (size_t(4) << 20))
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  par_memcpy(pin, src, bytes);
+  return hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
 // Whether every ray has the first ray's energy, and (if so) the box of the
 // ray positions: one pass over the records, split over a few host threads.
 bool one_energy_box(const RaySrc& rs, Box& box) {
@@ -302,9 +328,11 @@ bool one_energy_box(const RaySrc& rs, Box& box) {
 }
 
 // Uploads 13 f64 columns (path_up may be null: zeros) into `stage` at
-// offset 0, optionally permuted, and converts them into dst.
+// offset 0, optionally permuted, and converts them into dst.  pin: a
+// page-locked block of 13 columns the copies pass through (or null).
 int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* path_up,
-                const std::vector<uint32_t>* perm, uint32_t nr, RayRec64* dst) {
+                const std::vector<uint32_t>* perm, uint32_t nr, RayRec64* dst,
+                char* pin = nullptr) {
   if (!nr) return XRTK_OK;
   const double* cols[13] = {r->x,     r->y,     r->z,     r->nx,    r->ny,
                             r->nz,    r->nl,    r->es_re, r->es_im, r->ep_re,
@@ -321,12 +349,16 @@ int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* 
     if (!cols[c]) {
       e = launch_zero_f64(d, nr, ctx->stream);
     } else if (perm) {
-      tmp.resize(nr);
-      for (uint32_t i = 0; i < nr; ++i) tmp[i] = cols[c][(*perm)[i]];
-      e = hipMemcpyAsync(d, tmp.data(), cb, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp is reused
+      double* t = pin ? reinterpret_cast<double*>(pin + c * cb) : nullptr;
+      if (!t) {
+        tmp.resize(nr);
+        t = tmp.data();
+      }
+      for (uint32_t i = 0; i < nr; ++i) t[i] = cols[c][(*perm)[i]];
+      e = hipMemcpyAsync(d, t, cb, hipMemcpyHostToDevice, ctx->stream);
+      if (!pin && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp is reused
     } else {
-      e = hipMemcpyAsync(d, cols[c], cb, hipMemcpyHostToDevice, ctx->stream);
+      e = copy_h2d(ctx, pin ? pin + c * cb : nullptr, d, cols[c], cb);
     }
   }
   if (e == hipSuccess) e = launch_rays_from_cols(reinterpret_cast<const double*>(S), dst, nr,
@@ -472,6 +504,13 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   if (e == hipSuccess) e = ws.outbuf.ensure(sizeof(double) * 4 * np);
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* S = ws.stage.as<char>();
+  // the page-locked block (not for the allocation-free workspace; a failed
+  // allocation falls back to copies from the caller's memory)
+  char* pin = nullptr;
+  if (!ws.stage.arena &&
+      ws.hpin.ensure(std::max(ray_stage + pix_stage, sizeof(double) * 4 * (size_t)np)) ==
+          hipSuccess)
+    pin = static_cast<char*>(ws.hpin.p);
   // ---- rays
   if (rs.f32 && nr) {
     std::vector<XrtkGpuRay> tmp;
@@ -481,7 +520,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
       for (uint32_t i : perm) tmp.push_back(rs.f32[i]);
       src = tmp.data();
     }
-    e = hipMemcpyAsync(S, src, sizeof(XrtkGpuRay) * nr, hipMemcpyHostToDevice, ctx->stream);
+    e = copy_h2d(ctx, pin, S, src, sizeof(XrtkGpuRay) * nr);
     if (e == hipSuccess)
       e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ws.rays64.as<RayRec64>(),
                                   nr, ctx->stream);
@@ -489,7 +528,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     if (e != hipSuccess) return map_err(e);
   } else if (nr) {
     const int rc = upload_cols(ctx, ws.stage, rs.f64, nullptr, perm.empty() ? nullptr : &perm, nr,
-                               ws.rays64.as<RayRec64>());
+                               ws.rays64.as<RayRec64>(), pin);
     if (rc != XRTK_OK) return rc;
   }
   // ---- pixels
@@ -502,7 +541,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     float* X = reinterpret_cast<float*>(L + sizeof(XrtkGpuPixel) * np);
     d_ex = X;
     d_ez = X + lg.ex.size();
-    e = hipMemcpyAsync(L, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice, ctx->stream);
+    e = copy_h2d(ctx, pin ? pin + ray_stage : nullptr, L, ps.f32, sizeof(XrtkGpuPixel) * np);
     if (e == hipSuccess)
       e = hipMemcpyAsync(X, lg.ex.data(), sizeof(float) * lg.ex.size(), hipMemcpyHostToDevice,
                          ctx->stream);
@@ -520,8 +559,8 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
                         tw, npad, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), ctx->stream);
     d_perm = ws.perm.as<uint32_t>();
   } else if (ps.f32) {
-    e = hipMemcpyAsync(S + ray_stage, ps.f32, sizeof(XrtkGpuPixel) * np, hipMemcpyHostToDevice,
-                       ctx->stream);
+    e = copy_h2d(ctx, pin ? pin + ray_stage : nullptr, S + ray_stage, ps.f32,
+                 sizeof(XrtkGpuPixel) * np);
     if (e == hipSuccess)
       e = launch_pix_from_f32aos(S + ray_stage, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(),
                                  np, npad, (uint32_t)c32.P, ctx->stream);
@@ -529,10 +568,10 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   } else {
     double* P3 = reinterpret_cast<double*>(S + ray_stage);
     const size_t cb = sizeof(double) * np;
-    e = hipMemcpyAsync(P3, ps.x, cb, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(P3 + np, ps.y, cb, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(P3 + 2 * np, ps.z, cb, hipMemcpyHostToDevice, ctx->stream);
+    char* pp = pin ? pin + ray_stage : nullptr;
+    e = copy_h2d(ctx, pp, P3, ps.x, cb);
+    if (e == hipSuccess) e = copy_h2d(ctx, pp ? pp + cb : nullptr, P3 + np, ps.y, cb);
+    if (e == hipSuccess) e = copy_h2d(ctx, pp ? pp + 2 * cb : nullptr, P3 + 2 * np, ps.z, cb);
     if (e == hipSuccess)
       e = launch_pix_from_cols(P3, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), np, npad,
                                (uint32_t)c32.P, ctx->stream);
@@ -584,10 +623,15 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
                          false, true, nullptr, /*route_sync=*/true);
   if (rc != XRTK_OK) return rc;
   ph.next("xrtk.readback");
-  e = hipMemcpyAsync(out4, ws.outbuf.p, sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
-                     ctx->stream);
+  // (a large result comes back through the page-locked block, which the
+  // inputs no longer need once the stream has run this far)
+  const size_t ob = sizeof(double) * 4 * np;
+  const bool via_pin = pin && ob >= (size_t(4) << 20);
+  e = hipMemcpyAsync(via_pin ? static_cast<void*>(pin) : static_cast<void*>(out4), ws.outbuf.p,
+                     ob, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return map_err(e);
+  if (via_pin) par_memcpy(out4, pin, ob);
   record_timing(ctx);
   record_plan(ctx, plan, 1);
   if (plan.scr0) {
@@ -819,7 +863,13 @@ int xrtk_rayset_upload(xrtk_ctx* ctx, const XrtkRaysF64* rays, const double* pat
     if (ctx->ws.stage.ensure(13 * sizeof(double) * std::max<size_t>(n_rays, 1) + 256) != hipSuccess ||
         rs.rays.ensure(sizeof(RayRec64) * std::max<size_t>(n_rays, 1)) != hipSuccess)
       return XRTK_ENOMEM;
-    int rc = upload_cols(ctx, ctx->ws.stage, rays, path_up, nullptr, rs.nr, rs.rays.as<RayRec64>());
+    // (through the context's page-locked block, when it can have one; the
+    // synchronisation below frees it for the next call)
+    char* pin = nullptr;
+    if (ctx->ws.hpin.ensure(13 * sizeof(double) * std::max<size_t>(n_rays, 1)) == hipSuccess)
+      pin = static_cast<char*>(ctx->ws.hpin.p);
+    int rc = upload_cols(ctx, ctx->ws.stage, rays, path_up, nullptr, rs.nr, rs.rays.as<RayRec64>(),
+                         pin);
     if (rc == XRTK_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRTK_ELAUNCH;
     if (rc != XRTK_OK) {
       rs.rays.release();

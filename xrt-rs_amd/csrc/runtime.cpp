@@ -28,6 +28,27 @@ int map_err(hipError_t e) {
   return XRTK_ELAUNCH;
 }
 
+hipError_t HBuf::ensure(size_t bytes) {
+  if (bytes <= cap && p) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(bytes + bytes / 4, 1u << 20);
+  want = (want + 4095) & ~size_t(4095);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    return e;
+  }
+  cap = want;
+  return hipSuccess;
+}
+
+void HBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
 hipError_t DBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return hipSuccess;
   if (arena) {

@@ -97,13 +97,25 @@ struct DBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// Grow-only page-locked host buffer (hipHostMalloc).
+struct HBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes);
+  void release();
+};
+
 // Device buffers of one-shot solves: staged inputs, resident-form rays and
 // slots, per-instance work (FP32 records, tiles, partial sums), launch
-// metadata and the merged output.
+// metadata and the merged output; and the page-locked host block the inputs
+// pass through on their way to `stage` (unused by an arena workspace, which
+// allocates nothing).
 struct Workspace {
   DBuf stage, rays64, pix64, perm, work, meta, outbuf;
+  HBuf hpin;
   void release() {
     for (DBuf* b : {&stage, &rays64, &pix64, &perm, &work, &meta, &outbuf}) b->release();
+    hpin.release();
   }
   void use_arena(Arena* a) {
     for (DBuf* b : {&stage, &rays64, &pix64, &perm, &work, &meta, &outbuf}) b->arena = a;
