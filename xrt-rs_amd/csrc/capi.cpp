@@ -266,32 +266,6 @@ struct RaySrc {
   }
 };
 
-// memcpy split over up to 8 host threads (at least 4 MiB each).
-void par_memcpy(void* dst, const void* src, size_t n) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 22));
-  auto part = [&](size_t t) {
-    const size_t a = (n * t / T) & ~size_t(63), b = t + 1 == T ? n : (n * (t + 1) / T) & ~size_t(63);
-    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
-  };
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < T; ++t) th.emplace_back(part, t);
-  part(0);
-  for (auto& x : th) x.join();
-}
-
-// Host memory to device memory on the context's stream.  With `pin` (the
-// workspace's page-locked block, or null) a large copy is first gathered
-// into the block by a few host threads and then moved by one DMA, about
-// twice the rate of a copy from pageable memory; the block must stay as it
-// is until the stream has run the copy.
-hipError_t copy_h2d(xrtk_ctx* ctx, char* pin, void* dst, const void* src, size_t bytes) {
-  if (!pin || bytes < (size_t(4) << 20))
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
-  par_memcpy(pin, src, bytes);
-  return hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, ctx->stream);
-}
-
 // Whether every ray has the first ray's energy, and (if so) the box of the
 // ray positions: one pass over the records, split over a few host threads.
 bool one_energy_box(const RaySrc& rs, Box& box) {
@@ -358,7 +332,7 @@ int upload_cols(xrtk_ctx* ctx, DBuf& stage, const XrtkRaysF64* r, const double* 
       e = hipMemcpyAsync(d, t, cb, hipMemcpyHostToDevice, ctx->stream);
       if (!pin && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // tmp is reused
     } else {
-      e = copy_h2d(ctx, pin ? pin + c * cb : nullptr, d, cols[c], cb);
+      e = copy_h2d(ctx->stream, pin ? pin + c * cb : nullptr, d, cols[c], cb);
     }
   }
   if (e == hipSuccess) e = launch_rays_from_cols(reinterpret_cast<const double*>(S), dst, nr,
@@ -520,7 +494,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
       for (uint32_t i : perm) tmp.push_back(rs.f32[i]);
       src = tmp.data();
     }
-    e = copy_h2d(ctx, pin, S, src, sizeof(XrtkGpuRay) * nr);
+    e = copy_h2d(ctx->stream, pin, S, src, sizeof(XrtkGpuRay) * nr);
     if (e == hipSuccess)
       e = launch_rays_from_f32aos(reinterpret_cast<const float*>(S), ws.rays64.as<RayRec64>(),
                                   nr, ctx->stream);
@@ -541,7 +515,8 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     float* X = reinterpret_cast<float*>(L + sizeof(XrtkGpuPixel) * np);
     d_ex = X;
     d_ez = X + lg.ex.size();
-    e = copy_h2d(ctx, pin ? pin + ray_stage : nullptr, L, ps.f32, sizeof(XrtkGpuPixel) * np);
+    e = copy_h2d(ctx->stream, pin ? pin + ray_stage : nullptr, L, ps.f32,
+                 sizeof(XrtkGpuPixel) * np);
     if (e == hipSuccess)
       e = hipMemcpyAsync(X, lg.ex.data(), sizeof(float) * lg.ex.size(), hipMemcpyHostToDevice,
                          ctx->stream);
@@ -559,7 +534,7 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
                         tw, npad, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), ctx->stream);
     d_perm = ws.perm.as<uint32_t>();
   } else if (ps.f32) {
-    e = copy_h2d(ctx, pin ? pin + ray_stage : nullptr, S + ray_stage, ps.f32,
+    e = copy_h2d(ctx->stream, pin ? pin + ray_stage : nullptr, S + ray_stage, ps.f32,
                  sizeof(XrtkGpuPixel) * np);
     if (e == hipSuccess)
       e = launch_pix_from_f32aos(S + ray_stage, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(),
@@ -569,9 +544,10 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
     double* P3 = reinterpret_cast<double*>(S + ray_stage);
     const size_t cb = sizeof(double) * np;
     char* pp = pin ? pin + ray_stage : nullptr;
-    e = copy_h2d(ctx, pp, P3, ps.x, cb);
-    if (e == hipSuccess) e = copy_h2d(ctx, pp ? pp + cb : nullptr, P3 + np, ps.y, cb);
-    if (e == hipSuccess) e = copy_h2d(ctx, pp ? pp + 2 * cb : nullptr, P3 + 2 * np, ps.z, cb);
+    e = copy_h2d(ctx->stream, pp, P3, ps.x, cb);
+    if (e == hipSuccess) e = copy_h2d(ctx->stream, pp ? pp + cb : nullptr, P3 + np, ps.y, cb);
+    if (e == hipSuccess)
+      e = copy_h2d(ctx->stream, pp ? pp + 2 * cb : nullptr, P3 + 2 * np, ps.z, cb);
     if (e == hipSuccess)
       e = launch_pix_from_cols(P3, ws.pix64.as<PixF64>(), ws.perm.as<uint32_t>(), np, npad,
                                (uint32_t)c32.P, ctx->stream);

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace xrtk {
 
@@ -26,6 +27,27 @@ int map_err(hipError_t e) {
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return XRTK_ENOMEM;
   if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return XRTK_ENODEV;
   return XRTK_ELAUNCH;
+}
+
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 22));
+  auto part = [&](size_t t) {
+    const size_t a = (n * t / T) & ~size_t(63);
+    const size_t b = t + 1 == T ? n : (n * (t + 1) / T) & ~size_t(63);
+    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < T; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto& x : th) x.join();
+}
+
+hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_t bytes) {
+  if (!pin || bytes < (size_t(4) << 20))
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  par_memcpy(pin, src, bytes);
+  return hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, st);
 }
 
 hipError_t HBuf::ensure(size_t bytes) {

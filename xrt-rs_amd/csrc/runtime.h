@@ -105,6 +105,15 @@ struct HBuf {
   void release();
 };
 
+// memcpy split over up to 8 host threads (at least 4 MiB each).
+void par_memcpy(void* dst, const void* src, size_t n);
+// Host memory to device memory on `st`.  With `pin` (a page-locked block,
+// or null) a large copy is first gathered into the block by a few host
+// threads and then moved by one DMA, about twice the rate of a copy from
+// pageable memory; the block must stay as it is until the stream has run
+// the copy.
+hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_t bytes);
+
 // Device buffers of one-shot solves: staged inputs, resident-form rays and
 // slots, per-instance work (FP32 records, tiles, partial sums), launch
 // metadata and the merged output; and the page-locked host block the inputs
