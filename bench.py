@@ -32,7 +32,8 @@ Further legs, reported in the same JSON line:
             median of 3 runs;
   fresnel   fresnel_propagate of the same rays onto a 1024^2 screen (a
             complex product over the rays on the matrix cores, its factors
-            generated in the kernel from FP64-reduced phases), FP32 and FP64,
+            generated in the kernel from FP64-reduced phases): FP32 sums on
+            FP16 hi + lo parts (the default), on FP32 operands, and FP64,
             with the oracle's loop on a pixel sample;
   undulator Undulator::build_i_map of a planar undulator on a 1024 x 1024
             energy-angle grid (FP64), with the oracle on a point sample.
@@ -592,31 +593,41 @@ def leg_fresnel(args, ctx, xrtk, W):
     res = {"workload": f"fresnel_propagate, {rx.shape[0]} Gaussian-source rays, 10 keV, "
                        f"1000 mm, {side}^2 screen of +-0.1 mm", "pairs": pairs}
     out = {}
-    for mode, steps in (("f32r", args.fresnel_steps), ("f64", max(1, args.fresnel_steps // 2))):
-        xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode)
+    # f32r: FP32 factors, products on FP16 hi + lo parts (three MFMAs per
+    # real product: 24 MFMA flops per pair against the FP16 peak); f32ops:
+    # the same sums with FP32 operands (XrtkOpts.engine = 1, 8 flops per
+    # pair against the FP32 matrix peak); f64: FP64 throughout
+    legs = (("f32r", "f32r", 0, args.fresnel_steps, PEAK_MFMA_FP16, 24),
+            ("f32ops", "f32r", 1, args.fresnel_steps, PEAK_MFMA_FP32, 8),
+            ("f64", "f64", 0, max(1, args.fresnel_steps // 2), PEAK_MFMA_FP64, 8))
+    for name, mode, engine, steps, peak, fpp in legs:
+        xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode,
+                               engine=engine)
         wall, dev, kern = [], [], []
         for _ in range(steps):
             t = time.perf_counter()
-            r = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode)
+            r = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, mode=mode,
+                                       engine=engine)
             wall.append(time.perf_counter() - t)
             k_ms, d_ms = ctx.last_timing()
             kern.append(k_ms)
             dev.append(d_ms)
-        out[mode] = r.amplitude
+        out[name] = r.amplitude
         d_ms, k_ms = float(np.median(dev)), float(np.median(kern))
-        peak = PEAK_MFMA_FP32 if mode == "f32r" else PEAK_MFMA_FP64
         # the product kernel (factors generated in it, on the matrix cores)
-        # against the matrix peak; the merge of the ray slices follows it
-        tflops = pairs * 8 / (k_ms * 1e-3) / 1e12
-        res[mode] = {"value": pairs / float(np.median(wall)) / 1e9, "unit": "Gpair-evals/s",
+        # against the matrix peak of its operands; the merge follows it
+        tflops = pairs * fpp / (k_ms * 1e-3) / 1e12
+        res[name] = {"value": pairs / float(np.median(wall)) / 1e9, "unit": "Gpair-evals/s",
                      "ms_per_call": 1e3 * float(np.median(wall)), "device_ms": d_ms,
                      "kernel_ms": k_ms,
                      "device_gpairs_per_s": pairs / (d_ms * 1e-3) / 1e9,
                      "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak,
                                   "unit": "TFLOP/s", "frac": tflops / peak,
-                                  "flop_per_pair": 8, "kernel_ms": k_ms}}
-        log(f"fresnel {mode}: {1e3 * np.median(wall):.1f} ms/call (device {d_ms:.1f} ms, product "
-            f"kernel {k_ms:.1f} ms), {res[mode]['value']:.0f} Gpair/s, {tflops:.1f} TFLOP/s")
+                                  "flop_per_pair": fpp, "kernel_ms": k_ms}}
+        log(f"fresnel {name}: {1e3 * np.median(wall):.1f} ms/call (device {d_ms:.1f} ms, "
+            f"product kernel {k_ms:.1f} ms), {res[name]['value']:.0f} Gpair/s, "
+            f"{tflops:.1f} TFLOP/s of {peak:.0f}")
+    res["rel_l2_f32ops_vs_f64"] = W.rel_l2(out["f32ops"], out["f64"])
     res["rel_l2_f32r_vs_f64"] = W.rel_l2(out["f32r"], out["f64"])
     if not args.no_cpu:
         import pyoracle

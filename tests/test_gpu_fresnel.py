@@ -1,6 +1,8 @@
-"""Fresnel propagation on the device (src/waves/fresnel.rs:53-106): both
-factor precisions against the oracle, the reference's unit tests, closed
-forms, screen axes and ray chunking."""
+"""Fresnel propagation on the device (src/waves/fresnel.rs:53-106): the FP64
+kernel and both FP32-sum kernels (products on FP16 hi + lo parts, the
+default, and on FP32 operands, engine 1) against the oracle, the
+reference's unit tests, closed forms, screen axes, ray chunking and the
+amplitude scale."""
 import numpy as np
 import pytest
 
@@ -12,6 +14,11 @@ pytestmark = pytest.mark.gpu
 
 REL_F32 = 1e-5   # FP32 factors and products, FP64 phases and accumulation
 REL_F64 = 1e-11  # FP64 factors and products
+# One term's error, FP32 operands: the factors' FP32 rounding (~2^-23); FP16
+# parts: x - hi - lo <= 2^-22 |x| per factor and the dropped lo.lo, up to
+# ~3 x 2^-22 = 7e-7 per product.
+TERM_F32 = {"f32r": 1e-6, "f32ops": 2e-7}
+F32_MODES = ("f32r", "f32ops")
 
 
 def rel_l2(a, b):
@@ -23,8 +30,12 @@ def k_of(e):
 
 
 def run_both(ctx, rx, rz, es, e, d, scr):
-    return {m: xrtk.fresnel_propagate(rx, rz, es, e, d, scr, ctx=ctx, mode=m)
-            for m in ("f32r", "f64")}
+    """f64, f32r (FP16 parts) and f32ops (FP32 operands, engine 1)."""
+    out = {m: xrtk.fresnel_propagate(rx, rz, es, e, d, scr, ctx=ctx, mode=m)
+           for m in ("f32r", "f64")}
+    out["f32ops"] = xrtk.fresnel_propagate(rx, rz, es, e, d, scr, ctx=ctx, mode="f32r",
+                                           engine=1)
+    return out
 
 
 def test_reference_unit_tests(ctx):
@@ -32,7 +43,8 @@ def test_reference_unit_tests(ctx):
     r = xrtk.fresnel_propagate([0.0], [0.0], [1 + 0j], 10000.0, 1000.0,
                                xrtk.FresnelScreen(-0.1, 0.1, 21, -0.1, 0.1, 21), ctx=ctx)
     assert r.amplitude.shape == (21 * 21,)
-    assert abs(r.amplitude[10 * 21 + 10]) >= abs(r.amplitude[0]) * (1 - 1e-6)
+    # |U| = 1 at every pixel: centre >= corner up to two terms' rounding
+    assert abs(r.amplitude[10 * 21 + 10]) >= abs(r.amplitude[0]) * (1 - 2 * TERM_F32["f32r"])
     r = xrtk.fresnel_propagate([-0.01, 0.01], [0.0, 0.0], [1 + 0j, 1 + 0j], 10000.0, 1000.0,
                                xrtk.FresnelScreen(-0.1, 0.1, 101, 0.0, 0.0, 1), ctx=ctx)
     inten = r.intensity()
@@ -51,7 +63,7 @@ def test_closed_forms(ctx):
     for m, r in run_both(ctx, [x0], [z0], [es], 12000.0, 750.0, scr).items():
         X, Z = np.meshgrid(r.x_pixels, r.z_pixels)
         exp = es * np.exp(1j * (k_of(12000.0) / (2 * 750.0)) * ((X - x0) ** 2 + (Z - z0) ** 2))
-        tol = 1e-12 if m == "f64" else 2e-7
+        tol = 1e-12 if m == "f64" else TERM_F32[m]
         assert np.max(np.abs(r.amplitude - exp.ravel())) < tol, m
 
 
@@ -82,7 +94,8 @@ def test_vs_oracle_small(ctx, shape):
                               scr.z_min, scr.z_max, nz)
     got = run_both(ctx, rx, rz, es, 10000.0, 1000.0, scr)
     assert rel_l2(got["f64"].amplitude, ref) < REL_F64
-    assert rel_l2(got["f32r"].amplitude, ref) < REL_F32
+    for m in F32_MODES:
+        assert rel_l2(got[m].amplitude, ref) < REL_F32, m
 
 
 def test_config0_aperture_to_256sq(ctx):
@@ -94,10 +107,13 @@ def test_config0_aperture_to_256sq(ctx):
     ref, _, _ = O.fresnel_f64(rx, rz, es, 10000.0, 1000.0, -0.05, 0.05, 256, -0.05, 0.05, 256,
                               stride=stride)
     got = run_both(ctx, rx, rz, es, 10000.0, 1000.0, scr)
-    e64, e32 = rel_l2(got["f64"].amplitude[::stride], ref), rel_l2(got["f32r"].amplitude[::stride], ref)
-    print(f"fresnel 65536 x 256^2: f64 {e64:.2e}, f32 factors {e32:.2e}, "
-          f"f32 vs f64 over all pixels {rel_l2(got['f32r'].amplitude, got['f64'].amplitude):.2e}")
-    assert e64 < REL_F64 and e32 < REL_F32
+    e64 = rel_l2(got["f64"].amplitude[::stride], ref)
+    assert e64 < REL_F64
+    for m in F32_MODES:
+        e32 = rel_l2(got[m].amplitude[::stride], ref)
+        print(f"fresnel 65536 x 256^2: f64 {e64:.2e}, {m} {e32:.2e}, "
+              f"vs f64 over all pixels {rel_l2(got[m].amplitude, got['f64'].amplitude):.2e}")
+        assert e32 < REL_F32, m
 
 
 def test_ray_chunks(ctx):
@@ -110,7 +126,8 @@ def test_ray_chunks(ctx):
                               stride=stride)
     got = run_both(ctx, rx, rz, es, 8000.0, 500.0, scr)
     assert rel_l2(got["f64"].amplitude[::stride], ref) < REL_F64
-    assert rel_l2(got["f32r"].amplitude[::stride], ref) < REL_F32
+    for m in F32_MODES:
+        assert rel_l2(got[m].amplitude[::stride], ref) < REL_F32, m
 
 
 def test_ragged_rays_and_tiles(ctx):
@@ -122,19 +139,35 @@ def test_ragged_rays_and_tiles(ctx):
                               scr.z_min, scr.z_max, 131)
     got = run_both(ctx, rx, rz, es, 12000.0, 800.0, scr)
     assert rel_l2(got["f64"].amplitude, ref) < REL_F64
-    assert rel_l2(got["f32r"].amplitude, ref) < REL_F32
+    for m in F32_MODES:
+        assert rel_l2(got[m].amplitude, ref) < REL_F32, m
 
 
 def test_slices_set_by_the_fp32_cap(ctx):
     """A screen of 4096^2 pixels fills the device with tiles alone, so only
-    the 16,384-ray cap on an FP32 sum splits the rays: 40,000 rays take
-    three slices of 13,344, the last one partly zero-padded."""
+    the cap on an FP32 sum splits the rays: 40,000 rays take five slices of
+    8,000 with FP16 parts (cap 8,192) and three of 13,344 with FP32 operands
+    (cap 16,384), the last one partly zero-padded."""
     rx, rz, es = _aperture(40000, 0.1, 10000.0, 19)
     scr = xrtk.FresnelScreen(-0.1, 0.1, 4096, -0.1, 0.1, 4096)
     stride = 40009
     ref, _, _ = O.fresnel_f64(rx, rz, es, 10000.0, 1000.0, -0.1, 0.1, 4096, -0.1, 0.1, 4096,
                               stride=stride)
-    got = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx)
-    e32 = rel_l2(got.amplitude[::stride], ref)
-    print(f"fresnel 40000 x 4096^2 (three FP32 slices): {e32:.2e}")
-    assert e32 < REL_F32
+    for engine in (0, 1):
+        got = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx, engine=engine)
+        e32 = rel_l2(got.amplitude[::stride], ref)
+        print(f"fresnel 40000 x 4096^2, engine {engine}: {e32:.2e}")
+        assert e32 < REL_F32, engine
+
+
+def test_amplitude_scale_is_exact(ctx):
+    """The FP16-part kernel scales the amplitudes by a power of two and back:
+    amplitudes times 2^-70 or 2^50 give the same field times that power,
+    bit for bit."""
+    rx, rz, es = _aperture(3000, 0.05, 10000.0, 23)
+    scr = xrtk.FresnelScreen(-0.03, 0.03, 70, -0.03, 0.03, 50)
+    base = xrtk.fresnel_propagate(rx, rz, es, 10000.0, 1000.0, scr, ctx=ctx).amplitude
+    for p in (-70, 50):
+        got = xrtk.fresnel_propagate(rx, rz, es * 2.0 ** p, 10000.0, 1000.0, scr,
+                                     ctx=ctx).amplitude
+        assert np.array_equal(got, base * 2.0 ** p), p

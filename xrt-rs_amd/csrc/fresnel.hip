@@ -15,15 +15,23 @@
 // rows and columns block by block into LDS (each phase K d^2 evaluated in
 // FP64 and reduced to a fractional revolution first), then multiplies them
 // there:
-//   FP32 (XRTK_MODE_F32R): v_mfma_f32_32x32x2f32, 128 x 128 pixels per
-//        workgroup of 4 waves (64 x 64 each), sin/cos of the reduced phase
-//        by v_sin/v_cos; each workgroup's FP32 sums cover at most 16,384
-//        rays (a ray slice) and go to HBM as partial sums;
+//   FP32 (XRTK_MODE_F32R): 128 x 128 pixels per workgroup of 4 waves
+//        (64 x 64 each), sin/cos of the reduced phase by v_sin/v_cos; each
+//        workgroup's FP32 sums cover at most 8,192 rays (a ray slice; 16,384
+//        with FP32 operands) and go to HBM as partial sums.  The products run on
+//        v_mfma_f32_32x32x16_f16 with each FP32 factor split into FP16
+//        hi = RNE(x) and lo = RNE(x - hi) (x - hi - lo ~ 2^-22 |x|), three
+//        products hi.hi + hi.lo + lo.hi per real product, FP32 accumulation
+//        (~2^-21 per term); the amplitudes are scaled by a power of two so
+//        the largest |Re| + |Im| lies in [1/2, 1), and the merge scales back
+//        exactly.  XrtkOpts.engine = 1 takes the FP32 operands on
+//        v_mfma_f32_32x32x2f32 instead (the comparison kernel);
 //   FP64 (XRTK_MODE_F64): v_mfma_f64_16x16x4f64, 64 x 64 pixels per
 //        workgroup (32 x 32 per wave); e^{2 pi i f} of the reduced phase f
 //        from a 256-entry table and a degree-7 Taylor series (cis_rev_tab).
 // The complex product takes four real products per pair of factors:
-//   Re U += Zr Xr + Zi (-Xi),  Im U += Zr Xi + Zi Xr   (8 flops per pair).
+//   Re U += Zr Xr + Zi (-Xi),  Im U += Zr Xi + Zi Xr   (8 flops per pair;
+//   24 MFMA flops with the FP16 parts).
 // k_fresnel_merge* sums the ray slices' partials in FP64, in slice order.
 // Output index iz * nx + ix (fresnel.rs:78-99).
 #include <hip/hip_runtime.h>
@@ -46,10 +54,17 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 constexpr uint32_t kT32 = 128;     // FP32: tile side in pixels
 constexpr uint32_t kKB32 = 16;     // FP32: rays per LDS block
 constexpr uint32_t kSlice32 = 16384;  // FP32: most rays in one FP32 sum
+constexpr uint32_t kT16 = 128;     // FP16 parts: tile side in pixels
+constexpr uint32_t kKB16 = 16;     // FP16 parts: rays per LDS block (one K of the MFMA)
+constexpr uint32_t kSlice16 = 8192;   // FP16 parts: most rays in one FP32 sum
+static_assert(kT16 == kT32 && kKB16 == kKB32, "both FP32-sum kernels share one launch shape");
 constexpr uint32_t kT64 = 64;      // FP64: tile side
 constexpr uint32_t kKB64 = 8;      // FP64: rays per LDS block
 constexpr uint32_t kTab = 256;     // FP64: entries of the e^{2 pi i j / 256} table
@@ -177,6 +192,175 @@ __global__ __launch_bounds__(256, 2) void k_fresnel_mfma32(
     __syncthreads();
     gen32(P0, kb + 2u * kKB32, g, h, xg, zg, krev, crx, crz, ces);
     mma32<2>(P1, lane, zb, xb, ur, ui);
+    __syncthreads();
+  }
+  float2* Pp = partial + (size_t)blockIdx.y * nz * nx;
+  const uint32_t xl = x0 + xb + (lane & 31u);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const uint32_t x = xl + 32u * b;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t z = z0 + zb + 32u * a + 8u * (r / 4) + 4u * (lane >> 5) + (r % 4);
+        if (z < nz && x < nx) Pp[(size_t)z * nx + x] = make_float2(ur[a][b][r], ui[a][b][r]);
+      }
+    }
+}
+
+// ---- FP32 factors in FP16 parts ------------------------------------------------------
+// hi = RNE(x), lo = RNE(x - hi) of two values, packed in one register each;
+// m1 is -1 held in a register, so fma(hi, m1, x) rounded to FP16 is one
+// v_fma_mix instruction.
+__device__ __forceinline__ void split_pair(float a, float b, float m1, uint32_t& hi,
+                                           uint32_t& lo) {
+  const floatx2 p = {a, b};
+  const halfx2 h = __builtin_convertvector(p, halfx2);
+  const halfx2 l = {(_Float16)__builtin_fmaf((float)h.x, m1, a),
+                    (_Float16)__builtin_fmaf((float)h.y, m1, b)};
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// Factors of one block of 16 rays in the operand layout of
+// v_mfma_f32_32x32x16_f16 (lane l takes row / column l % 32 and rays
+// 8 (l / 32) .. + 7): part (re hi, re lo, im hi, im lo), pixel, half (8 rays,
+// 16 bytes).  The two halves of a pixel are stored swapped when (pixel / 4)
+// is odd, so that the 8 lanes of one ds_read_b128 pass cover distinct banks.
+struct Planes16 {
+  uint4 z[4][kT16][2];
+  uint4 x[4][kT16][2];
+};
+
+__device__ __forceinline__ uint32_t half_slot(uint32_t pix, uint32_t h) {
+  return h ^ ((pix >> 2) & 1u);
+}
+
+// Thread t: column / row g = t % 128, rays 8 h .. 8 h + 7 of the block
+// (h = t / 128): the phases as gen32, the factors split into FP16 parts.
+__device__ __forceinline__ void gen16(Planes16& P, uint32_t kb, uint32_t g, uint32_t h,
+                                      double xg, double zg, double krev, float m1,
+                                      const XRTK_FCONST double* crx,
+                                      const XRTK_FCONST double* crz,
+                                      const XRTK_FCONST double2* ces) {
+  float xr[8], xi[8], zr[8], zi[8];
+#pragma unroll
+  for (uint32_t i = 0; i < 8; ++i) {
+    const uint32_t k = kb + 8u * h + i;
+    const double xk = crx[k], zk = crz[k];
+    const double2 e = ces[k];
+    float fx, fz;
+    {
+#pragma clang fp contract(off)
+      const double dx = xg - xk, dz = zg - zk;
+      const double px = krev * (dx * dx), pz = krev * (dz * dz);
+      fx = (float)(px - rint(px));
+      fz = (float)(pz - rint(pz));
+    }
+    xr[i] = __builtin_amdgcn_cosf(fx);
+    xi[i] = __builtin_amdgcn_sinf(fx);
+    const float c = __builtin_amdgcn_cosf(fz), sn = __builtin_amdgcn_sinf(fz);
+    const float er = (float)e.x, ei = (float)e.y;
+    zr[i] = er * c - ei * sn;
+    zi[i] = er * sn + ei * c;
+  }
+  uint32_t q[8][4];  // [xr hi, xr lo, xi hi, xi lo, zr hi, zr lo, zi hi, zi lo][pair]
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    split_pair(xr[2 * j], xr[2 * j + 1], m1, q[0][j], q[1][j]);
+    split_pair(xi[2 * j], xi[2 * j + 1], m1, q[2][j], q[3][j]);
+    split_pair(zr[2 * j], zr[2 * j + 1], m1, q[4][j], q[5][j]);
+    split_pair(zi[2 * j], zi[2 * j + 1], m1, q[6][j], q[7][j]);
+  }
+  const uint32_t hs = half_slot(g, h);
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    P.x[p][g][hs] = make_uint4(q[p][0], q[p][1], q[p][2], q[p][3]);
+    P.z[p][g][hs] = make_uint4(q[4 + p][0], q[4 + p][1], q[4 + p][2], q[4 + p][3]);
+  }
+}
+
+__device__ __forceinline__ halfx8 as_h8(uint4 v) { return __builtin_bit_cast(halfx8, v); }
+
+__device__ __forceinline__ floatx16 mfma3(halfx8 ah, halfx8 al, halfx8 bh, halfx8 bl,
+                                          floatx16 c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+// The products of one block of 16 rays: each wave's 2 x 2 blocks of 32 x 32,
+// four real products per pair of complex factors, three MFMAs each.
+__device__ __forceinline__ void mma16(const Planes16& P, uint32_t lane, uint32_t zb, uint32_t xb,
+                                      floatx16 (&ur)[2][2], floatx16 (&ui)[2][2]) {
+  const uint32_t c = lane & 31u, kh = lane >> 5;
+  uint4 a[2][4], b[2][4], nb[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const uint32_t r = zb + 32u * t + c, x = xb + 32u * t + c;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      a[t][p] = P.z[p][r][half_slot(r, kh)];
+      b[t][p] = P.x[p][x][half_slot(x, kh)];
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const uint4 v = b[t][2 + p];
+      nb[t][p] = make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u,
+                            v.w ^ 0x80008000u);
+    }
+  }
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) {
+      const halfx8 zrh = as_h8(a[tr][0]), zrl = as_h8(a[tr][1]);
+      const halfx8 zih = as_h8(a[tr][2]), zil = as_h8(a[tr][3]);
+      const halfx8 xrh = as_h8(b[tc][0]), xrl = as_h8(b[tc][1]);
+      const halfx8 xih = as_h8(b[tc][2]), xil = as_h8(b[tc][3]);
+      ur[tr][tc] = mfma3(zrh, zrl, xrh, xrl, ur[tr][tc]);
+      ur[tr][tc] = mfma3(zih, zil, as_h8(nb[tc][0]), as_h8(nb[tc][1]), ur[tr][tc]);
+      ui[tr][tc] = mfma3(zrh, zrl, xih, xil, ui[tr][tc]);
+      ui[tr][tc] = mfma3(zih, zil, xrh, xrl, ui[tr][tc]);
+    }
+}
+
+// FP16-part kernel: the shape, LDS double buffering, slices and padding of
+// k_fresnel_mfma32, 16 rays per block.
+__global__ __launch_bounds__(256, 2) void k_fresnel_mfma16(
+    const double* __restrict__ xp, uint32_t nx, const double* __restrict__ zp, uint32_t nz,
+    const double* __restrict__ rx, const double* __restrict__ rz, const double2* __restrict__ es,
+    uint32_t slice, double krev, uint32_t n_tx, float2* __restrict__ partial, float m1) {
+  __shared__ Planes16 P0, P1;
+  const uint32_t t = threadIdx.x, lane = t & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const uint32_t tx = blockIdx.x % n_tx, tz = blockIdx.x / n_tx;
+  const uint32_t x0 = tx * kT16, z0 = tz * kT16;
+  const uint32_t g = t & 127u, h = __builtin_amdgcn_readfirstlane(t >> 7);
+  const double xg = xp[min(x0 + g, nx - 1u)];
+  const double zg = zp[min(z0 + g, nz - 1u)];
+  const uint32_t zb = (wave & 1u) * 64u, xb = (wave >> 1) * 64u;
+  floatx16 ur[2][2], ui[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) ur[a][b][r] = ui[a][b][r] = 0.f;
+  const XRTK_FCONST double* crx = (const XRTK_FCONST double*)rx;
+  const XRTK_FCONST double* crz = (const XRTK_FCONST double*)rz;
+  const XRTK_FCONST double2* ces = (const XRTK_FCONST double2*)es;
+  const uint32_t k0 = blockIdx.y * slice;
+  gen16(P0, k0, g, h, xg, zg, krev, m1, crx, crz, ces);
+  __syncthreads();
+  for (uint32_t kb = k0; kb < k0 + slice; kb += 2u * kKB16) {
+    gen16(P1, kb + kKB16, g, h, xg, zg, krev, m1, crx, crz, ces);
+    mma16(P0, lane, zb, xb, ur, ui);
+    __syncthreads();
+    gen16(P0, kb + 2u * kKB16, g, h, xg, zg, krev, m1, crx, crz, ces);
+    mma16(P1, lane, zb, xb, ur, ui);
     __syncthreads();
   }
   float2* Pp = partial + (size_t)blockIdx.y * nz * nx;
@@ -330,11 +514,12 @@ __global__ __launch_bounds__(256, 2) void k_fresnel_mfma64(
     }
 }
 
-// U[p] = sum over the ray slices of partial[s][p], in FP64 and slice order.
+// U[p] = sum over the ray slices of partial[s][p], in FP64 and slice order,
+// times `back` (a power of two: the amplitudes' scale undone).
 template <class T>
 __global__ __launch_bounds__(256) void k_fresnel_merge(const T* __restrict__ partial,
                                                        uint32_t n_slices, size_t np,
-                                                       double2* __restrict__ out) {
+                                                       double back, double2* __restrict__ out) {
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
        p += (size_t)gridDim.x * blockDim.x) {
     double re = 0.0, im = 0.0;
@@ -343,7 +528,7 @@ __global__ __launch_bounds__(256) void k_fresnel_merge(const T* __restrict__ par
       re += (double)v.x;
       im += (double)v.y;
     }
-    out[p] = make_double2(re, im);
+    out[p] = make_double2(re * back, im * back);
   }
 }
 
@@ -393,6 +578,8 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
   const uint32_t nx = screen->nx, nz = screen->nz;
   if ((uint64_t)nx * nz > 0x7fff0000ull || n_rays > 0x7fffffffull) return XRTK_EINVAL;
   const bool f64 = opts && opts->mode == XRTK_MODE_F64;
+  // FP32 factors in FP16 parts, unless engine = 1 asks for FP32 operands
+  const bool f16 = !f64 && !(opts && opts->engine == XRTK_ENGINE_VALU);
   try {
     std::vector<double> xp(nx), zp(nz);
     fresnel_axis(screen->x_min, screen->x_max, nx, xp.data());
@@ -411,13 +598,15 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     Phases ph;
     ph.next("xrtk.upload");
     // launch shape: tiles of the screen x ray slices; slices of at most
-    // kSlice32 rays for the FP32 sums, and enough of them for ~4 workgroups
+    // kSlice16 (FP16 parts) or kSlice32 (FP32 operands) rays for the FP32
+    // sums, and enough of them for ~4 workgroups
     // per CU over the launch
     const uint32_t T = f64 ? kT64 : kT32, KB = f64 ? kKB64 : kKB32;
     const uint32_t n_tx = (nx + T - 1) / T, n_tz = (nz + T - 1) / T;
     const uint64_t tiles = (uint64_t)n_tx * n_tz;
     const uint64_t want = std::max<uint64_t>(1, (4ull * ctx->n_cu + tiles - 1) / tiles);
-    uint64_t n_sl = std::max<uint64_t>(want, f64 ? 1 : (n_rays + kSlice32 - 1) / kSlice32);
+    const uint64_t cap = f16 ? kSlice16 : kSlice32;
+    uint64_t n_sl = std::max<uint64_t>(want, f64 ? 1 : (n_rays + cap - 1) / cap);
     n_sl = std::min<uint64_t>(n_sl, (n_rays + 2 * KB - 1) / (2 * KB));
     // a slice is a whole number of block pairs (the kernels alternate two LDS
     // buffers); the rays are padded with zeros to whole slices plus the one
@@ -443,7 +632,24 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     std::vector<double> rxp(nr_pad, 0.0), rzp(nr_pad, 0.0), es(2 * nr_pad, 0.0);
     std::memcpy(rxp.data(), ray_x, 8 * n_rays);
     std::memcpy(rzp.data(), ray_z, 8 * n_rays);
-    for (size_t j = 0; j < n_rays; ++j) { es[2 * j] = es_re[j]; es[2 * j + 1] = es_im[j]; }
+    // FP16 parts: amplitudes scaled by 2^-ex so the largest |Re| + |Im| is
+    // in [1/2, 1) (exact; the merge multiplies by 2^ex)
+    double scale = 1.0, back = 1.0;
+    if (f16) {
+      double amax = 0.0;
+      for (size_t j = 0; j < n_rays; ++j)
+        amax = std::max(amax, std::fabs(es_re[j]) + std::fabs(es_im[j]));
+      if (amax > 0.0 && std::isfinite(amax)) {
+        int ex = 0;
+        (void)std::frexp(amax, &ex);
+        scale = std::ldexp(1.0, -ex);
+        back = std::ldexp(1.0, ex);
+      }
+    }
+    for (size_t j = 0; j < n_rays; ++j) {
+      es[2 * j] = es_re[j] * scale;
+      es[2 * j + 1] = es_im[j] * scale;
+    }
     hipError_t e = hipMemcpyAsync(base + o_xp, xp.data(), 8 * nx, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_zp, zp.data(), 8 * nz, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_rx, rxp.data(), 8 * nr_pad, hipMemcpyHostToDevice, st);
@@ -470,16 +676,20 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
       (void)hipEventRecord(ctx->ev[2], st);
       if (e == hipSuccess)
         hipLaunchKernelGGL(k_fresnel_merge<double2>, dim3(grid_for(np)), dim3(256), 0, st, Pd,
-                           (uint32_t)n_sl, np, U);
+                           (uint32_t)n_sl, np, 1.0, U);
     } else {
       float2* Pf = reinterpret_cast<float2*>(base + o_p);
-      hipLaunchKernelGGL(k_fresnel_mfma32, grid, dim3(256), 0, st, dxp, nx, dzp, nz, drx, drz, des,
-                         (uint32_t)slice, krev, n_tx, Pf);
+      if (f16)
+        hipLaunchKernelGGL(k_fresnel_mfma16, grid, dim3(256), 0, st, dxp, nx, dzp, nz, drx, drz,
+                           des, (uint32_t)slice, krev, n_tx, Pf, -1.0f);
+      else
+        hipLaunchKernelGGL(k_fresnel_mfma32, grid, dim3(256), 0, st, dxp, nx, dzp, nz, drx, drz,
+                           des, (uint32_t)slice, krev, n_tx, Pf);
       e = hipGetLastError();
       (void)hipEventRecord(ctx->ev[2], st);
       if (e == hipSuccess)
         hipLaunchKernelGGL(k_fresnel_merge<float2>, dim3(grid_for(np)), dim3(256), 0, st, Pf,
-                           (uint32_t)n_sl, np, U);
+                           (uint32_t)n_sl, np, back, U);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return map_err(e);

@@ -556,10 +556,14 @@ class FresnelResult:
 
 
 def fresnel_propagate(ray_x, ray_z, ray_es, energy: float, dist: float, screen: FresnelScreen,
-                      ctx: Optional[GpuContext] = None, mode="f32r") -> FresnelResult:
+                      ctx: Optional[GpuContext] = None, mode="f32r",
+                      engine: int = 0) -> FresnelResult:
     """fresnel_propagate (src/waves/fresnel.rs:53-106) on the device: source
     rays at (x, z) on the plane z = 0 with complex amplitudes ray_es,
-    propagated dist mm at energy eV onto the screen grid."""
+    propagated dist mm at energy eV onto the screen grid.  mode "f32r":
+    FP32 factors from FP64 phases, FP32 sums per ray slice, FP64 merge; the
+    products on FP16 hi + lo parts (engine 0) or on FP32 operands
+    (engine 1).  mode "f64": FP64 throughout."""
     rx = np.ascontiguousarray(np.asarray(ray_x, dtype=np.float64).reshape(-1))
     rz = np.ascontiguousarray(np.asarray(ray_z, dtype=np.float64).reshape(-1))
     es = np.asarray(ray_es, dtype=np.complex128).reshape(-1)
@@ -574,7 +578,7 @@ def fresnel_propagate(ray_x, ray_z, ray_es, energy: float, dist: float, screen: 
     out = np.zeros(2 * int(screen.nx) * int(screen.nz))
     xp, zp = np.zeros(int(screen.nx)), np.zeros(int(screen.nz))
     sc = screen.to_c()
-    o = _lib.opts(_mode(mode))
+    o = _lib.opts(_mode(mode), engine=engine)
     check(lib.xrtk_fresnel_propagate(ctx.handle, dptr(rx), dptr(rz), dptr(er), dptr(ei),
                                      rx.shape[0], float(energy), float(dist), C.byref(sc),
                                      dptr(out), dptr(xp), dptr(zp), C.byref(o)),
