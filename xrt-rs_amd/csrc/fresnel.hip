@@ -226,15 +226,16 @@ __device__ __forceinline__ void split_pair(float a, float b, float m1, uint32_t&
 // Factors of one block of 16 rays in the operand layout of
 // v_mfma_f32_32x32x16_f16 (lane l takes row / column l % 32 and rays
 // 8 (l / 32) .. + 7): part (re hi, re lo, im hi, im lo), pixel, half (8 rays,
-// 16 bytes).  The two halves of a pixel are stored swapped when (pixel / 4)
-// is odd, so that the 8 lanes of one ds_read_b128 pass cover distinct banks.
+// 16 bytes).  The two halves of a pixel are stored swapped when bits 2 and 3
+// of the pixel index differ, so that 8 or 16 consecutive lanes of one
+// ds_read_b128 or ds_write_b128 (pixels 32 bytes apart) cover distinct banks.
 struct Planes16 {
   uint4 z[4][kT16][2];
   uint4 x[4][kT16][2];
 };
 
 __device__ __forceinline__ uint32_t half_slot(uint32_t pix, uint32_t h) {
-  return h ^ ((pix >> 2) & 1u);
+  return h ^ (((pix >> 2) ^ (pix >> 3)) & 1u);
 }
 
 // Thread t: column / row g = t % 128, rays 8 h .. 8 h + 7 of the block
