@@ -184,6 +184,19 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       if (g2->key == key) { ge = g2; break; }
     hipError_t e;
     Phases ph;
+    // results -> host, on the stream behind the kernels
+    auto enqueue_readback = [&]() -> hipError_t {
+      for (size_t i = 0; i < n_jobs; ++i) {
+        const size_t np = ge->outs[i].second;
+        if (!np) continue;
+        const hipError_t ce = hipMemcpyAsync(jobs[i].out4, ge->outs[i].first,
+                                             sizeof(double) * 4 * np, hipMemcpyDeviceToHost,
+                                             ctx->stream);
+        if (ce != hipSuccess) return ce;
+      }
+      return hipSuccess;
+    };
+    bool read_back = false;
     if (!ge) {
       ph.next("xrtk.jobs.build");  // layout, grids and the first (eager) run
       auto* ne = new xrtk_ctx::GraphEntry();
@@ -263,6 +276,10 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       e = hipGraphLaunch(ge->exec, ctx->stream);
       if (e != hipSuccess) return map_err(e);
       (void)hipEventRecord(ctx->ev[3], ctx->stream);
+      // the copies queue behind the graph: one wait for both
+      e = enqueue_readback();
+      if (e != hipSuccess) return map_err(e);
+      read_back = true;
       e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return map_err(e);
       float b = 0;
@@ -287,14 +304,13 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
     }
     // results -> host
     ph.next("xrtk.readback");
-    for (size_t i = 0; i < n_jobs; ++i) {
-      const size_t np = ge->outs[i].second;
-      if (!np) continue;
-      e = hipMemcpyAsync(jobs[i].out4, ge->outs[i].first, sizeof(double) * 4 * np,
-                         hipMemcpyDeviceToHost, ctx->stream);
+    if (!read_back) {
+      e = enqueue_readback();
       if (e != hipSuccess) return map_err(e);
+      e = hipStreamSynchronize(ctx->stream);
+    } else {
+      e = hipSuccess;
     }
-    e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) {
       // the variant flags the device planner chose for the first job (known
       // on the host when the items were routed)
