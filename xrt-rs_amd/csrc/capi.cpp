@@ -345,9 +345,52 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
                   double* out4, const XrtkOpts* opts) {
   Range whole("xrtk.solve");
   Phases ph;
-  ph.next("xrtk.plan");
   const uint32_t want = mode_from(opts);
   const uint32_t nr = (uint32_t)rs.nr, np = (uint32_t)ps.np;
+  // Staging sizes (the pixel part bounded before the list is examined: a
+  // float32 list on a grid adds one offset per row and per column).
+  const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
+                                  : align256(13 * sizeof(double) * (size_t)nr);
+  const size_t pix_bound = ps.grid  ? 0
+                           : ps.f32 ? align256(sizeof(XrtkGpuPixel) * np) +
+                                          sizeof(float) * ((size_t)np + 1)
+                                    : 3 * sizeof(double) * (size_t)np;
+  const size_t ob = sizeof(double) * 4 * np;
+  // the page-locked block (not for the allocation-free workspace; a failed
+  // allocation falls back to copies from the caller's memory), with 256
+  // bytes past the staging for the variant flags
+  char* pin = nullptr;
+  size_t pin_tail = 0;
+  if (!ws.stage.arena) {
+    pin_tail = align256(std::max(ray_stage + pix_bound, ob));
+    if (ws.hpin.ensure(pin_tail + 256) == hipSuccess) pin = static_cast<char*>(ws.hpin.p);
+  }
+  // float32 ray records in the given order go up first, through the block,
+  // while the host plans; a solve that needs them regrouped by energy sends
+  // them again in that order.  Every return waits for the stream, so the
+  // block is free when the next solve fills it.
+  bool rays_sent = false;
+  struct WaitOnExit {
+    hipStream_t st;
+    bool armed = false;
+    ~WaitOnExit() {
+      if (armed) (void)hipStreamSynchronize(st);
+    }
+  } wait_on_exit{ctx->stream};
+  if (pin && rs.f32 && nr) {
+    ph.next("xrtk.upload");
+    hipError_t e = ws.stage.ensure(ray_stage + pix_bound + 256);
+    if (e == hipSuccess) e = ws.rays64.ensure(sizeof(RayRec64) * nr);
+    if (e != hipSuccess) return XRTK_ENOMEM;
+    wait_on_exit.armed = true;
+    e = copy_h2d(ctx->stream, pin, ws.stage.p, rs.f32, sizeof(XrtkGpuRay) * nr);
+    if (e == hipSuccess)
+      e = launch_rays_from_f32aos(ws.stage.as<float>(), ws.rays64.as<RayRec64>(), nr,
+                                  ctx->stream);
+    if (e != hipSuccess) return map_err(e);
+    rays_sent = true;
+  }
+  ph.next("xrtk.plan");
   // A float32 list on a grid is laid out as that grid, so that it may run on
   // the matrix cores (XRTK_ENGINE_VALU keeps the list layout).
   ListGrid lg;
@@ -465,12 +508,11 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   }
   // ---- buffers
   ph.next("xrtk.upload");
-  const size_t ray_stage = rs.f32 ? align256(sizeof(XrtkGpuRay) * nr)
-                                  : align256(13 * sizeof(double) * (size_t)nr);
   const size_t offs = list_grid ? sizeof(float) * (lg.ex.size() + lg.ez.size()) : 0;
   const size_t pix_stage = ps.grid  ? 0
                            : ps.f32 ? sizeof(XrtkGpuPixel) * np + offs
                                     : 3 * sizeof(double) * (size_t)np;
+  // (no larger than what the rays' early upload reserved: no reallocation)
   hipError_t e = ws.stage.ensure(ray_stage + pix_stage + 256);
   if (e == hipSuccess) e = ws.rays64.ensure(sizeof(RayRec64) * std::max<uint32_t>(nr, 1));
   if (e == hipSuccess) e = ws.pix64.ensure(sizeof(PixF64) * npad);
@@ -478,15 +520,16 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   if (e == hipSuccess) e = ws.outbuf.ensure(sizeof(double) * 4 * np);
   if (e != hipSuccess) return XRTK_ENOMEM;
   char* S = ws.stage.as<char>();
-  // the page-locked block (not for the allocation-free workspace; a failed
-  // allocation falls back to copies from the caller's memory)
-  char* pin = nullptr;
-  if (!ws.stage.arena &&
-      ws.hpin.ensure(std::max(ray_stage + pix_stage, sizeof(double) * 4 * (size_t)np)) ==
-          hipSuccess)
-    pin = static_cast<char*>(ws.hpin.p);
+  if (rays_sent && !perm.empty()) {
+    // the block is read by the first upload until the stream has run it
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return map_err(e);
+    rays_sent = false;
+  }
   // ---- rays
-  if (rs.f32 && nr) {
+  if (rays_sent) {
+    // already on the device in the given order
+  } else if (rs.f32 && nr) {
     std::vector<XrtkGpuRay> tmp;
     const XrtkGpuRay* src = rs.f32;
     if (!perm.empty()) {
@@ -600,11 +643,15 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   if (rc != XRTK_OK) return rc;
   ph.next("xrtk.readback");
   // (a large result comes back through the page-locked block, which the
-  // inputs no longer need once the stream has run this far)
-  const size_t ob = sizeof(double) * 4 * np;
+  // inputs no longer need once the stream has run this far; the variant
+  // flags into the block's tail)
   const bool via_pin = pin && ob >= (size_t(4) << 20);
   e = hipMemcpyAsync(via_pin ? static_cast<void*>(pin) : static_cast<void*>(out4), ws.outbuf.p,
                      ob, hipMemcpyDeviceToHost, ctx->stream);
+  uint32_t* fl_pin = pin ? reinterpret_cast<uint32_t*>(pin + pin_tail) : nullptr;
+  if (e == hipSuccess && plan.scr0 && fl_pin)
+    e = hipMemcpyAsync(fl_pin, &plan.scr0->flags, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return map_err(e);
   if (via_pin) par_memcpy(out4, pin, ob);
@@ -612,7 +659,9 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
   record_plan(ctx, plan, 1);
   if (plan.scr0) {
     uint32_t fl = 0;
-    if (hipMemcpy(&fl, &plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) == hipSuccess)
+    if (fl_pin)
+      ctx->last_plan.inst_flags = (int32_t)*fl_pin;
+    else if (hipMemcpy(&fl, &plan.scr0->flags, sizeof(fl), hipMemcpyDeviceToHost) == hipSuccess)
       ctx->last_plan.inst_flags = (int32_t)fl;
   }
   return XRTK_OK;

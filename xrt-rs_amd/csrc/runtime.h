@@ -108,10 +108,10 @@ struct HBuf {
 // memcpy split over up to 8 host threads (at least 4 MiB each).
 void par_memcpy(void* dst, const void* src, size_t n);
 // Host memory to device memory on `st`.  With `pin` (a page-locked block,
-// or null) a large copy is first gathered into the block by a few host
-// threads and then moved by one DMA, about twice the rate of a copy from
-// pageable memory; the block must stay as it is until the stream has run
-// the copy.
+// or null) a large copy is gathered into the block by a few host threads
+// in 8 MiB chunks, each chunk's DMA queued as soon as it is in, about twice
+// the rate of a copy from pageable memory; the block must stay as it is
+// until the stream has run the copy.
 hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_t bytes);
 
 // Device buffers of one-shot solves: staged inputs, resident-form rays and
