@@ -395,7 +395,7 @@ def leg_headline(args, ctx, xrtk, W):
         p32 = xrtk.gpu_pixels(px, py, pz)
         xrtk.kirchhoff_gpu(ctx, r32, p32)
         t_os, pair_os = [], []
-        for _ in range(3):
+        for _ in range(8):
             t = time.perf_counter()
             xrtk.kirchhoff_gpu(ctx, r32, p32)
             t_os.append(time.perf_counter() - t)
@@ -403,6 +403,7 @@ def leg_headline(args, ctx, xrtk, W):
         plan_os = ctx.last_plan()
         oneshot = {"value": pairs / min(t_os) / 1e9, "unit": "Gpair-evals/s",
                    "ms_per_call": 1e3 * min(t_os), "pair_ms": min(pair_os),
+                   "calls": len(t_os), "median_ms_per_call": 1e3 * float(np.median(t_os)),
                    "pair_engine": "mfma" if plan_os["n_mfma"] else "valu",
                    "inst_flags": plan_os["inst_flags"],
                    "what": "xrtk_kirchhoff_gpu_f32 from host buffers (float32 GpuRay and "
