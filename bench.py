@@ -862,9 +862,10 @@ def leg_serving(args, ctx, xrtk, W, sets, specs, seconds, seed, budget, rate=Non
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # the first few solves after start-up include a one-time stall of ~1-5 ms
-    # (the GPU idle with work queued, around the third solve); five warmups
-    # keep it out of the timed steps
+    # the first few solves after start-up include a one-time slowdown of
+    # ~1-5 ms on the device (the pair kernel of about the third solve runs
+    # that much longer, once ~20 ms of dense matrix work has run); five
+    # warmups keep it out of the timed steps
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--fp64-steps", type=int, default=1)
