@@ -14,6 +14,7 @@
 //   cpu_fallback_f32                 bit-identical to oracle_fallback_f32
 // A sanitizer report aborts the run (halt_on_error); a mismatch exits 1.
 // With --plant-overflow or --plant-race it runs only that planted fault.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -23,6 +24,7 @@
 #include <vector>
 
 #include "../../include/xrtk.h"
+#include "../../xrt-rs_amd/csrc/host_copy.h"
 
 namespace xrtk {
 int cpu_fallback_f32(const XrtkGpuRay* rays, size_t nr, const XrtkGpuPixel* px, size_t np,
@@ -361,8 +363,53 @@ int planted(const char* what) {
 
 }  // namespace
 
+// par_memcpy and gather_chunks (the page-locked staging of the solves) on
+// ragged sizes around the thread and chunk cuts: every byte lands, chunks
+// are handed on in order, whole, and only once every part is in; after a
+// refused chunk no further one is handed on and the copy still completes.
+void test_host_copies() {
+  const size_t MiB = size_t(1) << 20;
+  const size_t sizes[] = {0, 1, 63, 64, 65, 4 * MiB - 1, 4 * MiB, 4 * MiB + 7, 8 * MiB,
+                          8 * MiB + 13, 21 * MiB + 5, 33 * MiB};
+  std::vector<char> src(33 * MiB + 64), dst(33 * MiB + 64);
+  for (size_t i = 0; i < src.size(); ++i) src[i] = (char)(i * 2654435761u >> 13);
+  for (size_t n : sizes) {
+    std::fill(dst.begin(), dst.end(), 0);
+    xrtk::par_memcpy(dst.data(), src.data(), n);
+    check(std::memcmp(dst.data(), src.data(), n) == 0, "par_memcpy bytes");
+    check(n == dst.size() || dst[n] == 0, "par_memcpy bound");
+    const size_t chunks[] = {8 * MiB, 3 * MiB + 1};
+    for (size_t chunk : chunks) {
+      std::fill(dst.begin(), dst.end(), 0);
+      size_t next = 0;
+      bool whole = true;
+      const bool ok = xrtk::gather_chunks(dst.data(), src.data(), n, chunk,
+                                          [&](size_t c0, size_t len) {
+                                            whole = whole && c0 == next && len > 0 &&
+                                                    std::memcmp(dst.data() + c0, src.data() + c0,
+                                                                len) == 0;
+                                            next = c0 + len;
+                                            return true;
+                                          });
+      check(ok && whole && next == n, "gather_chunks order and contents");
+      check(std::memcmp(dst.data(), src.data(), n) == 0, "gather_chunks bytes");
+      check(n == dst.size() || dst[n] == 0, "gather_chunks bound");
+      // refuse the second chunk
+      std::fill(dst.begin(), dst.end(), 0);
+      int calls = 0;
+      const bool ok2 = xrtk::gather_chunks(dst.data(), src.data(), n, chunk,
+                                           [&](size_t, size_t) { return ++calls < 2; });
+      const size_t nchunks = n ? (n + chunk - 1) / chunk : 0;
+      check(calls == (int)std::min<size_t>(nchunks, 2), "gather_chunks stops handing on");
+      check(ok2 == (nchunks < 2), "gather_chunks reports the refusal");
+      check(std::memcmp(dst.data(), src.data(), n) == 0, "gather_chunks completes the copy");
+    }
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc > 1) return planted(argv[1]);
+  test_host_copies();
   test_prepare_wave();
   test_beam_to_rays();
   test_diffraction_f64();

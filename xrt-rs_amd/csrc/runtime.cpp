@@ -8,13 +8,11 @@
 #include "runtime.h"
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
-#include <memory>
 #include <thread>
 
 namespace xrtk {
@@ -31,56 +29,18 @@ int map_err(hipError_t e) {
   return XRTK_ELAUNCH;
 }
 
-void par_memcpy(void* dst, const void* src, size_t n) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 22));
-  auto part = [&](size_t t) {
-    const size_t a = (n * t / T) & ~size_t(63);
-    const size_t b = t + 1 == T ? n : (n * (t + 1) / T) & ~size_t(63);
-    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
-  };
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < T; ++t) th.emplace_back(part, t);
-  part(0);
-  for (auto& x : th) x.join();
-}
-
 hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_t bytes) {
   if (!pin || bytes < (size_t(4) << 20))
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
-  // Chunks of 8 MiB: every thread copies its share of chunk c, and the
-  // calling thread queues chunk c's DMA once all shares are in, so the
-  // copies into the block run while the earlier chunks cross PCIe.
-  const size_t kChunk = size_t(8) << 20;
-  const size_t C = (bytes + kChunk - 1) / kChunk;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t T = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, bytes >> 22));
-  std::unique_ptr<std::atomic<uint32_t>[]> done(new std::atomic<uint32_t>[C]);
-  for (size_t c = 0; c < C; ++c) done[c].store(0, std::memory_order_relaxed);
-  const char* s = static_cast<const char*>(src);
-  auto share = [&](size_t t, size_t c) {
-    const size_t c0 = c * kChunk, n = std::min(kChunk, bytes - c0);
-    const size_t a = (n * t / T) & ~size_t(63);
-    const size_t b = t + 1 == T ? n : (n * (t + 1) / T) & ~size_t(63);
-    std::memcpy(pin + c0 + a, s + c0 + a, b - a);
-    done[c].fetch_add(1, std::memory_order_release);
-  };
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < T; ++t)
-    th.emplace_back([&, t]() {
-      for (size_t c = 0; c < C; ++c) share(t, c);
-    });
+  // chunks of 8 MiB: each chunk's DMA is queued as soon as it is in the
+  // block, while the host threads copy the next
   hipError_t e = hipSuccess;
-  for (size_t c = 0; c < C; ++c) {
-    share(0, c);
-    while (done[c].load(std::memory_order_acquire) < T) std::this_thread::yield();
-    if (e == hipSuccess) {
-      const size_t c0 = c * kChunk;
-      e = hipMemcpyAsync(static_cast<char*>(dst) + c0, pin + c0, std::min(kChunk, bytes - c0),
-                         hipMemcpyHostToDevice, st);
-    }
-  }
-  for (auto& x : th) x.join();
+  gather_chunks(pin, static_cast<const char*>(src), bytes, size_t(8) << 20,
+                [&](size_t c0, size_t n) {
+                  e = hipMemcpyAsync(static_cast<char*>(dst) + c0, pin + c0, n,
+                                     hipMemcpyHostToDevice, st);
+                  return e == hipSuccess;
+                });
   return e;
 }
 

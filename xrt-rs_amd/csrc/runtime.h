@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
+#include "host_copy.h"
+
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
@@ -105,8 +107,6 @@ struct HBuf {
   void release();
 };
 
-// memcpy split over up to 8 host threads (at least 4 MiB each).
-void par_memcpy(void* dst, const void* src, size_t n);
 // Host memory to device memory on `st`.  With `pin` (a page-locked block,
 // or null) a large copy is gathered into the block by a few host threads
 // in 8 MiB chunks, each chunk's DMA queued as soon as it is in, about twice
