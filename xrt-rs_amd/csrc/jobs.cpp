@@ -229,7 +229,9 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
         if (e == hipSuccess) {
           int rc2 = enqueue_job_grids(ctx, c32, jobs, specs);
           if (rc2 == XRTK_OK)
-            rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, false, nullptr, 0,
+            // (timed: the event records around the pair kernels are captured
+            // too, so every replay times its own pair kernels)
+            rc2 = run_instances(ctx, c32, c64, specs, ne->work, ne->meta, true, nullptr, 0,
                                 force, true, /*upload=*/false,
                                 ne->plan.routed ? &ne->plan.flags : nullptr);
           e = hipStreamEndCapture(ctx->stream, &graph);
@@ -282,10 +284,13 @@ int xrtk_solve_jobs(xrtk_ctx* ctx, const XrtkJob* jobs, size_t n_jobs, const Xrt
       read_back = true;
       e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) return map_err(e);
-      float b = 0;
+      float a = 0, b = 0;
       (void)hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[3]);
       ctx->last_solve_ms = b;
-      ctx->last_pair_ms = b;
+      // the pair kernels' span, from the event records inside the graph
+      ctx->last_pair_ms =
+          hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]) == hipSuccess && a > 0.f ? a : b;
+      (void)hipGetLastError();
     }
     {
       XrtkPlan& q = ctx->last_plan;

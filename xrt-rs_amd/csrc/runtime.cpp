@@ -501,7 +501,8 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
     return g;
   };
   ph.next("xrtk.pair");
-  if (timed) (void)hipEventRecord(ctx->ev[1], st);
+  // (external records: a captured graph records them at each replay)
+  if (timed) (void)hipEventRecordWithFlags(ctx->ev[1], st, hipEventRecordExternal);
   if (!items32.empty()) {
     const uint32_t remap = (!cfg32.persistent && n == 1) ? 1u : 0u;
     e = launch_pair(MODE_F32R, cfg32, d_insts, d_items32, (uint32_t)items32.size(), d_queue,
@@ -524,7 +525,7 @@ int run_instances(xrtk_ctx* ctx, const LaunchCfg& cfg32, const LaunchCfg& cfg64,
                     grid_of(cfg64, MODE_F64, items64.size()), remap, st);
     if (e != hipSuccess) return map_err(e);
   }
-  if (timed) (void)hipEventRecord(ctx->ev[2], st);
+  if (timed) (void)hipEventRecordWithFlags(ctx->ev[2], st, hipEventRecordExternal);
   // ---- merge
   ph.next("xrtk.merge");
   if (!first.empty()) {
