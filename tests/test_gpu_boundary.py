@@ -247,6 +247,24 @@ def test_no_graph_switch_gives_the_same_bits(monkeypatch):
         c.close()
 
 
+def test_graph_replays_time_their_pair_kernels(ctx):
+    """Replays of a captured batch record the pair-kernel events inside the
+    graph: every replay reports a pair time of its own, above zero and
+    within the replay's whole device time (and the same bits each time)."""
+    rays = W.point_source_aperture(65536, seed=62)
+    rs = xrtk.RaySet(ctx, rays)
+    try:
+        job = xrtk.Job(rs, 10000.0, grid(W.screen(1000.0, 0.02, 256)))
+        first = xrtk.solve_jobs(ctx, [job])[0]
+        for _ in range(4):
+            again = xrtk.solve_jobs(ctx, [job])[0]
+            pair_ms, solve_ms = ctx.last_timing()
+            assert 0.0 < pair_ms <= solve_ms + 1e-3, (pair_ms, solve_ms)
+            assert np.array_equal(again, first)
+    finally:
+        rs.release()
+
+
 # ---- beam_to_diffraction_rays on the device ---------------------------------------------
 def _beam(n, seed, amps=True):
     """A Beam's SoA columns after a curved optic: positions on a cap,
