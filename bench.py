@@ -50,6 +50,11 @@ import sys
 import threading
 import time
 
+# the CPU legs' OpenMP threads sleep between their parallel regions instead
+# of spinning into the GPU legs that follow them (whose host copies use the
+# same cores); set before the OpenMP runtime loads
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "xrt-rs_amd"), os.path.join(ROOT, "oracle")):
     if p not in sys.path:

@@ -407,9 +407,54 @@ void test_host_copies() {
   }
 }
 
+// scatter_chunks (results out of the page-locked block): segments with a
+// skipped one between them, ragged sizes around the chunk cuts, every byte
+// to its segment and nothing past it; a refused chunk stops the copy.
+void test_scatter() {
+  const size_t MiB = size_t(1) << 20;
+  const size_t ns[] = {1, 3, 1000, 131071, 131072, 655361, 1048577};
+  for (size_t n : ns) {
+    const size_t pad = ((n + 1) / 2) * 2 - n;
+    const size_t total = 8 * n + 8 * pad + 32 * n;
+    std::vector<char> src(total);
+    for (size_t i = 0; i < total; ++i) src[i] = (char)(i * 40503u >> 7);
+    std::vector<char> a(8 * n + 64, 0), b(16 * n + 64, 0), c(16 * n + 64, 0);
+    const xrtk::Segment segs[4] = {{a.data(), 8 * n}, {nullptr, 8 * pad}, {b.data(), 16 * n},
+                                   {c.data(), 16 * n}};
+    const size_t chunks[] = {4 * MiB, 1 * MiB + 3, 64};
+    for (size_t chunk : chunks) {
+      if (chunk < 4096 && n > 1000) continue;  // (many tiny chunks: small n only)
+      std::fill(a.begin(), a.end(), 0);
+      std::fill(b.begin(), b.end(), 0);
+      std::fill(c.begin(), c.end(), 0);
+      size_t calls = 0;
+      const bool ok = xrtk::scatter_chunks(src.data(), segs, 4, chunk, [&](size_t j) {
+        return j == calls++;
+      });
+      check(ok && calls == (total + chunk - 1) / chunk, "scatter_chunks waits for every chunk");
+      check(std::memcmp(a.data(), src.data(), 8 * n) == 0, "scatter_chunks segment 0");
+      check(std::memcmp(b.data(), src.data() + 8 * (n + pad), 16 * n) == 0,
+            "scatter_chunks segment 2");
+      check(std::memcmp(c.data(), src.data() + 8 * (n + pad) + 16 * n, 16 * n) == 0,
+            "scatter_chunks segment 3");
+      bool clean = true;
+      for (size_t i = 0; i < 64; ++i)
+        clean = clean && a[8 * n + i] == 0 && b[16 * n + i] == 0 && c[16 * n + i] == 0;
+      check(clean, "scatter_chunks bounds");
+    }
+    // refuse the first chunk: nothing is copied
+    std::fill(a.begin(), a.end(), 0);
+    const bool ok2 = xrtk::scatter_chunks(src.data(), segs, 4, 4096, [](size_t) { return false; });
+    bool none = !ok2;
+    for (size_t i = 0; i < 8 * n; ++i) none = none && a[i] == 0;
+    check(none, "scatter_chunks stops at a refused chunk");
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc > 1) return planted(argv[1]);
   test_host_copies();
+  test_scatter();
   test_prepare_wave();
   test_beam_to_rays();
   test_diffraction_f64();

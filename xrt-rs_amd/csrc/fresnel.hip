@@ -630,9 +630,23 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     const size_t total = o_u + sizeof(double2) * np;
     if (W.ensure(total) != hipSuccess) return XRTK_ENOMEM;
     char* base = W.as<char>();
-    std::vector<double> rxp(nr_pad, 0.0), rzp(nr_pad, 0.0), es(2 * nr_pad, 0.0);
-    std::memcpy(rxp.data(), ray_x, 8 * n_rays);
-    std::memcpy(rzp.data(), ray_z, 8 * n_rays);
+    // The inputs are laid out on the host as they lie on the device (pixel
+    // axes, ray x, ray z, scaled Es; zero padding past the rays) and go up
+    // in one copy, from the context's page-locked block when it can hold
+    // them (else from a pageable buffer); the result comes back through the
+    // same block.
+    char* pin = nullptr;
+    if (ctx->ws.hpin.ensure(std::max<size_t>(o_p, sizeof(double2) * np)) == hipSuccess)
+      pin = static_cast<char*>(ctx->ws.hpin.p);
+    std::vector<char> pageable;
+    if (!pin) pageable.resize(o_p);
+    char* hs = pin ? pin : pageable.data();
+    std::memset(hs, 0, o_p);
+    std::memcpy(hs + o_xp, xp.data(), 8 * (size_t)nx);
+    std::memcpy(hs + o_zp, zp.data(), 8 * (size_t)nz);
+    std::memcpy(hs + o_rx, ray_x, 8 * n_rays);
+    std::memcpy(hs + o_rz, ray_z, 8 * n_rays);
+    double* es = reinterpret_cast<double*>(hs + o_es);
     // FP16 parts: amplitudes scaled by 2^-ex so the largest |Re| + |Im| is
     // in [1/2, 1) (exact; the merge multiplies by 2^ex)
     double scale = 1.0, back = 1.0;
@@ -651,11 +665,7 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
       es[2 * j] = es_re[j] * scale;
       es[2 * j + 1] = es_im[j] * scale;
     }
-    hipError_t e = hipMemcpyAsync(base + o_xp, xp.data(), 8 * nx, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_zp, zp.data(), 8 * nz, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rx, rxp.data(), 8 * nr_pad, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_rz, rzp.data(), 8 * nr_pad, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_es, es.data(), 16 * nr_pad, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(base, hs, o_p, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return map_err(e);
     const double k = energy / kCHBAR_H * 1e7;
     const double krev = k / (2.0 * dist) / (2.0 * M_PI);
@@ -696,8 +706,8 @@ extern "C" int xrtk_fresnel_propagate(xrtk_ctx* ctx, const double* ray_x, const 
     if (e != hipSuccess) return map_err(e);
     (void)hipEventRecord(ctx->ev[3], st);
     ph.next("xrtk.readback");
-    e = hipMemcpyAsync(out2, U, sizeof(double2) * np, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const Segment seg{out2, sizeof(double2) * np};
+    e = copy_d2h(st, pin, U, &seg, 1);
     if (e != hipSuccess) return map_err(e);
     float ms = 0.f, kms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) != hipSuccess) ms = 0.f;

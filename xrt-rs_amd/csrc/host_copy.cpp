@@ -4,8 +4,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -83,6 +85,74 @@ bool gather_chunks(char* dst, const char* src, size_t bytes, size_t chunk,
     while (done[c].load(std::memory_order_acquire) < T) std::this_thread::yield();
     if (ok) ok = on_chunk(c * chunk, std::min(chunk, bytes - c * chunk));
   }
+  for (auto& x : th) x.join();
+  return ok;
+}
+
+bool scatter_chunks(const char* src, const Segment* segs, size_t n_segs, size_t chunk,
+                    const std::function<bool(size_t)>& wait_chunk) {
+  size_t bytes = 0;
+  for (size_t k = 0; k < n_segs; ++k) bytes += segs[k].len;
+  if (!bytes) return true;
+  chunk = std::max<size_t>(chunk, 64);
+  const size_t C = (bytes + chunk - 1) / chunk;
+  const size_t T = n_threads(bytes);
+  // bytes [a, b) of the whole to their segments
+  auto copy_range = [&](size_t a, size_t b) {
+    size_t off = 0;
+    for (size_t k = 0; k < n_segs && off < b; off += segs[k].len, ++k) {
+      const size_t lo = std::max(a, off), hi = std::min(b, off + segs[k].len);
+      if (lo < hi && segs[k].dst)
+        std::memcpy(static_cast<char*>(segs[k].dst) + (lo - off), src + lo, hi - lo);
+    }
+  };
+  auto share = [&](size_t t, size_t c) {
+    const size_t c0 = c * chunk, n = std::min(chunk, bytes - c0);
+    size_t a, b;
+    part_range(n, t, T, a, b);
+    copy_range(c0 + a, c0 + b);
+  };
+  // the workers sleep until the calling thread has released a chunk (the
+  // chunks arrive over PCIe; spinning would take cores from the caller's
+  // own threads meanwhile)
+  std::mutex mu;
+  std::condition_variable cv;
+  size_t released = 0;  // chunks [0, released) may be copied
+  bool stop = false;
+  std::vector<std::thread> th;
+  size_t started = 1;
+  try {
+    th.reserve(T);
+    for (; started < T; ++started)
+      th.emplace_back([&, t = started]() {
+        for (size_t c = 0; c < C; ++c) {
+          {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || released > c; });
+            if (released <= c) return;
+          }
+          share(t, c);
+        }
+      });
+  } catch (...) {
+  }
+  bool ok = true;
+  for (size_t c = 0; c < C; ++c) {
+    ok = wait_chunk(c);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (ok) released = c + 1; else stop = true;
+    }
+    cv.notify_all();
+    if (!ok) break;
+    share(0, c);
+    for (size_t t = started; t < T; ++t) share(t, c);
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    stop = true;
+  }
+  cv.notify_all();
   for (auto& x : th) x.join();
   return ok;
 }

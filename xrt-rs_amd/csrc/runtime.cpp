@@ -44,6 +44,44 @@ hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_
   return e;
 }
 
+hipError_t copy_d2h(hipStream_t st, char* pin, const void* src, const Segment* segs,
+                    size_t n_segs) {
+  size_t bytes = 0;
+  for (size_t k = 0; k < n_segs; ++k) bytes += segs[k].len;
+  hipError_t e = hipSuccess;
+  if (!pin || bytes < (size_t(4) << 20)) {
+    size_t off = 0;
+    for (size_t k = 0; k < n_segs && e == hipSuccess; off += segs[k].len, ++k)
+      if (segs[k].dst && segs[k].len)
+        e = hipMemcpyAsync(segs[k].dst, static_cast<const char*>(src) + off, segs[k].len,
+                           hipMemcpyDeviceToHost, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+  }
+  // chunks of 4 MiB into the block, each with an event; the host copies a
+  // chunk out as soon as its event has fired, while later ones cross PCIe
+  const size_t kChunk = size_t(4) << 20;
+  const size_t C = (bytes + kChunk - 1) / kChunk;
+  std::vector<hipEvent_t> ev(C, nullptr);
+  for (size_t c = 0; c < C && e == hipSuccess; ++c)
+    e = hipEventCreateWithFlags(&ev[c], hipEventDisableTiming);
+  for (size_t c = 0; c < C && e == hipSuccess; ++c) {
+    const size_t c0 = c * kChunk;
+    e = hipMemcpyAsync(pin + c0, static_cast<const char*>(src) + c0,
+                       std::min(kChunk, bytes - c0), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[c], st);
+  }
+  if (e == hipSuccess)
+    scatter_chunks(pin, segs, n_segs, kChunk, [&](size_t c) {
+      e = hipEventSynchronize(ev[c]);
+      return e == hipSuccess;
+    });
+  // (every queued copy has run before the block is handed back)
+  const hipError_t es = hipStreamSynchronize(st);
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return e != hipSuccess ? e : es;
+}
+
 hipError_t HBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return hipSuccess;
   release();

@@ -113,6 +113,13 @@ struct HBuf {
 // the rate of a copy from pageable memory; the block must stay as it is
 // until the stream has run the copy.
 hipError_t copy_h2d(hipStream_t st, char* pin, void* dst, const void* src, size_t bytes);
+// Device memory to host memory on `st`, synchronously: the bytes of `src`
+// (the segments back to back) to the segments' destinations.  With `pin` (a
+// page-locked block at least as large, or null) a large copy lands in the
+// block in 4 MiB chunks and a few host threads copy each chunk out as soon
+// as it has arrived.
+hipError_t copy_d2h(hipStream_t st, char* pin, const void* src, const Segment* segs,
+                    size_t n_segs);
 
 // Device buffers of one-shot solves: staged inputs, resident-form rays and
 // slots, per-instance work (FP32 records, tiles, partial sums), launch

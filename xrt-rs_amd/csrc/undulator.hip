@@ -173,7 +173,7 @@ void build_tables(const XrtkUndulator& u, UndConst* K, std::vector<double>& tab)
 
 // Runs the map on columns already on the device (d_e, d_th, d_ps); results
 // copied to the host buffers, through `pin` (a page-locked block of at least
-// 5 n doubles, or null) when they are large.
+// 5 n + 1 doubles, or null) when they are large.
 int run_map(xrtk_ctx* ctx, const XrtkUndulator& u, size_t n, const double* d_e,
             const double* d_th, const double* d_ps, char* work, double* inten, double* amp_s2,
             double* amp_p2, char* pin = nullptr) {
@@ -194,22 +194,14 @@ int run_map(xrtk_ctx* ctx, const XrtkUndulator& u, size_t n, const double* d_e,
   e = hipGetLastError();
   if (e != hipSuccess) return map_err(e);
   (void)hipEventRecord(ctx->ev[3], st);
-  const bool via_pin = pin && sizeof(double) * 5 * n >= (size_t(4) << 20);
-  double* h_int = via_pin ? reinterpret_cast<double*>(pin) : inten;
-  double* h_s = via_pin ? h_int + n : amp_s2;
-  double* h_p = via_pin ? h_int + 3 * n : amp_p2;
-  e = hipMemcpyAsync(h_int, d_int, sizeof(double) * n, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h_s, d_s, sizeof(double2) * n, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h_p, d_p, sizeof(double2) * n, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  // the three results lie back to back on the device (intensity padded to
+  // an even count)
+  const Segment segs[4] = {{inten, sizeof(double) * n},
+                           {nullptr, sizeof(double) * (((n + 1) / 2) * 2 - n)},
+                           {amp_s2, sizeof(double2) * n},
+                           {amp_p2, sizeof(double2) * n}};
+  e = copy_d2h(st, pin, d_int, segs, 4);
   if (e != hipSuccess) return map_err(e);
-  if (via_pin) {
-    par_memcpy(inten, h_int, sizeof(double) * n);
-    par_memcpy(amp_s2, h_s, sizeof(double2) * n);
-    par_memcpy(amp_p2, h_p, sizeof(double2) * n);
-  }
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) != hipSuccess) ms = 0.f;
   ctx->last_pair_ms = ms;
@@ -250,9 +242,10 @@ extern "C" int xrtk_undulator_i_map(xrtk_ctx* ctx, const XrtkUndulator* und,
     double* d_ps = d_th + n;
     hipStream_t st = ctx->stream;
     // inputs and results through the context's page-locked block (3 n then
-    // 5 n doubles; the map's synchronisation frees it for the next call)
+    // 5 n (+ 1 pad) doubles; the map's synchronisation frees it for the next
+    // call)
     char* pin = nullptr;
-    if (ctx->ws.hpin.ensure(sizeof(double) * 5 * n) == hipSuccess)
+    if (ctx->ws.hpin.ensure(sizeof(double) * (5 * n + 1)) == hipSuccess)
       pin = static_cast<char*>(ctx->ws.hpin.p);
     hipError_t e = copy_h2d(st, pin, d_e, energies, 8 * n);
     if (e == hipSuccess) e = copy_h2d(st, pin ? pin + 8 * n : nullptr, d_th, thetas, 8 * n);
