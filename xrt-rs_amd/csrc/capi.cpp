@@ -642,19 +642,16 @@ int solve_oneshot(xrtk_ctx* ctx, Workspace& ws, const RaySrc& rs, const PixSrc& 
                          false, true, nullptr, /*route_sync=*/true);
   if (rc != XRTK_OK) return rc;
   ph.next("xrtk.readback");
-  // (a large result comes back through the page-locked block, which the
-  // inputs no longer need once the stream has run this far; the variant
-  // flags into the block's tail)
-  const bool via_pin = pin && ob >= (size_t(4) << 20);
-  e = hipMemcpyAsync(via_pin ? static_cast<void*>(pin) : static_cast<void*>(out4), ws.outbuf.p,
-                     ob, hipMemcpyDeviceToHost, ctx->stream);
+  // (the variant flags into the page-locked block's tail, then a large
+  // result through the block, which the inputs no longer need once the
+  // stream has run this far; copy_d2h waits for the stream)
   uint32_t* fl_pin = pin ? reinterpret_cast<uint32_t*>(pin + pin_tail) : nullptr;
-  if (e == hipSuccess && plan.scr0 && fl_pin)
+  if (plan.scr0 && fl_pin)
     e = hipMemcpyAsync(fl_pin, &plan.scr0->flags, sizeof(uint32_t), hipMemcpyDeviceToHost,
                        ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  const Segment res{out4, ob};
+  if (e == hipSuccess) e = copy_d2h(ctx->stream, pin, ws.outbuf.p, &res, 1);
   if (e != hipSuccess) return map_err(e);
-  if (via_pin) par_memcpy(out4, pin, ob);
   record_timing(ctx);
   record_plan(ctx, plan, 1);
   if (plan.scr0) {
